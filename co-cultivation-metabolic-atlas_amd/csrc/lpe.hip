@@ -1,0 +1,1337 @@
+// Host runtime and C ABI of the MI355X batched LP engine (see include/lpe.h).
+//
+// One handle = one shared-pattern LP batch resident in HBM.  The handle owns a
+// private non-blocking HIP stream (default priority) that waits on the
+// caller's stream at entry and is waited on by it at exit, so callers may pass
+// torch's current stream while PDHG chunks are captured into hipGraphs on the
+// private stream.
+#include "lpe.h"
+#include "lpe_device.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace lpe;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(LPE_EDEVICE, std::string(#expr) + " failed: " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define CHK(expr)                        \
+    do {                                 \
+        int rc_ = (expr);                \
+        if (rc_ != LPE_OK) return rc_;   \
+    } while (0)
+
+constexpr int ROWS_PER_BLOCK_EW = 16;   // rows per block of elementwise kernels
+
+inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+Shape make_shape(int rows, int ld, int batch, int rpw, int lpl, int* grid) {
+    Shape s;
+    s.rows = rows;
+    s.ld = ld;
+    s.batch = batch;
+    s.rpw = rpw;
+    const int W = WAVE * lpl;
+    s.n_tiles = ceil_div(ld, W);
+    s.n_rb = std::max(1, ceil_div(rows, WAVES_PER_BLOCK * rpw));
+    s.n_real = s.n_tiles * s.n_rb;
+    *grid = ((s.n_real + 7) / 8) * 8;
+    return s;
+}
+
+template <typename T>
+struct Iter {
+    T* x = nullptr; T* xbar = nullptr; T* y = nullptr;
+    T* xsum = nullptr; T* ysum = nullptr; T* xlast = nullptr; T* ylast = nullptr;
+    T* c = nullptr; T* rhs = nullptr; T* lo = nullptr; T* up = nullptr;
+    T* val = nullptr; T* tval = nullptr; T* var = nullptr;
+    T* sigma = nullptr; T* tau = nullptr;
+};
+
+}  // namespace
+
+struct lpe_handle {
+    int m = 0, n = 0, neq = 0, B = 0, ld = 0;
+    long nnz = 0;
+    int nvar = 0;
+    int precision = LPE_PREC_FP64, method = LPE_METHOD_RESTARTED;
+    double eta = 0.99, theta = 1.0, pw = 1.0;
+    int ruiz = 10, shared_c = 0;
+    int dev = 0;
+
+    // host structure
+    std::vector<int> rp, ci, cp, ri, rslot, cslot, var_row, var_col;
+    std::vector<double> val;          // template values (CSR order)
+    std::vector<long> csr_to_csc;     // position in CSC of CSR entry k
+    std::vector<double> dr, dc, sig, tau;
+    std::vector<double> var_absmax;   // [nvar] max |scaled value| over the batch
+
+    // device structure
+    int *d_rp = nullptr, *d_ci = nullptr, *d_rslot = nullptr;
+    int *d_cp = nullptr, *d_ri = nullptr, *d_cslot = nullptr;
+    int *d_vrow = nullptr, *d_vcol = nullptr;
+    double *d_val_o = nullptr, *d_tval_o = nullptr, *d_val_s = nullptr, *d_tval_s = nullptr;
+    float *d_val_f = nullptr, *d_tval_f = nullptr;
+    double *d_var_o = nullptr, *d_var_s = nullptr;
+    float* d_var_f = nullptr;
+    double *d_dr = nullptr, *d_dc = nullptr, *d_idr = nullptr, *d_idc = nullptr;
+    double *d_sig = nullptr, *d_tau = nullptr;
+    float *d_sig_f = nullptr, *d_tau_f = nullptr;
+    double *d_sigref = nullptr, *d_tauref = nullptr;
+
+    // problem data
+    double *d_c_o = nullptr, *d_rhs_o = nullptr, *d_lo_o = nullptr, *d_up_o = nullptr;
+    double *d_c_s = nullptr, *d_rhs_s = nullptr, *d_lo_s = nullptr, *d_up_s = nullptr;
+    float *d_c_f = nullptr, *d_rhs_f = nullptr, *d_lo_f = nullptr, *d_up_f = nullptr;
+
+    // iterates
+    Iter<double> it64;
+    Iter<float> it32;
+
+    // per-LP state
+    LpState* d_st = nullptr;
+    int *d_flags = nullptr, *d_cnt = nullptr, *d_source = nullptr, *d_sel = nullptr, *d_passed = nullptr;
+    double *d_om = nullptr, *d_iom = nullptr, *d_bnorm = nullptr, *d_cnorm = nullptr;
+    float *d_om_f = nullptr, *d_iom_f = nullptr;
+
+    // reductions
+    double* d_partial = nullptr;
+    size_t partial_elems = 0;
+    double *d_red_a = nullptr, *d_red_b = nullptr;
+    int *d_ops_cr = nullptr, *d_ops_cc = nullptr, *d_ops_xr = nullptr, *d_ops_xc = nullptr, *d_ops_sum = nullptr;
+
+    // polish / results
+    double *d_xs = nullptr, *d_rp_ = nullptr, *d_pp = nullptr, *d_wp = nullptr;
+    double *d_rd = nullptr, *d_pd = nullptr, *d_wd = nullptr;
+    double *d_q1 = nullptr, *d_q2 = nullptr, *d_s1 = nullptr, *d_s2 = nullptr;
+    uint8_t *d_fmask = nullptr, *d_rmask = nullptr;
+    double *d_rho = nullptr, *d_alpha = nullptr, *d_beta = nullptr;
+    double *d_xpol = nullptr, *d_ypol = nullptr, *d_xres = nullptr, *d_yres = nullptr;
+    double* d_metrics = nullptr;
+    double *d_xin = nullptr, *d_yin = nullptr;    // user iterates for lpe_certify / lpe_spmv
+    int* d_err = nullptr;
+
+    // host mirrors (pinned)
+    LpState* h_st = nullptr;
+    int* h_sel = nullptr;
+    int* h_passed = nullptr;
+    int* h_err = nullptr;
+
+    hipStream_t s = nullptr;
+    hipEvent_t ev_sync = nullptr, ev_a = nullptr, ev_b = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int graph_K = -1, graph_rpw = -1, graph_lpl = -1;
+
+    int rpw = 1, lpl = 1;
+    bool have_vectors = false;
+    bool ready = false;
+    std::vector<void*> allocations;
+};
+
+namespace {
+
+template <typename T>
+int dalloc(lpe_handle* h, T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T));
+    if (e != hipSuccess) return fail(LPE_EDEVICE, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    h->allocations.push_back(q);
+    *p = static_cast<T*>(q);
+    hipError_t z = hipMemset(q, 0, count * sizeof(T));
+    if (z != hipSuccess) return fail(LPE_EDEVICE, "hipMemset failed");
+    return LPE_OK;
+}
+
+template <typename T>
+int upload(T* d, const T* hsrc, size_t count) {
+    if (count == 0) return LPE_OK;
+    HIPCHK(hipMemcpy(d, hsrc, count * sizeof(T), hipMemcpyHostToDevice));
+    return LPE_OK;
+}
+
+// Caller stream -> engine stream ordering.
+int enter(lpe_handle* h, void* stream) {
+    HIPCHK(hipSetDevice(h->dev));
+    hipStream_t cs = (hipStream_t)stream;
+    HIPCHK(hipEventRecord(h->ev_sync, cs));
+    HIPCHK(hipStreamWaitEvent(h->s, h->ev_sync, 0));
+    return LPE_OK;
+}
+
+int leave(lpe_handle* h, void* stream) {
+    hipStream_t cs = (hipStream_t)stream;
+    HIPCHK(hipEventRecord(h->ev_sync, h->s));
+    HIPCHK(hipStreamWaitEvent(cs, h->ev_sync, 0));
+    HIPCHK(hipGetLastError());
+    return LPE_OK;
+}
+
+dim3 ew_grid(int rows, int ld) { return dim3(ceil_div(ld, 256), std::max(1, ceil_div(rows, ROWS_PER_BLOCK_EW))); }
+
+// ---------------------------------------------------------------------------
+// Metric, scaling
+// ---------------------------------------------------------------------------
+void ruiz_equilibrate(lpe_handle* h) {
+    const int m = h->m, n = h->n;
+    h->dr.assign(m, 1.0);
+    h->dc.assign(n, 1.0);
+    if (h->method != LPE_METHOD_RESTARTED) return;
+    std::vector<double> rmax(m), cmax(n);
+    for (int it = 0; it < h->ruiz; ++it) {
+        std::fill(rmax.begin(), rmax.end(), 0.0);
+        std::fill(cmax.begin(), cmax.end(), 0.0);
+        for (int i = 0; i < m; ++i)
+            for (int k = h->rp[i]; k < h->rp[i + 1]; ++k) {
+                const double v = std::fabs(h->val[k] * h->dr[i] * h->dc[h->ci[k]]);
+                rmax[i] = std::max(rmax[i], v);
+                cmax[h->ci[k]] = std::max(cmax[h->ci[k]], v);
+            }
+        for (int i = 0; i < m; ++i)
+            if (rmax[i] > 0.0) h->dr[i] /= std::sqrt(rmax[i]);
+        for (int j = 0; j < n; ++j)
+            if (cmax[j] > 0.0) h->dc[j] /= std::sqrt(cmax[j]);
+    }
+}
+
+// Shared diagonal metric of the equilibrated matrix (Pock-Chambolle, alpha=1):
+// sigma_i = eta / sum_j |a^_ij|, tau_j = eta / sum_i |a^_ij|; per-LP entries use
+// the largest magnitude over the batch.
+void shared_metric(lpe_handle* h) {
+    const int m = h->m, n = h->n;
+    std::vector<double> rs(m, 0.0), cs(n, 0.0);
+    for (int i = 0; i < m; ++i)
+        for (int k = h->rp[i]; k < h->rp[i + 1]; ++k) {
+            const int sl = h->rslot[k];
+            const double v = sl < 0 ? std::fabs(h->val[k] * h->dr[i] * h->dc[h->ci[k]]) : h->var_absmax[sl];
+            rs[i] += v;
+            cs[h->ci[k]] += v;
+        }
+    h->sig.resize(m);
+    h->tau.resize(n);
+    for (int i = 0; i < m; ++i) h->sig[i] = rs[i] > 0.0 ? h->eta / rs[i] : 1.0;
+    for (int j = 0; j < n; ++j) h->tau[j] = cs[j] > 0.0 ? h->eta / cs[j] : 1.0;
+}
+
+int upload_metric(lpe_handle* h) {
+    CHK(upload(h->d_sig, h->sig.data(), h->m));
+    CHK(upload(h->d_tau, h->tau.data(), h->n));
+    std::vector<float> f(std::max(h->m, h->n));
+    for (int i = 0; i < h->m; ++i) f[i] = (float)h->sig[i];
+    CHK(upload(h->d_sig_f, f.data(), h->m));
+    for (int j = 0; j < h->n; ++j) f[j] = (float)h->tau[j];
+    CHK(upload(h->d_tau_f, f.data(), h->n));
+    return LPE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Launch helpers
+// ---------------------------------------------------------------------------
+Op<double> op_orig(lpe_handle* h, bool transpose) {
+    Op<double> o;
+    o.ptr = transpose ? h->d_cp : h->d_rp;
+    o.idx = transpose ? h->d_ri : h->d_ci;
+    o.slot = transpose ? h->d_cslot : h->d_rslot;
+    o.val = transpose ? h->d_tval_o : h->d_val_o;
+    o.var = h->d_var_o;
+    return o;
+}
+
+Op<double> op_scaled(lpe_handle* h, bool transpose) {
+    Op<double> o = op_orig(h, transpose);
+    o.val = transpose ? h->d_tval_s : h->d_val_s;
+    o.var = h->d_var_s;
+    return o;
+}
+
+Op<float> op_f32(lpe_handle* h, bool transpose) {
+    Op<float> o;
+    o.ptr = transpose ? h->d_cp : h->d_rp;
+    o.idx = transpose ? h->d_ri : h->d_ci;
+    o.slot = transpose ? h->d_cslot : h->d_rslot;
+    o.val = transpose ? h->d_tval_f : h->d_val_f;
+    o.var = h->d_var_f;
+    return o;
+}
+
+template <typename T>
+Op<T> op_iter(lpe_handle* h, bool transpose);
+template <>
+Op<double> op_iter<double>(lpe_handle* h, bool transpose) { return op_scaled(h, transpose); }
+template <>
+Op<float> op_iter<float>(lpe_handle* h, bool transpose) { return op_f32(h, transpose); }
+
+template <typename T>
+Iter<T>& iter_of(lpe_handle* h);
+template <>
+Iter<double>& iter_of<double>(lpe_handle* h) { return h->it64; }
+template <>
+Iter<float>& iter_of<float>(lpe_handle* h) { return h->it32; }
+
+template <typename T, int LPL, bool REF>
+void launch_step_lpl(lpe_handle* h, int which) {
+    Iter<T>& it = iter_of<T>(h);
+    if (h->m > 0 && (which & 1)) {
+        int grid;
+        DualArgs<T> d;
+        d.sh = make_shape(h->m, h->ld, h->B, h->rpw, LPL, &grid);
+        d.A = op_iter<T>(h, false);
+        d.xbar = it.xbar; d.y = it.y; d.rhs = it.rhs;
+        d.sigma = REF ? (const T*)h->d_sigref : it.sigma;
+        d.omega = nullptr;
+        if constexpr (std::is_same<T, double>::value) d.omega = h->d_om; else d.omega = h->d_om_f;
+        d.flags = h->d_flags; d.ysum = it.ysum; d.neq = h->neq;
+        hipLaunchKernelGGL((k_dual_step<T, LPL, REF>), dim3(grid), dim3(BLOCK), 0, h->s, d);
+    }
+    if (!(which & 2)) return;
+    int grid;
+    PrimalArgs<T> p;
+    p.sh = make_shape(h->n, h->ld, h->B, h->rpw, LPL, &grid);
+    p.AT = op_iter<T>(h, true);
+    p.y = it.y; p.x = it.x; p.xbar = it.xbar; p.c = it.c; p.lower = it.lo; p.upper = it.up;
+    p.tau = REF ? (const T*)h->d_tauref : it.tau;
+    if constexpr (std::is_same<T, double>::value) p.inv_omega = h->d_iom; else p.inv_omega = h->d_iom_f;
+    p.flags = h->d_flags; p.xsum = it.xsum; p.theta = (T)h->theta; p.c_shared = h->shared_c;
+    hipLaunchKernelGGL((k_primal_step<T, LPL, REF>), dim3(grid), dim3(BLOCK), 0, h->s, p);
+}
+
+template <typename T, bool REF>
+void launch_step_t(lpe_handle* h, int which) {
+    switch (h->lpl) {
+        case 4: launch_step_lpl<T, 4, REF>(h, which); break;
+        case 2: launch_step_lpl<T, 2, REF>(h, which); break;
+        default: launch_step_lpl<T, 1, REF>(h, which); break;
+    }
+}
+
+// which: 1 = dual half step, 2 = primal half step, 3 = full iteration
+void launch_step(lpe_handle* h, int which = 3) {
+    if (h->method == LPE_METHOD_FIXED) launch_step_t<double, true>(h, which);
+    else if (h->precision == LPE_PREC_FP32) launch_step_t<float, false>(h, which);
+    else launch_step_t<double, false>(h, which);
+}
+
+int kernel_launches_per_iteration(lpe_handle* h) { return h->m > 0 ? 2 : 1; }
+
+int run_iterations(lpe_handle* h, long iters, int chunk, int use_graph) {
+    if (iters <= 0) return LPE_OK;
+    if (chunk < 1) chunk = 64;
+    long chunks = use_graph ? iters / chunk : 0;
+    long tail = iters - chunks * chunk;
+    if (chunks > 0) {
+        if (!h->graph || h->graph_K != chunk || h->graph_rpw != h->rpw || h->graph_lpl != h->lpl) {
+            if (h->graph) { hipGraphExecDestroy(h->graph); h->graph = nullptr; }
+            hipGraph_t g;
+            HIPCHK(hipStreamBeginCapture(h->s, hipStreamCaptureModeThreadLocal));
+            for (int i = 0; i < chunk; ++i) launch_step(h);
+            HIPCHK(hipStreamEndCapture(h->s, &g));
+            hipError_t e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            if (e != hipSuccess) { h->graph = nullptr; return fail(LPE_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)); }
+            h->graph_K = chunk; h->graph_rpw = h->rpw; h->graph_lpl = h->lpl;
+        }
+        for (long c = 0; c < chunks; ++c) HIPCHK(hipGraphLaunch(h->graph, h->s));
+    }
+    for (long i = 0; i < tail; ++i) launch_step(h);
+    HIPCHK(hipGetLastError());
+    return LPE_OK;
+}
+
+void reduce(lpe_handle* h, const double* partial, double* out, const int* ops, int nv, int n_rb) {
+    hipLaunchKernelGGL(k_reduce_partials, dim3(ceil_div(h->ld, 256), nv), dim3(256), 0, h->s, partial, out, ops,
+                       nv, n_rb, h->ld);
+}
+
+constexpr int CHECK_RPW = 4;
+constexpr int POLISH_RPW = 4;
+
+// Device check: KKT / certificate values of (current, average), decisions, actions.
+template <typename T>
+int launch_check_t(lpe_handle* h, int K, int certify, double ep, double ed, double eg) {
+    Iter<T>& it = iter_of<T>(h);
+    const bool restarted = h->method == LPE_METHOD_RESTARTED && K > 0;
+    int grid_r = 0, grid_c = 0;
+    Shape sr = make_shape(std::max(h->m, 1), h->ld, h->B, CHECK_RPW, 1, &grid_r);
+    Shape sc = make_shape(h->n, h->ld, h->B, CHECK_RPW, 1, &grid_c);
+    if (h->m > 0) {
+        CheckRowsArgs<T> r;
+        r.sh = sr; r.A = op_scaled(h, false);
+        r.x = it.x; r.xsum = restarted ? it.xsum : it.x; r.y = it.y; r.ysum = restarted ? it.ysum : it.y;
+        r.ylast = restarted ? it.ylast : it.y;
+        r.rhs = h->d_rhs_s; r.dr = h->d_dr; r.flags = h->d_flags; r.cnt = h->d_cnt; r.K = K; r.neq = h->neq;
+        r.with_avg = restarted ? 1 : 0; r.partial = h->d_partial;
+        hipLaunchKernelGGL((k_check_rows<T>), dim3(grid_r), dim3(BLOCK), 0, h->s, r);
+        reduce(h, h->d_partial, h->d_red_a, h->d_ops_cr, 2 * CR_N, sr.n_rb);
+    } else {
+        HIPCHK(hipMemsetAsync(h->d_red_a, 0, sizeof(double) * 2 * CR_N * h->ld, h->s));
+    }
+    CheckColsArgs<T> c;
+    c.sh = sc; c.AT = op_scaled(h, true);
+    c.x = it.x; c.xsum = restarted ? it.xsum : it.x; c.xlast = restarted ? it.xlast : it.x;
+    c.y = it.y; c.ysum = restarted ? it.ysum : it.y;
+    c.c = h->d_c_s; c.lower = h->d_lo_s; c.upper = h->d_up_s; c.dc = h->d_dc; c.flags = h->d_flags;
+    c.cnt = h->d_cnt; c.K = K; c.c_shared = h->shared_c; c.with_avg = restarted ? 1 : 0;
+    c.partial = h->d_partial;
+    hipLaunchKernelGGL((k_check_cols<T>), dim3(grid_c), dim3(BLOCK), 0, h->s, c);
+    reduce(h, h->d_partial, h->d_red_b, h->d_ops_cc, 2 * CC_N, sc.n_rb);
+    DecideArgs d;
+    d.rows = h->d_red_a; d.cols = h->d_red_b; d.st = h->d_st; d.flags = h->d_flags; d.cnt = h->d_cnt;
+    d.omega64 = h->d_om; d.omega32 = h->d_om_f; d.inv_omega64 = h->d_iom; d.inv_omega32 = h->d_iom_f;
+    d.bnorm = h->d_bnorm; d.cnorm = h->d_cnorm; d.batch = h->B; d.ld = h->ld; d.K = K;
+    d.restarted = restarted ? 1 : 0; d.certify = certify; d.eps_p = ep; d.eps_d = ed; d.eps_g = eg;
+    hipLaunchKernelGGL(k_check_decide, dim3(ceil_div(h->B, 256)), dim3(256), 0, h->s, d);
+    if (h->method == LPE_METHOD_RESTARTED) {
+        ApplyArgs<T> ax;
+        ax.rows = h->n; ax.ld = h->ld; ax.rows_per_block = ROWS_PER_BLOCK_EW; ax.st = h->d_st;
+        ax.v = it.x; ax.vsum = it.xsum; ax.vlast = it.xlast; ax.vbar = it.xbar;
+        hipLaunchKernelGGL((k_check_apply<T>), ew_grid(h->n, h->ld), dim3(256), 0, h->s, ax);
+        if (h->m > 0) {
+            ApplyArgs<T> ay;
+            ay.rows = h->m; ay.ld = h->ld; ay.rows_per_block = ROWS_PER_BLOCK_EW; ay.st = h->d_st;
+            ay.v = it.y; ay.vsum = it.ysum; ay.vlast = it.ylast; ay.vbar = nullptr;
+            hipLaunchKernelGGL((k_check_apply<T>), ew_grid(h->m, h->ld), dim3(256), 0, h->s, ay);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    return LPE_OK;
+}
+
+int launch_check(lpe_handle* h, int K, int certify, double ep, double ed, double eg) {
+    if (h->method != LPE_METHOD_FIXED && h->precision == LPE_PREC_FP32)
+        return launch_check_t<float>(h, K, certify, ep, ed, eg);
+    return launch_check_t<double>(h, K, certify, ep, ed, eg);
+}
+
+// FP64 certificate of internal original-unit (x, y) into d_metrics / d_passed.
+int launch_certificate(lpe_handle* h, const double* x, const double* y, double ep, double ed, double eg) {
+    int grid_r = 0, grid_c = 0;
+    Shape sr = make_shape(std::max(h->m, 1), h->ld, h->B, CHECK_RPW, 1, &grid_r);
+    Shape sc = make_shape(h->n, h->ld, h->B, CHECK_RPW, 1, &grid_c);
+    if (h->m > 0) {
+        CertRowsArgs r;
+        r.sh = sr; r.A = op_orig(h, false); r.x = x; r.y = y; r.rhs = h->d_rhs_o; r.neq = h->neq;
+        r.partial = h->d_partial;
+        hipLaunchKernelGGL(k_cert_rows, dim3(grid_r), dim3(BLOCK), 0, h->s, r);
+        reduce(h, h->d_partial, h->d_red_a, h->d_ops_xr, XR_N, sr.n_rb);
+    }
+    CertColsArgs c;
+    c.sh = sc; c.AT = op_orig(h, true); c.x = x; c.y = y; c.c = h->d_c_o; c.lower = h->d_lo_o;
+    c.upper = h->d_up_o; c.c_shared = h->shared_c; c.partial = h->d_partial;
+    hipLaunchKernelGGL(k_cert_cols, dim3(grid_c), dim3(BLOCK), 0, h->s, c);
+    reduce(h, h->d_partial, h->d_red_b, h->d_ops_xc, XC_N, sc.n_rb);
+    CertFinalArgs f;
+    f.rows = h->m > 0 ? h->d_red_a : nullptr; f.cols = h->d_red_b; f.metrics = h->d_metrics;
+    f.passed = h->d_passed; f.batch = h->B; f.ld = h->ld; f.eps_p = ep; f.eps_d = ed; f.eps_g = eg;
+    hipLaunchKernelGGL(k_cert_final, dim3(ceil_div(h->B, 256)), dim3(256), 0, h->s, f);
+    HIPCHK(hipGetLastError());
+    return LPE_OK;
+}
+
+// Original-unit FP64 copy of the current iterate into (xo, yo).
+int iterate_to_original(lpe_handle* h, double* xo, double* yo) {
+    if (h->method != LPE_METHOD_FIXED && h->precision == LPE_PREC_FP32) {
+        hipLaunchKernelGGL((k_scale_copy<float>), ew_grid(h->n, h->ld), dim3(256), 0, h->s, h->it32.x, xo, h->n,
+                           h->ld, ROWS_PER_BLOCK_EW, h->d_dc, (const int*)nullptr, 0);
+        if (h->m > 0)
+            hipLaunchKernelGGL((k_scale_copy<float>), ew_grid(h->m, h->ld), dim3(256), 0, h->s, h->it32.y, yo, h->m,
+                               h->ld, ROWS_PER_BLOCK_EW, h->d_dr, (const int*)nullptr, 0);
+    } else {
+        hipLaunchKernelGGL((k_scale_copy<double>), ew_grid(h->n, h->ld), dim3(256), 0, h->s, h->it64.x, xo, h->n,
+                           h->ld, ROWS_PER_BLOCK_EW, h->d_dc, (const int*)nullptr, 0);
+        if (h->m > 0)
+            hipLaunchKernelGGL((k_scale_copy<double>), ew_grid(h->m, h->ld), dim3(256), 0, h->s, h->it64.y, yo, h->m,
+                               h->ld, ROWS_PER_BLOCK_EW, h->d_dr, (const int*)nullptr, 0);
+    }
+    HIPCHK(hipGetLastError());
+    return LPE_OK;
+}
+
+// Build the result buffers: polished LPs keep their polish result, the others
+// take the current iterate.
+int build_results(lpe_handle* h) {
+    CHK(iterate_to_original(h, h->d_xin, h->d_yin));
+    hipLaunchKernelGGL(k_masked_copy, ew_grid(h->n, h->ld), dim3(256), 0, h->s, (const double*)h->d_xin, h->d_xres,
+                       h->n, h->ld, ROWS_PER_BLOCK_EW, (const int*)h->d_source, 0);
+    if (h->m > 0)
+        hipLaunchKernelGGL(k_masked_copy, ew_grid(h->m, h->ld), dim3(256), 0, h->s, (const double*)h->d_yin,
+                           h->d_yres, h->m, h->ld, ROWS_PER_BLOCK_EW, (const int*)h->d_source, 0);
+    HIPCHK(hipGetLastError());
+    return LPE_OK;
+}
+
+// One polish attempt for the LPs in h->d_sel (device, already uploaded).
+template <typename T>
+int polish_t(lpe_handle* h, int cg_iters, double tol) {
+    Iter<T>& it = iter_of<T>(h);
+    const long ld = h->ld;
+    int grid_r = 0, grid_c = 0;
+    Shape sr = make_shape(std::max(h->m, 1), h->ld, h->B, POLISH_RPW, 1, &grid_r);
+    Shape sc = make_shape(h->n, h->ld, h->B, POLISH_RPW, 1, &grid_c);
+    {
+        PolishColsInitArgs<T> a;
+        a.sh = sc; a.AT = op_scaled(h, true); a.x = it.x; a.y = it.y; a.c = h->d_c_s; a.c_shared = h->shared_c;
+        a.dc = h->d_dc; a.lower = h->d_lo_o; a.upper = h->d_up_o; a.sel = h->d_sel; a.xs = h->d_xs;
+        a.fmask = h->d_fmask; a.tol = tol; a.tol_far = 1e-2; a.tol_r = 1e-5;
+        hipLaunchKernelGGL((k_polish_cols_init<T>), dim3(grid_c), dim3(BLOCK), 0, h->s, a);
+    }
+    if (h->m > 0) {
+        PolishRowsInitArgs<T> a;
+        a.sh = sr; a.A = op_orig(h, false); a.xs = h->d_xs; a.y = it.y; a.rhs = h->d_rhs_o; a.dr = h->d_dr;
+        a.sel = h->d_sel; a.rmask = h->d_rmask; a.rp = h->d_rp_; a.pp = h->d_pp; a.wp = h->d_wp; a.wd = h->d_wd;
+        a.neq = h->neq; a.tol_y = 1e-9; a.tol_s = 1e-7; a.partial = h->d_partial;
+        hipLaunchKernelGGL((k_polish_rows_init<T>), dim3(grid_r), dim3(BLOCK), 0, h->s, a);
+        // rho_p into red_a[0]
+        reduce(h, h->d_partial, h->d_red_a, h->d_ops_sum, 1, sr.n_rb);
+        // dual setup: q1 = D (c^ - A^T wd); r_d = p_d = R A q1
+        PolishColsArgs pc;
+        pc.sh = sc; pc.AT = op_scaled(h, true); pc.p1 = h->d_wd; pc.p2 = nullptr; pc.q1 = h->d_q1; pc.q2 = nullptr;
+        pc.c = h->d_c_s; pc.c_shared = h->shared_c; pc.fmask = h->d_fmask; pc.sel = h->d_sel;
+        hipLaunchKernelGGL(k_polish_cols, dim3(grid_c), dim3(BLOCK), 0, h->s, pc);
+        PolishRowsArgs pr;
+        pr.sh = sr; pr.A = op_scaled(h, false); pr.q1 = h->d_q1; pr.q2 = nullptr; pr.s1 = nullptr; pr.s2 = nullptr;
+        pr.p1 = nullptr; pr.p2 = nullptr; pr.setup_r = h->d_rd; pr.setup_p = h->d_pd; pr.rmask = h->d_rmask;
+        pr.sel = h->d_sel; pr.partial = h->d_partial;
+        hipLaunchKernelGGL(k_polish_rows, dim3(grid_r), dim3(BLOCK), 0, h->s, pr);
+        // rho_d into red_a[ld]: reduce 1 value (the setup writes dot into slot 0)
+        reduce(h, h->d_partial, h->d_red_a + ld, h->d_ops_sum, 1, sr.n_rb);
+        CgScalarArgs cs;
+        cs.red = h->d_red_a; cs.rho = h->d_rho; cs.alpha = h->d_alpha; cs.beta = h->d_beta; cs.sel = h->d_sel;
+        cs.batch = h->B; cs.ld = h->ld; cs.mode = 0;
+        hipLaunchKernelGGL(k_cg_scalars, dim3(ceil_div(h->B, 256)), dim3(256), 0, h->s, cs);
+        for (int k = 0; k < cg_iters; ++k) {
+            PolishColsArgs c1;
+            c1.sh = sc; c1.AT = op_scaled(h, true); c1.p1 = h->d_pp; c1.p2 = h->d_pd; c1.q1 = h->d_q1;
+            c1.q2 = h->d_q2; c1.c = nullptr; c1.c_shared = 0; c1.fmask = h->d_fmask; c1.sel = h->d_sel;
+            hipLaunchKernelGGL(k_polish_cols, dim3(grid_c), dim3(BLOCK), 0, h->s, c1);
+            PolishRowsArgs c2;
+            c2.sh = sr; c2.A = op_scaled(h, false); c2.q1 = h->d_q1; c2.q2 = h->d_q2; c2.s1 = h->d_s1; c2.s2 = h->d_s2;
+            c2.p1 = h->d_pp; c2.p2 = h->d_pd; c2.setup_r = nullptr; c2.setup_p = nullptr; c2.rmask = h->d_rmask;
+            c2.sel = h->d_sel; c2.partial = h->d_partial;
+            hipLaunchKernelGGL(k_polish_rows, dim3(grid_r), dim3(BLOCK), 0, h->s, c2);
+            reduce(h, h->d_partial, h->d_red_a, h->d_ops_sum, 2, sr.n_rb);
+            cs.mode = 1;
+            hipLaunchKernelGGL(k_cg_scalars, dim3(ceil_div(h->B, 256)), dim3(256), 0, h->s, cs);
+            CgUpdateArgs u;
+            u.sh = sr; u.w1 = h->d_wp; u.r1 = h->d_rp_; u.p1 = h->d_pp; u.s1 = h->d_s1;
+            u.w2 = h->d_wd; u.r2 = h->d_rd; u.p2 = h->d_pd; u.s2 = h->d_s2; u.alpha = h->d_alpha; u.sel = h->d_sel;
+            u.partial = h->d_partial;
+            hipLaunchKernelGGL(k_cg_update, dim3(grid_r), dim3(BLOCK), 0, h->s, u);
+            reduce(h, h->d_partial, h->d_red_a, h->d_ops_sum, 2, sr.n_rb);
+            cs.mode = 2;
+            hipLaunchKernelGGL(k_cg_scalars, dim3(ceil_div(h->B, 256)), dim3(256), 0, h->s, cs);
+            CgDirArgs dd;
+            dd.rows = h->m; dd.ld = h->ld; dd.rows_per_block = ROWS_PER_BLOCK_EW; dd.p1 = h->d_pp; dd.r1 = h->d_rp_;
+            dd.p2 = h->d_pd; dd.r2 = h->d_rd; dd.beta = h->d_beta; dd.sel = h->d_sel;
+            hipLaunchKernelGGL(k_cg_direction, ew_grid(h->m, h->ld), dim3(256), 0, h->s, dd);
+        }
+    }
+    {
+        PolishFinalColsArgs f;
+        f.sh = sc; f.AT = op_scaled(h, true); f.wp = h->d_wp; f.xs = h->d_xs; f.dc = h->d_dc; f.lower = h->d_lo_o;
+        f.upper = h->d_up_o; f.fmask = h->d_fmask; f.sel = h->d_sel; f.xout = h->d_xpol;
+        if (h->m == 0) {
+            // no rows: x is the snapped point
+            f.fmask = h->d_fmask;
+        }
+        hipLaunchKernelGGL(k_polish_final_cols, dim3(grid_c), dim3(BLOCK), 0, h->s, f);
+    }
+    if (h->m > 0) {
+        PolishFinalRowsArgs f;
+        f.rows = h->m; f.ld = h->ld; f.rows_per_block = ROWS_PER_BLOCK_EW; f.neq = h->neq; f.wd = h->d_wd;
+        f.dr = h->d_dr; f.rmask = h->d_rmask; f.sel = h->d_sel; f.yout = h->d_ypol;
+        hipLaunchKernelGGL(k_polish_final_rows, ew_grid(h->m, h->ld), dim3(256), 0, h->s, f);
+    }
+    HIPCHK(hipGetLastError());
+    return LPE_OK;
+}
+
+int polish_selected(lpe_handle* h, int cg_iters, double tol) {
+    if (h->m == 0) {
+        // box-only LPs: wp == 0, so the polish is the snapped point
+        HIPCHK(hipMemsetAsync(h->d_wp, 0, sizeof(double) * 1, h->s));
+    }
+    if (h->method != LPE_METHOD_FIXED && h->precision == LPE_PREC_FP32) return polish_t<float>(h, cg_iters, tol);
+    return polish_t<double>(h, cg_iters, tol);
+}
+
+int alloc_all(lpe_handle* h) {
+    const size_t ld = h->ld, m = h->m, n = h->n, nnz = h->nnz, nv = std::max(h->nvar, 1);
+    CHK(dalloc(h, &h->d_rp, m + 1)); CHK(dalloc(h, &h->d_ci, nnz)); CHK(dalloc(h, &h->d_rslot, nnz));
+    CHK(dalloc(h, &h->d_cp, n + 1)); CHK(dalloc(h, &h->d_ri, nnz)); CHK(dalloc(h, &h->d_cslot, nnz));
+    CHK(dalloc(h, &h->d_vrow, nv)); CHK(dalloc(h, &h->d_vcol, nv));
+    CHK(dalloc(h, &h->d_val_o, nnz)); CHK(dalloc(h, &h->d_tval_o, nnz));
+    CHK(dalloc(h, &h->d_val_s, nnz)); CHK(dalloc(h, &h->d_tval_s, nnz));
+    CHK(dalloc(h, &h->d_val_f, nnz)); CHK(dalloc(h, &h->d_tval_f, nnz));
+    CHK(dalloc(h, &h->d_var_o, nv * ld)); CHK(dalloc(h, &h->d_var_s, nv * ld)); CHK(dalloc(h, &h->d_var_f, nv * ld));
+    CHK(dalloc(h, &h->d_dr, m)); CHK(dalloc(h, &h->d_dc, n)); CHK(dalloc(h, &h->d_idr, m)); CHK(dalloc(h, &h->d_idc, n));
+    CHK(dalloc(h, &h->d_sig, m)); CHK(dalloc(h, &h->d_tau, n)); CHK(dalloc(h, &h->d_sig_f, m)); CHK(dalloc(h, &h->d_tau_f, n));
+    const bool ref = h->method == LPE_METHOD_FIXED;
+    if (ref) { CHK(dalloc(h, &h->d_sigref, m * ld)); CHK(dalloc(h, &h->d_tauref, n * ld)); }
+    const size_t nc = h->shared_c ? n : n * ld;
+    CHK(dalloc(h, &h->d_c_o, nc)); CHK(dalloc(h, &h->d_c_s, nc));
+    CHK(dalloc(h, &h->d_rhs_o, m * ld)); CHK(dalloc(h, &h->d_rhs_s, m * ld));
+    CHK(dalloc(h, &h->d_lo_o, n * ld)); CHK(dalloc(h, &h->d_up_o, n * ld));
+    CHK(dalloc(h, &h->d_lo_s, n * ld)); CHK(dalloc(h, &h->d_up_s, n * ld));
+    const bool f32 = !ref && h->precision == LPE_PREC_FP32;
+    if (f32) {
+        CHK(dalloc(h, &h->d_c_f, nc)); CHK(dalloc(h, &h->d_rhs_f, m * ld));
+        CHK(dalloc(h, &h->d_lo_f, n * ld)); CHK(dalloc(h, &h->d_up_f, n * ld));
+        Iter<float>& it = h->it32;
+        CHK(dalloc(h, &it.x, n * ld)); CHK(dalloc(h, &it.xbar, n * ld)); CHK(dalloc(h, &it.y, m * ld));
+        CHK(dalloc(h, &it.xsum, n * ld)); CHK(dalloc(h, &it.ysum, m * ld));
+        CHK(dalloc(h, &it.xlast, n * ld)); CHK(dalloc(h, &it.ylast, m * ld));
+        it.c = h->d_c_f; it.rhs = h->d_rhs_f; it.lo = h->d_lo_f; it.up = h->d_up_f;
+        it.val = h->d_val_f; it.tval = h->d_tval_f; it.var = h->d_var_f; it.sigma = h->d_sig_f; it.tau = h->d_tau_f;
+    } else {
+        Iter<double>& it = h->it64;
+        CHK(dalloc(h, &it.x, n * ld)); CHK(dalloc(h, &it.xbar, n * ld)); CHK(dalloc(h, &it.y, m * ld));
+        if (!ref) {
+            CHK(dalloc(h, &it.xsum, n * ld)); CHK(dalloc(h, &it.ysum, m * ld));
+            CHK(dalloc(h, &it.xlast, n * ld)); CHK(dalloc(h, &it.ylast, m * ld));
+        }
+        it.c = ref ? h->d_c_o : h->d_c_s; it.rhs = ref ? h->d_rhs_o : h->d_rhs_s;
+        it.lo = ref ? h->d_lo_o : h->d_lo_s; it.up = ref ? h->d_up_o : h->d_up_s;
+        it.val = h->d_val_s; it.tval = h->d_tval_s; it.var = h->d_var_s; it.sigma = h->d_sig; it.tau = h->d_tau;
+    }
+    CHK(dalloc(h, &h->d_st, ld)); CHK(dalloc(h, &h->d_flags, ld)); CHK(dalloc(h, &h->d_cnt, ld));
+    CHK(dalloc(h, &h->d_source, ld)); CHK(dalloc(h, &h->d_sel, ld)); CHK(dalloc(h, &h->d_passed, ld));
+    CHK(dalloc(h, &h->d_om, ld)); CHK(dalloc(h, &h->d_iom, ld)); CHK(dalloc(h, &h->d_om_f, ld)); CHK(dalloc(h, &h->d_iom_f, ld));
+    CHK(dalloc(h, &h->d_bnorm, ld)); CHK(dalloc(h, &h->d_cnorm, ld));
+    // partial buffers: max over kernels of n_rb * NV * ld
+    int g;
+    Shape sr = make_shape(std::max(h->m, 1), h->ld, h->B, CHECK_RPW, 1, &g);
+    Shape sc = make_shape(h->n, h->ld, h->B, CHECK_RPW, 1, &g);
+    const size_t pr = (size_t)sr.n_rb * 2 * CR_N, pc = (size_t)sc.n_rb * 2 * CC_N;
+    h->partial_elems = std::max(pr, pc) * ld;
+    CHK(dalloc(h, &h->d_partial, h->partial_elems));
+    CHK(dalloc(h, &h->d_red_a, (size_t)2 * CC_N * ld)); CHK(dalloc(h, &h->d_red_b, (size_t)2 * CC_N * ld));
+    const int ops_cr[2 * CR_N] = {RSUM, RMAX, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM, RMAX, RSUM, RSUM, RMAX, RSUM, RSUM};
+    const int ops_cc[2 * CC_N] = {RSUM, RMAX, RSUM, RSUM, RSUM, RMAX, RSUM, RSUM,
+                                  RSUM, RMAX, RSUM, RSUM, RSUM, RMAX, RSUM, RSUM};
+    const int ops_xr[XR_N] = {RMAX, RMAX, RSUM, RMAX, RSUM};
+    const int ops_xc[XC_N] = {RMAX, RMAX, RMAX, RMAX, RSUM, RSUM, RSUM, RSUM};
+    const int ops_sum[2] = {RSUM, RSUM};
+    CHK(dalloc(h, &h->d_ops_cr, 2 * CR_N)); CHK(upload(h->d_ops_cr, ops_cr, 2 * CR_N));
+    CHK(dalloc(h, &h->d_ops_cc, 2 * CC_N)); CHK(upload(h->d_ops_cc, ops_cc, 2 * CC_N));
+    CHK(dalloc(h, &h->d_ops_xr, XR_N)); CHK(upload(h->d_ops_xr, ops_xr, XR_N));
+    CHK(dalloc(h, &h->d_ops_xc, XC_N)); CHK(upload(h->d_ops_xc, ops_xc, XC_N));
+    CHK(dalloc(h, &h->d_ops_sum, 2)); CHK(upload(h->d_ops_sum, ops_sum, 2));
+    // polish and results
+    CHK(dalloc(h, &h->d_xs, n * ld)); CHK(dalloc(h, &h->d_fmask, n * ld)); CHK(dalloc(h, &h->d_rmask, m * ld));
+    CHK(dalloc(h, &h->d_rp_, m * ld)); CHK(dalloc(h, &h->d_pp, m * ld)); CHK(dalloc(h, &h->d_wp, m * ld));
+    CHK(dalloc(h, &h->d_rd, m * ld)); CHK(dalloc(h, &h->d_pd, m * ld)); CHK(dalloc(h, &h->d_wd, m * ld));
+    CHK(dalloc(h, &h->d_q1, n * ld)); CHK(dalloc(h, &h->d_q2, n * ld));
+    CHK(dalloc(h, &h->d_s1, m * ld)); CHK(dalloc(h, &h->d_s2, m * ld));
+    CHK(dalloc(h, &h->d_rho, 2 * ld)); CHK(dalloc(h, &h->d_alpha, 2 * ld)); CHK(dalloc(h, &h->d_beta, 2 * ld));
+    CHK(dalloc(h, &h->d_xpol, n * ld)); CHK(dalloc(h, &h->d_ypol, m * ld));
+    CHK(dalloc(h, &h->d_xres, n * ld)); CHK(dalloc(h, &h->d_yres, m * ld));
+    CHK(dalloc(h, &h->d_xin, n * ld)); CHK(dalloc(h, &h->d_yin, m * ld));
+    CHK(dalloc(h, &h->d_metrics, (size_t)8 * ld));
+    CHK(dalloc(h, &h->d_err, 8));
+    HIPCHK(hipHostMalloc((void**)&h->h_st, sizeof(LpState) * ld));
+    HIPCHK(hipHostMalloc((void**)&h->h_sel, sizeof(int) * ld));
+    HIPCHK(hipHostMalloc((void**)&h->h_passed, sizeof(int) * ld));
+    HIPCHK(hipHostMalloc((void**)&h->h_err, sizeof(int) * 8));
+    return LPE_OK;
+}
+
+void free_all(lpe_handle* h) {
+    if (h->graph) hipGraphExecDestroy(h->graph);
+    for (void* p : h->allocations) hipFree(p);
+    h->allocations.clear();
+    if (h->h_st) hipHostFree(h->h_st);
+    if (h->h_sel) hipHostFree(h->h_sel);
+    if (h->h_passed) hipHostFree(h->h_passed);
+    if (h->h_err) hipHostFree(h->h_err);
+    if (h->ev_sync) hipEventDestroy(h->ev_sync);
+    if (h->ev_a) hipEventDestroy(h->ev_a);
+    if (h->ev_b) hipEventDestroy(h->ev_b);
+    if (h->s) hipStreamDestroy(h->s);
+}
+
+int init_state(lpe_handle* h) {
+    for (int b = 0; b < h->ld; ++b) {
+        LpState& s = h->h_st[b];
+        std::memset(&s, 0, sizeof(s));
+        s.omega = h->method == LPE_METHOD_RESTARTED ? h->pw : 1.0;
+        s.kkt_last = INFINITY;
+        s.kkt_prev = INFINITY;
+        s.rel_kkt = INFINITY;
+        s.avg_div = 1.0;
+        s.accepted_iter = -1;
+        s.status = b < h->B ? 0 : 1;
+    }
+    HIPCHK(hipMemcpyAsync(h->d_st, h->h_st, sizeof(LpState) * h->ld, hipMemcpyHostToDevice, h->s));
+    std::vector<int> flags(h->ld), zeros(h->ld, 0);
+    std::vector<double> om(h->ld), iom(h->ld);
+    std::vector<float> omf(h->ld), iomf(h->ld);
+    for (int b = 0; b < h->ld; ++b) {
+        flags[b] = b < h->B ? 0 : 1;
+        om[b] = h->h_st[b].omega; iom[b] = 1.0 / om[b]; omf[b] = (float)om[b]; iomf[b] = (float)iom[b];
+    }
+    HIPCHK(hipMemcpyAsync(h->d_flags, flags.data(), sizeof(int) * h->ld, hipMemcpyHostToDevice, h->s));
+    HIPCHK(hipMemcpyAsync(h->d_cnt, zeros.data(), sizeof(int) * h->ld, hipMemcpyHostToDevice, h->s));
+    HIPCHK(hipMemcpyAsync(h->d_source, zeros.data(), sizeof(int) * h->ld, hipMemcpyHostToDevice, h->s));
+    HIPCHK(hipMemcpyAsync(h->d_om, om.data(), sizeof(double) * h->ld, hipMemcpyHostToDevice, h->s));
+    HIPCHK(hipMemcpyAsync(h->d_iom, iom.data(), sizeof(double) * h->ld, hipMemcpyHostToDevice, h->s));
+    HIPCHK(hipMemcpyAsync(h->d_om_f, omf.data(), sizeof(float) * h->ld, hipMemcpyHostToDevice, h->s));
+    HIPCHK(hipMemcpyAsync(h->d_iom_f, iomf.data(), sizeof(float) * h->ld, hipMemcpyHostToDevice, h->s));
+    // host vectors must outlive the async copies
+    HIPCHK(hipStreamSynchronize(h->s));
+    return LPE_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* lpe_version(void) { return "lpe 0.1 gfx950"; }
+
+int lpe_last_error(char* buf, int64_t size) {
+    if (!buf || size <= 0) return LPE_EINVAL;
+    std::snprintf(buf, (size_t)size, "%s", g_error.c_str());
+    return LPE_OK;
+}
+
+int lpe_create(const lpe_desc* d, lpe_handle** out) {
+    if (!d || !out) return fail(LPE_EINVAL, "desc and out are required");
+    *out = nullptr;
+    if (d->m < 0 || d->n < 1 || d->batch < 1 || d->neq < 0 || d->neq > d->m || d->nnz < 0)
+        return fail(LPE_EINVAL, "invalid dimensions (need n >= 1, batch >= 1, 0 <= neq <= m)");
+    if (d->precision != LPE_PREC_FP64 && d->precision != LPE_PREC_FP32) return fail(LPE_EINVAL, "invalid precision");
+    if (d->method != LPE_METHOD_FIXED && d->method != LPE_METHOD_RESTARTED) return fail(LPE_EINVAL, "invalid method");
+    if (d->method == LPE_METHOD_FIXED && d->precision != LPE_PREC_FP64)
+        return fail(LPE_EINVAL, "the fixed-metric method runs in FP64");
+    if (!(d->step_safety > 0.0 && d->step_safety < 1.0)) return fail(LPE_EINVAL, "step_safety must be strictly between zero and one");
+    if (!(d->theta >= 0.0 && d->theta <= 1.0)) return fail(LPE_EINVAL, "theta must be finite and between zero and one");
+    if (!(d->primal_weight > 0.0) || !std::isfinite(d->primal_weight)) return fail(LPE_EINVAL, "primal_weight must be finite and strictly positive");
+    const long m = d->m, n = d->n, nnz = d->nnz;
+    if ((long)d->batch * std::max(m, n) >= (1L << 31) - 1 || nnz >= (1L << 31) - 1)
+        return fail(LPE_EINVAL, "batch * dimension exceeds int32 capacity");
+    if (!d->row_ptr || (nnz > 0 && (!d->col_idx || !d->values))) return fail(LPE_EINVAL, "CSR arrays are required");
+    if (d->row_ptr[0] != 0 || d->row_ptr[m] != nnz) return fail(LPE_EINVAL, "Invalid CSR row pointers");
+    for (long i = 0; i < m; ++i) {
+        if (d->row_ptr[i + 1] < d->row_ptr[i]) return fail(LPE_EINVAL, "Invalid CSR row pointers");
+        for (long k = d->row_ptr[i]; k < d->row_ptr[i + 1]; ++k) {
+            const int c = d->col_idx[k];
+            if (c < 0 || c >= n) return fail(LPE_EINVAL, "Invalid CSR column indices");
+            if (k > d->row_ptr[i] && c <= d->col_idx[k - 1]) return fail(LPE_EINVAL, "CSR rows must have sorted, unique column indices");
+            if (!std::isfinite(d->values[k])) return fail(LPE_EINVAL, "Matrix values must be finite");
+        }
+    }
+    if (d->n_var < 0 || (d->n_var > 0 && !d->var_pos)) return fail(LPE_EINVAL, "invalid per-LP coefficient positions");
+
+    lpe_handle* h = new lpe_handle();
+    h->m = d->m; h->n = d->n; h->neq = d->neq; h->B = d->batch; h->nnz = nnz; h->nvar = d->n_var;
+    h->ld = ((d->batch + 255) / 256) * 256;
+    h->precision = d->precision; h->method = d->method; h->eta = d->step_safety; h->theta = d->theta;
+    h->pw = d->primal_weight; h->ruiz = std::max(0, d->ruiz_iterations); h->shared_c = d->shared_c ? 1 : 0;
+    hipGetDevice(&h->dev);
+    h->rp.assign(d->row_ptr, d->row_ptr + m + 1);
+    h->ci.assign(d->col_idx, d->col_idx + nnz);
+    h->val.assign(d->values, d->values + nnz);
+    h->rslot.assign(nnz, -1);
+    h->var_row.resize(h->nvar);
+    h->var_col.resize(h->nvar);
+    std::vector<int> row_of(nnz);
+    for (long i = 0; i < m; ++i)
+        for (long k = h->rp[i]; k < h->rp[i + 1]; ++k) row_of[k] = (int)i;
+    for (int s = 0; s < h->nvar; ++s) {
+        const long p = d->var_pos[s];
+        if (p < 0 || p >= nnz || h->rslot[p] >= 0) { delete h; return fail(LPE_EINVAL, "per-LP positions must be unique entries of the pattern"); }
+        h->rslot[p] = s;
+        h->var_row[s] = row_of[p];
+        h->var_col[s] = h->ci[p];
+    }
+    // transposed pattern (CSC == CSR of A^T), entries ordered by row inside each column
+    h->cp.assign(n + 1, 0);
+    for (long k = 0; k < nnz; ++k) h->cp[h->ci[k] + 1]++;
+    for (long j = 0; j < n; ++j) h->cp[j + 1] += h->cp[j];
+    h->ri.resize(nnz); h->cslot.resize(nnz); h->csr_to_csc.resize(nnz);
+    {
+        std::vector<int> fill(h->cp.begin(), h->cp.end() - 1);
+        for (long i = 0; i < m; ++i)
+            for (long k = h->rp[i]; k < h->rp[i + 1]; ++k) {
+                const int pos = fill[h->ci[k]]++;
+                h->ri[pos] = (int)i;
+                h->cslot[pos] = h->rslot[k];
+                h->csr_to_csc[k] = pos;
+            }
+    }
+    hipError_t e = hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete h; return fail(LPE_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
+    hipEventCreateWithFlags(&h->ev_sync, hipEventDisableTiming);
+    hipEventCreate(&h->ev_a);
+    hipEventCreate(&h->ev_b);
+    int rc = alloc_all(h);
+    if (rc != LPE_OK) { free_all(h); delete h; return rc; }
+    ruiz_equilibrate(h);
+    h->var_absmax.assign(std::max(h->nvar, 1), 0.0);
+    for (int s = 0; s < h->nvar; ++s) {
+        const long p = d->var_pos[s];
+        h->var_absmax[s] = std::fabs(h->val[p] * h->dr[h->var_row[s]] * h->dc[h->var_col[s]]);
+    }
+    shared_metric(h);
+    // device structure
+    std::vector<double> tval(nnz), sval(nnz), tsval(nnz);
+    std::vector<float> fval(nnz), tfval(nnz);
+    for (long i = 0; i < m; ++i)
+        for (long k = h->rp[i]; k < h->rp[i + 1]; ++k) {
+            const long t = h->csr_to_csc[k];
+            const double sv = h->val[k] * h->dr[i] * h->dc[h->ci[k]];
+            tval[t] = h->val[k]; sval[k] = sv; tsval[t] = sv; fval[k] = (float)sv; tfval[t] = (float)sv;
+        }
+    std::vector<double> idr(m), idc(n);
+    for (long i = 0; i < m; ++i) idr[i] = 1.0 / h->dr[i];
+    for (long j = 0; j < n; ++j) idc[j] = 1.0 / h->dc[j];
+    rc = LPE_OK;
+    if (rc == LPE_OK) rc = upload(h->d_rp, h->rp.data(), m + 1);
+    if (rc == LPE_OK) rc = upload(h->d_ci, h->ci.data(), nnz);
+    if (rc == LPE_OK) rc = upload(h->d_rslot, h->rslot.data(), nnz);
+    if (rc == LPE_OK) rc = upload(h->d_cp, h->cp.data(), n + 1);
+    if (rc == LPE_OK) rc = upload(h->d_ri, h->ri.data(), nnz);
+    if (rc == LPE_OK) rc = upload(h->d_cslot, h->cslot.data(), nnz);
+    if (rc == LPE_OK) rc = upload(h->d_vrow, h->var_row.data(), h->nvar);
+    if (rc == LPE_OK) rc = upload(h->d_vcol, h->var_col.data(), h->nvar);
+    if (rc == LPE_OK) rc = upload(h->d_val_o, h->val.data(), nnz);
+    if (rc == LPE_OK) rc = upload(h->d_tval_o, tval.data(), nnz);
+    if (rc == LPE_OK) rc = upload(h->d_val_s, sval.data(), nnz);
+    if (rc == LPE_OK) rc = upload(h->d_tval_s, tsval.data(), nnz);
+    if (rc == LPE_OK) rc = upload(h->d_val_f, fval.data(), nnz);
+    if (rc == LPE_OK) rc = upload(h->d_tval_f, tfval.data(), nnz);
+    if (rc == LPE_OK) rc = upload(h->d_dr, h->dr.data(), m);
+    if (rc == LPE_OK) rc = upload(h->d_dc, h->dc.data(), n);
+    if (rc == LPE_OK) rc = upload(h->d_idr, idr.data(), m);
+    if (rc == LPE_OK) rc = upload(h->d_idc, idc.data(), n);
+    if (rc == LPE_OK) rc = upload_metric(h);
+    if (rc == LPE_OK && h->nvar > 0) {
+        // per-LP values default to the template value
+        std::vector<double> v((size_t)h->nvar * h->ld);
+        for (int s = 0; s < h->nvar; ++s)
+            for (int b = 0; b < h->ld; ++b) v[(size_t)s * h->ld + b] = h->val[d->var_pos[s]];
+        rc = upload(h->d_var_o, v.data(), v.size());
+    }
+    if (rc == LPE_OK) rc = init_state(h);
+    if (rc != LPE_OK) { free_all(h); delete h; return rc; }
+    h->rpw = 1;
+    h->lpl = 1;
+    *out = h;
+    return LPE_OK;
+}
+
+int lpe_destroy(lpe_handle* h) {
+    if (!h) return LPE_OK;
+    hipSetDevice(h->dev);
+    if (h->s) hipStreamSynchronize(h->s);
+    free_all(h);
+    delete h;
+    return LPE_OK;
+}
+
+int lpe_update(lpe_handle* h, const lpe_vectors* v, void* stream) {
+    if (!h || !v) return fail(LPE_EINVAL, "handle and vectors are required");
+    if (!v->c || !v->lower || !v->upper || (h->m > 0 && !v->rhs))
+        return fail(LPE_EINVAL, "Initial problem vectors c, rhs, lower and upper are required");
+    CHK(enter(h, stream));
+    const long B = h->B, m = h->m, n = h->n, ld = h->ld;
+    const long nc = h->shared_c ? n : B * n;
+    const long nv = v->var_values ? B * h->nvar : 0;
+    HIPCHK(hipMemsetAsync(h->d_err, 0, sizeof(int) * 8, h->s));
+    hipLaunchKernelGGL(k_validate, dim3(512), dim3(256), 0, h->s, v->c, nc, v->rhs, B * m, v->lower, v->upper, B * n,
+                       v->var_values, nv, h->d_err);
+    HIPCHK(hipMemcpyAsync(h->h_err, h->d_err, sizeof(int) * 8, hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    if (h->h_err[0]) return fail(LPE_EINVAL, "objective contains invalid values");
+    if (h->h_err[1]) return fail(LPE_EINVAL, "rhs contains invalid values");
+    if (h->h_err[2]) return fail(LPE_EINVAL, "Bounds require lower <= upper and finite feasible points");
+    if (h->h_err[3]) return fail(LPE_EINVAL, "per-LP coefficients must be finite");
+    // all inputs valid: copy (no state has changed before this point)
+    h->ready = false;
+    const dim3 tb(256);
+    auto tin = [&](const double* src, double* dst, long len, double fill) {
+        if (len == 0) return;
+        hipLaunchKernelGGL((k_to_internal<double>), dim3(ceil_div(len, 64), ceil_div(ld, 64)), tb, 0, h->s, src, dst,
+                           (int)len, (int)B, (int)ld, (const double*)nullptr, 0, fill);
+    };
+    if (h->shared_c) HIPCHK(hipMemcpyAsync(h->d_c_o, v->c, sizeof(double) * n, hipMemcpyDeviceToDevice, h->s));
+    else tin(v->c, h->d_c_o, n, 0.0);
+    tin(v->rhs, h->d_rhs_o, m, 0.0);
+    tin(v->lower, h->d_lo_o, n, 0.0);
+    tin(v->upper, h->d_up_o, n, 0.0);
+    if (v->var_values && h->nvar > 0) tin(v->var_values, h->d_var_o, h->nvar, 0.0);
+    // scaled copies
+    const bool ref = h->method == LPE_METHOD_FIXED;
+    if (h->shared_c) {
+        std::vector<double> hc(n);
+        HIPCHK(hipMemcpyAsync(hc.data(), h->d_c_o, sizeof(double) * n, hipMemcpyDeviceToHost, h->s));
+        HIPCHK(hipStreamSynchronize(h->s));
+        for (long j = 0; j < n; ++j) hc[j] *= h->dc[j];
+        CHK(upload(h->d_c_s, hc.data(), n));
+        if (h->d_c_f) {
+            std::vector<float> f(n);
+            for (long j = 0; j < n; ++j) f[j] = (float)hc[j];
+            CHK(upload(h->d_c_f, f.data(), n));
+        }
+    } else {
+        hipLaunchKernelGGL((k_scale_copy<double>), ew_grid(n, ld), tb, 0, h->s, (const double*)h->d_c_o, h->d_c_s, (int)n,
+                           (int)ld, ROWS_PER_BLOCK_EW, (const double*)h->d_dc, (const int*)nullptr, 0);
+    }
+    if (m > 0)
+        hipLaunchKernelGGL((k_scale_copy<double>), ew_grid(m, ld), tb, 0, h->s, (const double*)h->d_rhs_o, h->d_rhs_s,
+                           (int)m, (int)ld, ROWS_PER_BLOCK_EW, (const double*)h->d_dr, (const int*)nullptr, 0);
+    hipLaunchKernelGGL((k_scale_copy<double>), ew_grid(n, ld), tb, 0, h->s, (const double*)h->d_lo_o, h->d_lo_s, (int)n,
+                       (int)ld, ROWS_PER_BLOCK_EW, (const double*)h->d_idc, (const int*)nullptr, 0);
+    hipLaunchKernelGGL((k_scale_copy<double>), ew_grid(n, ld), tb, 0, h->s, (const double*)h->d_up_o, h->d_up_s, (int)n,
+                       (int)ld, ROWS_PER_BLOCK_EW, (const double*)h->d_idc, (const int*)nullptr, 0);
+    if (h->nvar > 0) {
+        hipLaunchKernelGGL((k_var_scale<double>), dim3(ceil_div(ld, 256), h->nvar), tb, 0, h->s, (const double*)h->d_var_o,
+                           h->d_var_s, h->nvar, (int)ld, (const int*)h->d_vrow, (const int*)h->d_vcol,
+                           (const double*)h->d_dr, (const double*)h->d_dc);
+        hipLaunchKernelGGL((k_var_scale<float>), dim3(ceil_div(ld, 256), h->nvar), tb, 0, h->s, (const double*)h->d_var_o,
+                           h->d_var_f, h->nvar, (int)ld, (const int*)h->d_vrow, (const int*)h->d_vcol,
+                           (const double*)h->d_dr, (const double*)h->d_dc);
+    }
+    if (h->d_rhs_f) {
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(ceil_div(m * ld, 256)), tb, 0, h->s, (const double*)h->d_rhs_s, h->d_rhs_f, m * ld);
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(ceil_div(n * ld, 256)), tb, 0, h->s, (const double*)h->d_lo_s, h->d_lo_f, n * ld);
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(ceil_div(n * ld, 256)), tb, 0, h->s, (const double*)h->d_up_s, h->d_up_f, n * ld);
+        if (!h->shared_c)
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(ceil_div(n * ld, 256)), tb, 0, h->s, (const double*)h->d_c_s, h->d_c_f, n * ld);
+    }
+    if (ref) {
+        int grid;
+        RefMetricArgs r;
+        r.eta = h->eta; r.weight = h->pw;
+        if (m > 0) {
+            r.sh = make_shape((int)m, (int)ld, h->B, 1, 1, &grid); r.A = op_orig(h, false); r.out = h->d_sigref; r.divide = 0;
+            hipLaunchKernelGGL(k_ref_metric, dim3(grid), dim3(BLOCK), 0, h->s, r);
+        }
+        r.sh = make_shape((int)n, (int)ld, h->B, 1, 1, &grid); r.A = op_orig(h, true); r.out = h->d_tauref; r.divide = 1;
+        hipLaunchKernelGGL(k_ref_metric, dim3(grid), dim3(BLOCK), 0, h->s, r);
+    } else if (h->nvar > 0) {
+        // shared metric from the largest per-LP magnitudes of the batch
+        std::vector<double> hv((size_t)h->nvar * ld);
+        HIPCHK(hipMemcpyAsync(hv.data(), h->d_var_s, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, h->s));
+        HIPCHK(hipStreamSynchronize(h->s));
+        for (int s = 0; s < h->nvar; ++s) {
+            double mx = 0.0;
+            for (long b = 0; b < B; ++b) mx = std::max(mx, std::fabs(hv[(size_t)s * ld + b]));
+            h->var_absmax[s] = mx;
+        }
+        shared_metric(h);
+        CHK(upload_metric(h));
+    }
+    hipLaunchKernelGGL(k_norms, dim3(ceil_div(ld, 256)), tb, 0, h->s, (const double*)h->d_rhs_s, (int)m, (int)ld, 0, h->d_bnorm);
+    hipLaunchKernelGGL(k_norms, dim3(ceil_div(ld, 256)), tb, 0, h->s, (const double*)h->d_c_s, (int)n, (int)ld, h->shared_c, h->d_cnorm);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->s));
+    h->have_vectors = true;
+    return leave(h, stream);
+}
+
+int lpe_reset(lpe_handle* h, const double* x0, const double* y0, void* stream) {
+    if (!h) return fail(LPE_EINVAL, "handle is required");
+    if (!h->have_vectors) return fail(LPE_ESTATE, "lpe_update is required before lpe_reset");
+    CHK(enter(h, stream));
+    const long B = h->B, m = h->m, n = h->n, ld = h->ld;
+    if (x0 || y0) {
+        HIPCHK(hipMemsetAsync(h->d_err, 0, sizeof(int) * 8, h->s));
+        hipLaunchKernelGGL(k_validate, dim3(256), dim3(256), 0, h->s, x0, x0 ? B * n : 0, y0, y0 ? B * m : 0,
+                           (const double*)nullptr, (const double*)nullptr, 0L, (const double*)nullptr, 0L, h->d_err);
+        HIPCHK(hipMemcpyAsync(h->h_err, h->d_err, sizeof(int) * 8, hipMemcpyDeviceToHost, h->s));
+        HIPCHK(hipStreamSynchronize(h->s));
+        if (h->h_err[0]) return fail(LPE_EINVAL, "initial_x must contain only finite values");
+        if (h->h_err[1]) return fail(LPE_EINVAL, "initial_y must contain only finite values");
+    }
+    const dim3 tb(256);
+    const bool f32 = h->method != LPE_METHOD_FIXED && h->precision == LPE_PREC_FP32;
+    auto load = [&](const double* src, long len, const double* inv_scale, auto* dst) {
+        using TO = typename std::remove_pointer<decltype(dst)>::type;
+        if (len == 0) return;
+        if (src)
+            hipLaunchKernelGGL((k_to_internal<TO>), dim3(ceil_div(len, 64), ceil_div(ld, 64)), tb, 0, h->s, src, dst,
+                               (int)len, (int)B, (int)ld, inv_scale, 1, 0.0);
+        else
+            hipMemsetAsync(dst, 0, sizeof(TO) * len * ld, h->s);
+    };
+    if (f32) {
+        Iter<float>& it = h->it32;
+        load(x0, n, h->d_idc, it.x);
+        load(y0, m, h->d_idr, it.y);
+        hipLaunchKernelGGL((k_project<float>), ew_grid(n, ld), tb, 0, h->s, it.x, it.xbar, it.xlast, it.xsum,
+                           (const float*)it.lo, (const float*)it.up, (int)n, (int)ld, ROWS_PER_BLOCK_EW, 0, h->neq);
+        if (m > 0)
+            hipLaunchKernelGGL((k_project<float>), ew_grid(m, ld), tb, 0, h->s, it.y, (float*)nullptr, it.ylast, it.ysum,
+                               (const float*)nullptr, (const float*)nullptr, (int)m, (int)ld, ROWS_PER_BLOCK_EW, 1, h->neq);
+    } else {
+        Iter<double>& it = h->it64;
+        load(x0, n, h->d_idc, it.x);
+        load(y0, m, h->d_idr, it.y);
+        hipLaunchKernelGGL((k_project<double>), ew_grid(n, ld), tb, 0, h->s, it.x, it.xbar, it.xlast, it.xsum,
+                           (const double*)it.lo, (const double*)it.up, (int)n, (int)ld, ROWS_PER_BLOCK_EW, 0, h->neq);
+        if (m > 0)
+            hipLaunchKernelGGL((k_project<double>), ew_grid(m, ld), tb, 0, h->s, it.y, (double*)nullptr, it.ylast, it.ysum,
+                               (const double*)nullptr, (const double*)nullptr, (int)m, (int)ld, ROWS_PER_BLOCK_EW, 1, h->neq);
+    }
+    HIPCHK(hipGetLastError());
+    CHK(init_state(h));
+    h->ready = true;
+    return leave(h, stream);
+}
+
+int lpe_run(lpe_handle* h, int64_t iterations, int32_t chunk, int32_t use_graph, void* stream) {
+    if (!h) return fail(LPE_EINVAL, "handle is required");
+    if (iterations < 0) return fail(LPE_EINVAL, "iterations must be non-negative");
+    if (!h->ready) return fail(LPE_ESTATE, "reset is required after a problem update");
+    CHK(enter(h, stream));
+    CHK(run_iterations(h, iterations, chunk, use_graph));
+    return leave(h, stream);
+}
+
+int lpe_set_accepted(lpe_handle* h, const uint8_t* mask, void* stream) {
+    if (!h || !mask) return fail(LPE_EINVAL, "handle and mask are required");
+    if (!h->ready) return fail(LPE_ESTATE, "reset is required after a problem update");
+    CHK(enter(h, stream));
+    std::vector<int> flags(h->ld, 1);
+    HIPCHK(hipMemcpyAsync(flags.data(), h->d_flags, sizeof(int) * h->ld, hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    for (int b = 0; b < h->B; ++b)
+        if (flags[b] && !mask[b]) return fail(LPE_EINVAL, "Cannot unfreeze accepted environments without reset");
+    for (int b = 0; b < h->B; ++b) flags[b] = mask[b] ? 1 : 0;
+    HIPCHK(hipMemcpyAsync(h->d_flags, flags.data(), sizeof(int) * h->ld, hipMemcpyHostToDevice, h->s));
+    // frozen LPs: xbar = x
+    const bool f32 = h->method != LPE_METHOD_FIXED && h->precision == LPE_PREC_FP32;
+    if (f32) hipLaunchKernelGGL((k_masked_copy_t<float>), ew_grid(h->n, h->ld), dim3(256), 0, h->s, (const float*)h->it32.x, h->it32.xbar, h->n, h->ld, ROWS_PER_BLOCK_EW, (const int*)h->d_flags, 1);
+    else hipLaunchKernelGGL((k_masked_copy_t<double>), ew_grid(h->n, h->ld), dim3(256), 0, h->s, (const double*)h->it64.x, h->it64.xbar, h->n, h->ld, ROWS_PER_BLOCK_EW, (const int*)h->d_flags, 1);
+    HIPCHK(hipStreamSynchronize(h->s));
+    return leave(h, stream);
+}
+
+int lpe_solve(lpe_handle* h, const lpe_solve_opts* o, lpe_solve_info* info, void* stream) {
+    if (!h || !o) return fail(LPE_EINVAL, "handle and options are required");
+    if (!h->ready) return fail(LPE_ESTATE, "reset is required after a problem update");
+    if (o->max_iterations < 0 || o->check_interval < 1) return fail(LPE_EINVAL, "invalid iteration budget or check interval");
+    const auto t0 = std::chrono::steady_clock::now();
+    CHK(enter(h, stream));
+    if (o->rows_per_wave > 0) h->rpw = o->rows_per_wave;
+    if (o->lps_per_lane == 1 || o->lps_per_lane == 2 || o->lps_per_lane == 4) h->lpl = o->lps_per_lane;
+    const double ep = o->eps_certify_primal > 0 ? o->eps_certify_primal : 1e-5;
+    const double ed = o->eps_certify_dual > 0 ? o->eps_certify_dual : 1e-7;
+    const double eg = o->eps_certify_gap > 0 ? o->eps_certify_gap : 1e-7;
+    const int K = o->check_interval;
+    const bool do_polish = o->polish != 0;
+    const int cg = o->polish_cg_iterations > 0 ? o->polish_cg_iterations : 200;
+    std::vector<double> polish_kkt(h->B, o->polish_kkt > 0 ? o->polish_kkt : 1e-3);
+    lpe_solve_info inf;
+    std::memset(&inf, 0, sizeof(inf));
+    float ms = 0.0f;
+    long iters = 0;
+    int running = 0;
+    hipEvent_t e0 = h->ev_a, e1 = h->ev_b;
+    // initial state readback
+    HIPCHK(hipMemcpyAsync(h->h_st, h->d_st, sizeof(LpState) * h->ld, hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    for (int b = 0; b < h->B; ++b) running += h->h_st[b].status == 0;
+    auto after_chunk = [&](int k) -> int {
+        HIPCHK(hipEventRecord(e0, h->s));
+        CHK(launch_check(h, k, 1, ep, ed, eg));
+        HIPCHK(hipMemcpyAsync(h->h_st, h->d_st, sizeof(LpState) * h->ld, hipMemcpyDeviceToHost, h->s));
+        HIPCHK(hipEventRecord(e1, h->s));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        inf.check_ms += ms;
+        running = 0;
+        int nsel = 0;
+        for (int b = 0; b < h->ld; ++b) {
+            h->h_sel[b] = 0;
+            if (b >= h->B || h->h_st[b].status != 0) continue;
+            running++;
+            if (do_polish && h->h_st[b].rel_kkt <= polish_kkt[b]) { h->h_sel[b] = 1; nsel++; }
+        }
+        const bool last = iters >= o->max_iterations;
+        if (!(do_polish && nsel > 0 && (nsel * 4 >= running || last || nsel == running))) return LPE_OK;
+        HIPCHK(hipEventRecord(e0, h->s));
+        HIPCHK(hipMemcpyAsync(h->d_sel, h->h_sel, sizeof(int) * h->ld, hipMemcpyHostToDevice, h->s));
+        CHK(polish_selected(h, cg, 1e-6));
+        CHK(launch_certificate(h, h->d_xpol, h->d_ypol, ep, ed, eg));
+        HIPCHK(hipMemcpyAsync(h->h_passed, h->d_passed, sizeof(int) * h->ld, hipMemcpyDeviceToHost, h->s));
+        HIPCHK(hipEventRecord(e1, h->s));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        inf.polish_ms += ms;
+        inf.polish_attempts++;
+        bool any = false;
+        for (int b = 0; b < h->B; ++b) {
+            if (!h->h_sel[b]) continue;
+            if (h->h_passed[b]) {
+                any = true;
+                running--;
+            } else {
+                h->h_sel[b] = 0;
+                polish_kkt[b] *= 0.1;
+            }
+        }
+        if (any) {
+            // accepted: keep the polish result, freeze the LP
+            HIPCHK(hipMemcpyAsync(h->d_sel, h->h_sel, sizeof(int) * h->ld, hipMemcpyHostToDevice, h->s));
+            hipLaunchKernelGGL(k_masked_copy, ew_grid(h->n, h->ld), dim3(256), 0, h->s, (const double*)h->d_xpol,
+                               h->d_xres, h->n, h->ld, ROWS_PER_BLOCK_EW, (const int*)h->d_sel, 1);
+            if (h->m > 0)
+                hipLaunchKernelGGL(k_masked_copy, ew_grid(h->m, h->ld), dim3(256), 0, h->s, (const double*)h->d_ypol,
+                                   h->d_yres, h->m, h->ld, ROWS_PER_BLOCK_EW, (const int*)h->d_sel, 1);
+            hipLaunchKernelGGL(k_accept_polished, dim3(ceil_div(h->ld, 256)), dim3(256), 0, h->s, h->d_st,
+                               h->d_flags, h->d_source, (const int*)h->d_sel, h->ld);
+            HIPCHK(hipGetLastError());
+        }
+        return LPE_OK;
+    };
+    // iteration 0: a warm start may already be optimal (src/gpu_pdhg_corrector.py:318-328)
+    if (running > 0) CHK(after_chunk(0));
+    while (running > 0 && iters < o->max_iterations) {
+        const int k = (int)std::min<long>(K, o->max_iterations - iters);
+        HIPCHK(hipEventRecord(e0, h->s));
+        CHK(run_iterations(h, k, k, o->use_graph));
+        HIPCHK(hipEventRecord(e1, h->s));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        inf.pdhg_ms += ms;
+        inf.pdhg_kernel_launches += (long)k * kernel_launches_per_iteration(h);
+        iters += k;
+        inf.chunks++;
+        CHK(after_chunk(k));
+    }
+    // final results and final certificate authority (src/gpu_resident_reduced_pdhg.py:147-150)
+    CHK(build_results(h));
+    CHK(launch_certificate(h, h->d_xres, h->d_yres, ep, ed, eg));
+    HIPCHK(hipMemcpyAsync(h->h_passed, h->d_passed, sizeof(int) * h->ld, hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipMemcpyAsync(h->h_st, h->d_st, sizeof(LpState) * h->ld, hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    int certified = 0, restarts = 0;
+    for (int b = 0; b < h->B; ++b) {
+        LpState& s = h->h_st[b];
+        if (s.status == LPE_STATUS_OPTIMAL && !h->h_passed[b]) s.status = LPE_STATUS_LIMIT;
+        if (s.status == 0) s.status = LPE_STATUS_LIMIT;
+        certified += s.status == LPE_STATUS_OPTIMAL;
+        restarts += s.restarts;
+    }
+    HIPCHK(hipMemcpyAsync(h->d_st, h->h_st, sizeof(LpState) * h->ld, hipMemcpyHostToDevice, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    inf.iterations = iters;
+    inf.certified = certified;
+    inf.restarts = restarts;
+    inf.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (info) *info = inf;
+    return leave(h, stream);
+}
+
+int lpe_certify(lpe_handle* h, const double* x, const double* y, double* metrics, double ep, double ed, double eg,
+                void* stream) {
+    if (!h || !metrics) return fail(LPE_EINVAL, "handle and metrics are required");
+    if (!h->have_vectors) return fail(LPE_ESTATE, "lpe_update is required before lpe_certify");
+    if (h->m > 0 && ((x == nullptr) != (y == nullptr)))
+        return fail(LPE_EINVAL, "x and y must both be given or both be NULL");
+    CHK(enter(h, stream));
+    const long B = h->B, m = h->m, n = h->n, ld = h->ld;
+    const dim3 tb(256);
+    if (x) {
+        hipLaunchKernelGGL((k_to_internal<double>), dim3(ceil_div(n, 64), ceil_div(ld, 64)), tb, 0, h->s, x, h->d_xin,
+                           (int)n, (int)B, (int)ld, (const double*)nullptr, 0, 0.0);
+        if (m > 0)
+            hipLaunchKernelGGL((k_to_internal<double>), dim3(ceil_div(m, 64), ceil_div(ld, 64)), tb, 0, h->s, y, h->d_yin,
+                               (int)m, (int)B, (int)ld, (const double*)nullptr, 0, 0.0);
+    } else {
+        CHK(iterate_to_original(h, h->d_xin, h->d_yin));
+    }
+    CHK(launch_certificate(h, h->d_xin, h->d_yin, ep > 0 ? ep : 1e-5, ed > 0 ? ed : 1e-7, eg > 0 ? eg : 1e-7));
+    HIPCHK(hipMemcpyAsync(metrics, h->d_metrics, sizeof(double) * 8 * B, hipMemcpyDeviceToDevice, h->s));
+    return leave(h, stream);
+}
+
+int lpe_state(lpe_handle* h, double* x, double* y, int32_t* status, void* stream) {
+    if (!h) return fail(LPE_EINVAL, "handle is required");
+    if (!h->ready) return fail(LPE_ESTATE, "reset is required after a problem update");
+    CHK(enter(h, stream));
+    const long B = h->B, m = h->m, n = h->n, ld = h->ld;
+    CHK(build_results(h));
+    const dim3 tb(256);
+    if (x)
+        hipLaunchKernelGGL((k_to_external<double>), dim3(ceil_div(n, 64), ceil_div(ld, 64)), tb, 0, h->s,
+                           (const double*)h->d_xres, x, (int)n, (int)B, (int)ld, (const double*)nullptr, 0);
+    if (y && m > 0)
+        hipLaunchKernelGGL((k_to_external<double>), dim3(ceil_div(m, 64), ceil_div(ld, 64)), tb, 0, h->s,
+                           (const double*)h->d_yres, y, (int)m, (int)B, (int)ld, (const double*)nullptr, 0);
+    HIPCHK(hipGetLastError());
+    if (status) {
+        HIPCHK(hipMemcpyAsync(h->h_st, h->d_st, sizeof(LpState) * h->ld, hipMemcpyDeviceToHost, h->s));
+        HIPCHK(hipStreamSynchronize(h->s));
+        for (long b = 0; b < B; ++b) status[b] = h->h_st[b].status;
+    }
+    return leave(h, stream);
+}
+
+int lpe_polish(lpe_handle* h, const uint8_t* mask, int32_t cg_iterations, void* stream) {
+    if (!h) return fail(LPE_EINVAL, "handle is required");
+    if (!h->ready) return fail(LPE_ESTATE, "reset is required after a problem update");
+    CHK(enter(h, stream));
+    HIPCHK(hipMemcpyAsync(h->h_st, h->d_st, sizeof(LpState) * h->ld, hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    for (int b = 0; b < h->ld; ++b)
+        h->h_sel[b] = (b < h->B && (mask ? mask[b] != 0 : h->h_st[b].status == 0)) ? 1 : 0;
+    HIPCHK(hipMemcpyAsync(h->d_sel, h->h_sel, sizeof(int) * h->ld, hipMemcpyHostToDevice, h->s));
+    CHK(polish_selected(h, cg_iterations > 0 ? cg_iterations : 200, 1e-6));
+    CHK(launch_certificate(h, h->d_xpol, h->d_ypol, 1e-5, 1e-7, 1e-7));
+    HIPCHK(hipMemcpyAsync(h->h_passed, h->d_passed, sizeof(int) * h->ld, hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    for (int b = 0; b < h->ld; ++b) h->h_sel[b] = h->h_sel[b] && h->h_passed[b];
+    HIPCHK(hipMemcpyAsync(h->d_sel, h->h_sel, sizeof(int) * h->ld, hipMemcpyHostToDevice, h->s));
+    hipLaunchKernelGGL(k_masked_copy, ew_grid(h->n, h->ld), dim3(256), 0, h->s, (const double*)h->d_xpol, h->d_xres,
+                       h->n, h->ld, ROWS_PER_BLOCK_EW, (const int*)h->d_sel, 1);
+    if (h->m > 0)
+        hipLaunchKernelGGL(k_masked_copy, ew_grid(h->m, h->ld), dim3(256), 0, h->s, (const double*)h->d_ypol, h->d_yres,
+                           h->m, h->ld, ROWS_PER_BLOCK_EW, (const int*)h->d_sel, 1);
+    hipLaunchKernelGGL(k_accept_polished, dim3(ceil_div(h->ld, 256)), dim3(256), 0, h->s, h->d_st, h->d_flags,
+                       h->d_source, (const int*)h->d_sel, h->ld);
+    HIPCHK(hipGetLastError());
+    return leave(h, stream);
+}
+
+int lpe_spmv(lpe_handle* h, int32_t transpose, int32_t precision, const double* in, double* out, void* stream) {
+    if (!h || !in || !out) return fail(LPE_EINVAL, "handle, in and out are required");
+    if (precision != LPE_PREC_FP64 && precision != LPE_PREC_FP32) return fail(LPE_EINVAL, "invalid precision");
+    CHK(enter(h, stream));
+    const long B = h->B, ld = h->ld;
+    const long lin = transpose ? h->m : h->n, lout = transpose ? h->n : h->m;
+    if (lout == 0) return leave(h, stream);
+    const dim3 tb(256);
+    int grid;
+    Shape sh = make_shape((int)lout, (int)ld, h->B, 1, 1, &grid);
+    if (precision == LPE_PREC_FP64) {
+        double* xin = transpose ? h->d_yin : h->d_xin;
+        double* yout = transpose ? h->d_xin : h->d_yin;
+        if (lin > 0)
+            hipLaunchKernelGGL((k_to_internal<double>), dim3(ceil_div(lin, 64), ceil_div(ld, 64)), tb, 0, h->s, in, xin,
+                               (int)lin, (int)B, (int)ld, (const double*)nullptr, 0, 0.0);
+        SpmvArgs<double, double> a;
+        a.sh = sh; a.A = op_orig(h, transpose != 0); a.x = xin; a.y = yout;
+        hipLaunchKernelGGL((k_spmv<double, double>), dim3(grid), dim3(BLOCK), 0, h->s, a);
+        hipLaunchKernelGGL((k_to_external<double>), dim3(ceil_div(lout, 64), ceil_div(ld, 64)), tb, 0, h->s,
+                           (const double*)yout, out, (int)lout, (int)B, (int)ld, (const double*)nullptr, 0);
+    } else {
+        // FP32 operands: the original values rounded to float
+        float* xin = reinterpret_cast<float*>(transpose ? h->d_yin : h->d_xin);
+        float* yout = reinterpret_cast<float*>(transpose ? h->d_xin : h->d_yin);
+        float* vals = reinterpret_cast<float*>(h->d_q1);
+        if (lin > 0)
+            hipLaunchKernelGGL((k_to_internal<float>), dim3(ceil_div(lin, 64), ceil_div(ld, 64)), tb, 0, h->s, in, xin,
+                               (int)lin, (int)B, (int)ld, (const double*)nullptr, 0, 0.0);
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(ceil_div(h->nnz, 256)), tb, 0, h->s,
+                           (const double*)(transpose ? h->d_tval_o : h->d_val_o), vals, (long)h->nnz);
+        float* vvar = reinterpret_cast<float*>(h->d_q2);
+        if (h->nvar > 0)
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(ceil_div((long)h->nvar * ld, 256)), tb, 0, h->s,
+                               (const double*)h->d_var_o, vvar, (long)h->nvar * ld);
+        SpmvArgs<float, float> a;
+        a.sh = sh; a.A = op_f32(h, transpose != 0); a.A.val = vals; a.A.var = vvar; a.x = xin; a.y = yout;
+        hipLaunchKernelGGL((k_spmv<float, float>), dim3(grid), dim3(BLOCK), 0, h->s, a);
+        hipLaunchKernelGGL((k_to_external<float>), dim3(ceil_div(lout, 64), ceil_div(ld, 64)), tb, 0, h->s,
+                           (const float*)yout, out, (int)lout, (int)B, (int)ld, (const double*)nullptr, 0);
+    }
+    HIPCHK(hipGetLastError());
+    return leave(h, stream);
+}
+
+int lpe_get_scaling(lpe_handle* h, double* row_scale, double* col_scale, double* sigma, double* tau) {
+    if (!h) return fail(LPE_EINVAL, "handle is required");
+    if (row_scale) std::copy(h->dr.begin(), h->dr.end(), row_scale);
+    if (col_scale) std::copy(h->dc.begin(), h->dc.end(), col_scale);
+    if (h->method == LPE_METHOD_FIXED) {
+        // per-LP metric: return LP 0's values
+        if (sigma && h->m > 0) {
+            for (int i = 0; i < h->m; ++i)
+                HIPCHK(hipMemcpy(sigma + i, h->d_sigref + (long)i * h->ld, sizeof(double), hipMemcpyDeviceToHost));
+        }
+        if (tau) {
+            for (int j = 0; j < h->n; ++j)
+                HIPCHK(hipMemcpy(tau + j, h->d_tauref + (long)j * h->ld, sizeof(double), hipMemcpyDeviceToHost));
+        }
+        return LPE_OK;
+    }
+    if (sigma) std::copy(h->sig.begin(), h->sig.end(), sigma);
+    if (tau) std::copy(h->tau.begin(), h->tau.end(), tau);
+    return LPE_OK;
+}
+
+int lpe_get_primal_weight(lpe_handle* h, double* omega) {
+    if (!h || !omega) return fail(LPE_EINVAL, "handle and omega are required");
+    HIPCHK(hipMemcpy(h->h_st, h->d_st, sizeof(LpState) * h->ld, hipMemcpyDeviceToHost));
+    for (int b = 0; b < h->B; ++b) omega[b] = h->h_st[b].omega;
+    return LPE_OK;
+}
+
+int lpe_set_launch(lpe_handle* h, int32_t rows_per_wave, int32_t lps_per_lane) {
+    if (!h) return fail(LPE_EINVAL, "handle is required");
+    if (rows_per_wave < 1 || rows_per_wave > 64) return fail(LPE_EINVAL, "rows_per_wave must be in [1, 64]");
+    if (lps_per_lane != 1 && lps_per_lane != 2 && lps_per_lane != 4) return fail(LPE_EINVAL, "lps_per_lane must be 1, 2 or 4");
+    h->rpw = rows_per_wave;
+    h->lpl = lps_per_lane;
+    return LPE_OK;
+}
+
+int lpe_time_iterations(lpe_handle* h, int64_t iterations, int32_t chunk, int32_t use_graph, double* ms_out,
+                        void* stream) {
+    if (!h || !ms_out) return fail(LPE_EINVAL, "handle and ms are required");
+    if (!h->ready) return fail(LPE_ESTATE, "reset is required after a problem update");
+    CHK(enter(h, stream));
+    HIPCHK(hipEventRecord(h->ev_a, h->s));
+    CHK(run_iterations(h, iterations, chunk, use_graph));
+    HIPCHK(hipEventRecord(h->ev_b, h->s));
+    HIPCHK(hipEventSynchronize(h->ev_b));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, h->ev_a, h->ev_b));
+    *ms_out = ms;
+    return leave(h, stream);
+}
+
+int lpe_time_kernel(lpe_handle* h, int32_t kind, int32_t reps, double* ms_out, void* stream) {
+    if (!h || !ms_out) return fail(LPE_EINVAL, "handle and ms are required");
+    if (kind != 0 && kind != 1) return fail(LPE_EINVAL, "kind must be 0 (dual step) or 1 (primal step)");
+    if (reps < 1) return fail(LPE_EINVAL, "reps must be positive");
+    if (!h->ready) return fail(LPE_ESTATE, "reset is required after a problem update");
+    if (kind == 0 && h->m == 0) return fail(LPE_EINVAL, "no rows");
+    CHK(enter(h, stream));
+    HIPCHK(hipEventRecord(h->ev_a, h->s));
+    for (int i = 0; i < reps; ++i) launch_step(h, kind == 0 ? 1 : 2);
+    HIPCHK(hipEventRecord(h->ev_b, h->s));
+    HIPCHK(hipEventSynchronize(h->ev_b));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, h->ev_a, h->ev_b));
+    *ms_out = ms / reps;
+    return leave(h, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,1444 @@
+// Device code of the MI355X batched LP engine.
+//
+// Data layout: every per-LP vector is stored batch-minor, element (j, b) of a
+// length-L vector at [j * ld + b] with ld = padded batch.  A 64-lane wavefront
+// therefore owns one matrix row (or column) for 64 * LPL consecutive LPs: the
+// row's structure and shared coefficients are wave-uniform (scalar loads),
+// and every vector access - including the gathers x[col][b] - is a contiguous
+// 256-2048 byte segment.  The shared matrix is read once per wave instead of
+// once per LP.
+//
+// Work decomposition: a block of 4 waves covers `rpw` rows per wave for one
+// batch tile of W = 64 * LPL LPs.  Blocks are remapped so that each XCD walks
+// a contiguous range of (tile, row-block) pairs; the gathered vectors of a
+// tile then stay in that XCD's L2.
+//
+// Floating point: compiled with -ffp-contract=off so that every product and
+// sum is rounded as written (the order of the reference's separate FP64
+// operations, src/gpu_pdhg_workspace.py:122-134 "--fmad=false").
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lpe {
+
+constexpr int WAVE = 64;
+constexpr int WAVES_PER_BLOCK = 4;
+constexpr int BLOCK = WAVE * WAVES_PER_BLOCK;
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+template <typename T, int N>
+__device__ __forceinline__ void load_vec(const T* __restrict__ p, T (&v)[N]) {
+    if constexpr (N == 1) {
+        v[0] = p[0];
+    } else {
+        struct alignas(sizeof(T) * N) Pack { T e[N]; };
+        Pack q = *reinterpret_cast<const Pack*>(p);
+#pragma unroll
+        for (int t = 0; t < N; ++t) v[t] = q.e[t];
+    }
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void store_vec(T* __restrict__ p, const T (&v)[N]) {
+    if constexpr (N == 1) {
+        p[0] = v[0];
+    } else {
+        struct alignas(sizeof(T) * N) Pack { T e[N]; };
+        Pack q;
+#pragma unroll
+        for (int t = 0; t < N; ++t) q.e[t] = v[t];
+        *reinterpret_cast<Pack*>(p) = q;
+    }
+}
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// XCD-aware block remap.  The grid is padded to a multiple of 8; block g is
+// dispatched to XCD g % 8, which receives the contiguous virtual range
+// [x * G/8, (x+1) * G/8).  Returns false for padding blocks.
+__device__ __forceinline__ bool map_block(int n_real, int n_rb, int& tile, int& rb) {
+    const int g = blockIdx.x;
+    const int G = gridDim.x;
+    const int v = (G % 8 == 0) ? (g % 8) * (G / 8) + g / 8 : g;
+    if (v >= n_real) return false;
+    tile = v / n_rb;
+    rb = v - tile * n_rb;
+    return true;
+}
+
+struct Shape {
+    int rows;      // rows of the operator this launch walks (m or n)
+    int ld;        // leading dimension (padded batch)
+    int batch;     // real batch
+    int rpw;       // rows per wave
+    int n_rb;      // row blocks
+    int n_tiles;
+    int n_real;    // n_tiles * n_rb
+};
+
+// Gather-accumulate one sparse row for LPL consecutive LPs.
+// acc[t] = sum_k a_k * x[idx_k][b0 + t], sequential in storage order.
+template <typename TV, typename TX, typename TA, int LPL>
+__device__ __forceinline__ void row_dot(const int* __restrict__ ptr, const int* __restrict__ idx,
+                                        const int* __restrict__ slot, const TV* __restrict__ val,
+                                        const TV* __restrict__ var, const TX* __restrict__ x,
+                                        long ld, int b0, int row, TA (&acc)[LPL]) {
+#pragma unroll
+    for (int t = 0; t < LPL; ++t) acc[t] = TA(0);
+    const int s = ptr[row], e = ptr[row + 1];
+    for (int k = s; k < e; ++k) {
+        const int col = idx[k];
+        const int sl = slot[k];
+        TX xv[LPL];
+        load_vec<TX, LPL>(x + (long)col * ld + b0, xv);
+        if (sl < 0) {
+            const TA a = (TA)val[k];
+#pragma unroll
+            for (int t = 0; t < LPL; ++t) acc[t] = acc[t] + a * (TA)xv[t];
+        } else {
+            TV vv[LPL];
+            load_vec<TV, LPL>(var + (long)sl * ld + b0, vv);
+#pragma unroll
+            for (int t = 0; t < LPL; ++t) acc[t] = acc[t] + (TA)vv[t] * (TA)xv[t];
+        }
+    }
+}
+
+// Sparse operator in one orientation (CSR of A, or CSR of A^T).
+template <typename TV>
+struct Op {
+    const int* ptr;
+    const int* idx;
+    const int* slot;
+    const TV* val;
+    const TV* var;   // [n_var][ld]
+};
+
+// ---------------------------------------------------------------------------
+// PDHG half steps
+// ---------------------------------------------------------------------------
+template <typename T>
+struct DualArgs {
+    Shape sh;
+    Op<T> A;
+    const T* xbar;       // [n][ld]
+    T* y;                // [m][ld]
+    const T* rhs;        // [m][ld]
+    const T* sigma;      // [m] shared, or [m][ld] when REF
+    const T* omega;      // [ld] primal weight (RESTARTED); unused when REF
+    const int* flags;    // [ld] nonzero = frozen
+    T* ysum;             // [m][ld] running sum (RESTARTED)
+    int neq;
+};
+
+// y+ = y + sigma*(rhs - A xbar), inequality rows clipped to <= 0, NaN passes,
+// frozen LPs keep y (src/gpu_pdhg_workspace.py:122-127; corrector :340-344).
+template <typename T, int LPL, bool REF>
+__global__ void __launch_bounds__(BLOCK) k_dual_step(DualArgs<T> a) {
+    int tile, rb;
+    if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int b0 = tile * WAVE * LPL + lane * LPL;
+    if (b0 >= a.sh.ld) return;
+    const long ld = a.sh.ld;
+    int fr[LPL];
+    T om[LPL];
+#pragma unroll
+    for (int t = 0; t < LPL; ++t) {
+        fr[t] = a.flags[b0 + t];
+        om[t] = REF ? T(1) : a.omega[b0 + t];
+    }
+    bool all_frozen = true;
+#pragma unroll
+    for (int t = 0; t < LPL; ++t) all_frozen = all_frozen && fr[t];
+    if (__all(all_frozen)) return;
+    const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
+    for (int r = 0; r < a.sh.rpw; ++r) {
+        const int row = uniform(r0 + r);
+        if (row >= a.sh.rows) break;
+        T ax[LPL];
+        row_dot<T, T, T, LPL>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.xbar, ld, b0, row, ax);
+        T yv[LPL], bv[LPL], sg[LPL];
+        load_vec<T, LPL>(a.y + row * ld + b0, yv);
+        load_vec<T, LPL>(a.rhs + row * ld + b0, bv);
+        if (REF) {
+            load_vec<T, LPL>(a.sigma + row * ld + b0, sg);
+        } else {
+            const T s = a.sigma[row];
+#pragma unroll
+            for (int t = 0; t < LPL; ++t) sg[t] = s * om[t];
+        }
+        const bool ineq = row >= a.neq;
+        T out[LPL];
+#pragma unroll
+        for (int t = 0; t < LPL; ++t) {
+            const T delta = sg[t] * (bv[t] - ax[t]);
+            const T cand = yv[t] + delta;
+            T nv = cand;
+            if (ineq && !__builtin_isnan(cand)) nv = __builtin_fmin(cand, T(0));
+            out[t] = fr[t] ? yv[t] : nv;
+        }
+        store_vec<T, LPL>(a.y + row * ld + b0, out);
+        if (!REF) {
+            T s[LPL];
+            load_vec<T, LPL>(a.ysum + row * ld + b0, s);
+#pragma unroll
+            for (int t = 0; t < LPL; ++t) s[t] = fr[t] ? s[t] : s[t] + out[t];
+            store_vec<T, LPL>(a.ysum + row * ld + b0, s);
+        }
+    }
+}
+
+template <typename T>
+struct PrimalArgs {
+    Shape sh;
+    Op<T> AT;
+    const T* y;          // [m][ld]
+    T* x;                // [n][ld]
+    T* xbar;             // [n][ld]
+    const T* c;          // [n] (c_shared) or [n][ld]
+    const T* lower;      // [n][ld]
+    const T* upper;      // [n][ld]
+    const T* tau;        // [n] shared, or [n][ld] when REF
+    const T* inv_omega;  // [ld]
+    const int* flags;
+    T* xsum;             // [n][ld]
+    T theta;
+    int c_shared;
+};
+
+// x+ = clip(x - tau*(c - A^T y+), l, u); xbar = x+ + theta*(x+ - x); frozen
+// LPs keep x (src/gpu_pdhg_workspace.py:128-134; corrector :345-357).
+template <typename T, int LPL, bool REF>
+__global__ void __launch_bounds__(BLOCK) k_primal_step(PrimalArgs<T> a) {
+    int tile, rb;
+    if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int b0 = tile * WAVE * LPL + lane * LPL;
+    if (b0 >= a.sh.ld) return;
+    const long ld = a.sh.ld;
+    int fr[LPL];
+    T iw[LPL];
+#pragma unroll
+    for (int t = 0; t < LPL; ++t) {
+        fr[t] = a.flags[b0 + t];
+        iw[t] = REF ? T(1) : a.inv_omega[b0 + t];
+    }
+    bool all_frozen = true;
+#pragma unroll
+    for (int t = 0; t < LPL; ++t) all_frozen = all_frozen && fr[t];
+    if (__all(all_frozen)) return;
+    const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
+    for (int r = 0; r < a.sh.rpw; ++r) {
+        const int col = uniform(r0 + r);
+        if (col >= a.sh.rows) break;
+        T aty[LPL];
+        row_dot<T, T, T, LPL>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.y, ld, b0, col, aty);
+        T xv[LPL], lo[LPL], hi[LPL], cv[LPL], tv[LPL];
+        load_vec<T, LPL>(a.x + col * ld + b0, xv);
+        load_vec<T, LPL>(a.lower + col * ld + b0, lo);
+        load_vec<T, LPL>(a.upper + col * ld + b0, hi);
+        if (a.c_shared) {
+            const T cc = a.c[col];
+#pragma unroll
+            for (int t = 0; t < LPL; ++t) cv[t] = cc;
+        } else {
+            load_vec<T, LPL>(a.c + col * ld + b0, cv);
+        }
+        if (REF) {
+            load_vec<T, LPL>(a.tau + col * ld + b0, tv);
+        } else {
+            const T tt = a.tau[col];
+#pragma unroll
+            for (int t = 0; t < LPL; ++t) tv[t] = tt * iw[t];
+        }
+        T nx[LPL], nb[LPL];
+#pragma unroll
+        for (int t = 0; t < LPL; ++t) {
+            const T step = tv[t] * (cv[t] - aty[t]);
+            const T cand = xv[t] - step;
+            T nxt;
+            if (fr[t]) {
+                nxt = xv[t];
+            } else if (__builtin_isnan(cand)) {
+                nxt = cand;
+            } else {
+                nxt = __builtin_fmin(__builtin_fmax(cand, lo[t]), hi[t]);
+            }
+            const T ext = a.theta * (nxt - xv[t]);
+            nx[t] = nxt;
+            nb[t] = nxt + ext;
+        }
+        store_vec<T, LPL>(a.x + col * ld + b0, nx);
+        store_vec<T, LPL>(a.xbar + col * ld + b0, nb);
+        if (!REF) {
+            T s[LPL];
+            load_vec<T, LPL>(a.xsum + col * ld + b0, s);
+#pragma unroll
+            for (int t = 0; t < LPL; ++t) s[t] = fr[t] ? s[t] : s[t] + nx[t];
+            store_vec<T, LPL>(a.xsum + col * ld + b0, s);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Block-level reduction of per-LP partial values.  Every wave holds NV values
+// for its lane's LPL LPs; the block combines the 4 waves in fixed order and
+// writes partial[(rb * NV + v) * ld + b].
+// ---------------------------------------------------------------------------
+enum RedOp { RSUM = 0, RMAX = 1 };
+
+template <int NV, int LPL>
+__device__ __forceinline__ void block_partials(double (&val)[NV][LPL], const int (&ops)[NV],
+                                               double* __restrict__ partial, int rb, long ld, int b0,
+                                               double* lds, bool valid) {
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    // lds: [WAVES][NV][64*LPL]
+    constexpr int W = WAVE * LPL;
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+        for (int t = 0; t < LPL; ++t) lds[(wave * NV + v) * W + lane * LPL + t] = val[v][t];
+    __syncthreads();
+    if (wave == 0 && valid) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+#pragma unroll
+            for (int t = 0; t < LPL; ++t) {
+                double acc = lds[(0 * NV + v) * W + lane * LPL + t];
+                for (int w = 1; w < WAVES_PER_BLOCK; ++w) {
+                    const double o = lds[(w * NV + v) * W + lane * LPL + t];
+                    acc = ops[v] == RSUM ? acc + o : (o > acc || __builtin_isnan(o) ? o : acc);
+                }
+                partial[((long)rb * NV + v) * ld + b0 + t] = acc;
+            }
+        }
+    }
+}
+
+// out[v * ld + b] = reduce_rb partial[(rb * NV + v) * ld + b] in rb order.
+__global__ void k_reduce_partials(const double* __restrict__ partial, double* __restrict__ out,
+                                  const int* __restrict__ ops, int nv, int n_rb, int ld) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = blockIdx.y;
+    if (b >= ld || v >= nv) return;
+    const int op = ops[v];
+    double acc = partial[(long)v * ld + b];
+    for (int r = 1; r < n_rb; ++r) {
+        const double o = partial[((long)r * nv + v) * ld + b];
+        acc = op == RSUM ? acc + o : (o > acc || __builtin_isnan(o) ? o : acc);
+    }
+    out[(long)v * ld + b] = acc;
+}
+
+
+// ---------------------------------------------------------------------------
+// Device checks of restarted PDHG (FP64 arithmetic on the scaled problem).
+// Current iterate (x, y) and average (xsum/cnt, ysum/cnt) are evaluated in
+// one pass; each pass writes NV block partials per LP.
+// ---------------------------------------------------------------------------
+enum CheckRowVal {
+    CR_RP2 = 0,    // sum of squared scaled primal residual
+    CR_PERR,       // max original-unit row error (eq |Ax-b|, ineq (Ax-b)+)
+    CR_BY,         // sum b^ . y^
+    CR_ROWGAP,     // sum |y (b - Ax)| over inequality rows
+    CR_YPOS,       // max original y over inequality rows
+    CR_DY2,        // sum (y - y_last)^2
+    CR_NONFIN,     // count of non-finite activity / y
+    CR_N
+};
+enum CheckColVal {
+    CC_RD2 = 0,    // sum of squared scaled dual residual
+    CC_DVIOL,      // max original dual violation on infinite bounds
+    CC_BOUNDTERM,  // sum of l^ r^+ - u^ r^- (finite bounds)
+    CC_POBJ,       // sum c^ x^
+    CC_BOUNDGAP,   // sum |r^ (x^ - target)|
+    CC_BVIOL,      // max original bound violation
+    CC_DX2,        // sum (x - x_last)^2
+    CC_NONFIN,
+    CC_N
+};
+
+template <typename T>
+struct CheckRowsArgs {
+    Shape sh;
+    Op<double> A;            // scaled FP64 values
+    const T* x;              // [n][ld]
+    const T* xsum;
+    const T* y;              // [m][ld]
+    const T* ysum;
+    const T* ylast;
+    const double* rhs;       // scaled FP64 [m][ld]
+    const double* dr;        // [m]
+    const int* flags;
+    const int* cnt;          // iterations summed in xsum/ysum before this chunk
+    int K;                   // iterations in this chunk
+    int neq;
+    int with_avg;
+    double* partial;         // [n_rb][2*CR_N][ld]
+};
+
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_check_rows(CheckRowsArgs<T> a) {
+    __shared__ double lds[WAVES_PER_BLOCK * 2 * CR_N * WAVE];
+    int tile, rb;
+    if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int b0 = tile * WAVE + lane;
+    const bool valid = b0 < a.sh.ld;
+    const long ld = a.sh.ld;
+    double v[2 * CR_N][1];
+    for (int i = 0; i < 2 * CR_N; ++i) v[i][0] = 0.0;
+    const bool active = valid && !a.flags[valid ? b0 : 0];
+    const double inv = active ? 1.0 / (double)(a.cnt[b0] + a.K) : 0.0;
+    if (active) {
+        const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
+        for (int r = 0; r < a.sh.rpw; ++r) {
+            const int row = uniform(r0 + r);
+            if (row >= a.sh.rows) break;
+            const bool ineq = row >= a.neq;
+            const double b = a.rhs[row * ld + b0];
+            const double d = a.dr[row];
+            for (int which = 0; which < (a.with_avg ? 2 : 1); ++which) {
+                double ax[1];
+                double yv;
+                if (which == 0) {
+                    row_dot<double, T, double, 1>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.x, ld, b0, row, ax);
+                    yv = (double)a.y[row * ld + b0];
+                } else {
+                    row_dot<double, T, double, 1>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.xsum, ld, b0, row, ax);
+                    ax[0] = ax[0] * inv;
+                    yv = (double)a.ysum[row * ld + b0] * inv;
+                }
+                double (*w)[1] = v + which * CR_N;
+                const double res = b - ax[0];
+                const double rp = ineq ? (res < 0.0 ? res : 0.0) : res;
+                w[CR_RP2][0] += rp * rp;
+                const double err = ineq ? -res : __builtin_fabs(res);
+                const double perr = err / d;
+                if (perr > w[CR_PERR][0] || __builtin_isnan(perr)) w[CR_PERR][0] = perr;
+                w[CR_BY][0] += b * yv;
+                if (ineq) {
+                    w[CR_ROWGAP][0] += __builtin_fabs(yv * res);
+                    const double yo = yv * d;
+                    if (yo > w[CR_YPOS][0] || __builtin_isnan(yo)) w[CR_YPOS][0] = yo;
+                }
+                const double dy = yv - (double)a.ylast[row * ld + b0];
+                w[CR_DY2][0] += dy * dy;
+                if (!__builtin_isfinite(ax[0]) || !__builtin_isfinite(yv)) w[CR_NONFIN][0] += 1.0;
+            }
+        }
+    }
+    const int ops[2 * CR_N] = {RSUM, RMAX, RSUM, RSUM, RMAX, RSUM, RSUM,
+                               RSUM, RMAX, RSUM, RSUM, RMAX, RSUM, RSUM};
+    block_partials<2 * CR_N, 1>(v, ops, a.partial, rb, ld, b0, lds, valid);
+}
+
+template <typename T>
+struct CheckColsArgs {
+    Shape sh;
+    Op<double> AT;           // scaled FP64
+    const T* x;
+    const T* xsum;
+    const T* xlast;
+    const T* y;
+    const T* ysum;
+    const double* c;         // scaled FP64, [n] or [n][ld]
+    const double* lower;     // scaled FP64 [n][ld]
+    const double* upper;
+    const double* dc;        // [n]
+    const int* flags;
+    const int* cnt;
+    int K;
+    int c_shared;
+    int with_avg;
+    double* partial;         // [n_rb][2*CC_N][ld]
+};
+
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_check_cols(CheckColsArgs<T> a) {
+    __shared__ double lds[WAVES_PER_BLOCK * 2 * CC_N * WAVE];
+    int tile, rb;
+    if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int b0 = tile * WAVE + lane;
+    const bool valid = b0 < a.sh.ld;
+    const long ld = a.sh.ld;
+    double v[2 * CC_N][1];
+    for (int i = 0; i < 2 * CC_N; ++i) v[i][0] = 0.0;
+    const bool active = valid && !a.flags[valid ? b0 : 0];
+    const double inv = active ? 1.0 / (double)(a.cnt[b0] + a.K) : 0.0;
+    if (active) {
+        const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
+        for (int r = 0; r < a.sh.rpw; ++r) {
+            const int col = uniform(r0 + r);
+            if (col >= a.sh.rows) break;
+            const double cc = a.c_shared ? a.c[col] : a.c[col * ld + b0];
+            const double lo = a.lower[col * ld + b0];
+            const double hi = a.upper[col * ld + b0];
+            const double d = a.dc[col];
+            for (int which = 0; which < (a.with_avg ? 2 : 1); ++which) {
+                double aty[1];
+                double xv;
+                if (which == 0) {
+                    row_dot<double, T, double, 1>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.y, ld, b0, col, aty);
+                    xv = (double)a.x[col * ld + b0];
+                } else {
+                    row_dot<double, T, double, 1>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.ysum, ld, b0, col, aty);
+                    aty[0] = aty[0] * inv;
+                    xv = (double)a.xsum[col * ld + b0] * inv;
+                }
+                double (*w)[1] = v + which * CC_N;
+                const double rc = cc - aty[0];
+                const bool lfin = __builtin_isfinite(lo), ufin = __builtin_isfinite(hi);
+                const double rdl = (!lfin && rc > 0.0) ? rc : 0.0;
+                const double rdu = (!ufin && rc < 0.0) ? -rc : 0.0;
+                w[CC_RD2][0] += rdl * rdl + rdu * rdu;
+                const double dv = (rdl > rdu ? rdl : rdu) / d;
+                if (dv > w[CC_DVIOL][0] || __builtin_isnan(dv)) w[CC_DVIOL][0] = dv;
+                double bt = 0.0;
+                if (rc > 0.0 && lfin) bt = lo * rc;
+                else if (rc < 0.0 && ufin) bt = hi * rc;
+                w[CC_BOUNDTERM][0] += bt;
+                w[CC_POBJ][0] += cc * xv;
+                const double tgt = rc >= 0.0 ? lo : hi;
+                if (__builtin_isfinite(tgt)) w[CC_BOUNDGAP][0] += __builtin_fabs(rc * (xv - tgt));
+                const double bv = ((lo - xv) > (xv - hi) ? (lo - xv) : (xv - hi)) * d;
+                if (bv > w[CC_BVIOL][0] || __builtin_isnan(bv)) w[CC_BVIOL][0] = bv;
+                const double dx = xv - (double)a.xlast[col * ld + b0];
+                w[CC_DX2][0] += dx * dx;
+                if (!__builtin_isfinite(rc) || !__builtin_isfinite(xv)) w[CC_NONFIN][0] += 1.0;
+            }
+        }
+    }
+    const int ops[2 * CC_N] = {RSUM, RMAX, RSUM, RSUM, RSUM, RMAX, RSUM, RSUM,
+                               RSUM, RMAX, RSUM, RSUM, RSUM, RMAX, RSUM, RSUM};
+    block_partials<2 * CC_N, 1>(v, ops, a.partial, rb, ld, b0, lds, valid);
+}
+
+
+// ---------------------------------------------------------------------------
+// Per-LP restart / acceptance decisions (one thread per LP).
+//
+// Restart rule (PDLP: Applegate et al., NeurIPS 2021, sec. 3; cuPDLP.jl, Lu &
+// Yang 2023): the candidate is the current iterate or the average, whichever
+// has the smaller weighted KKT error
+//     K_w(z) = sqrt(w^2 |r_p|^2 + |r_d|^2 / w^2 + gap^2);
+// restart when K(cand) <= 0.2 K(last restart), or K(cand) <= 0.8 K(last) and
+// K(cand) > K(previous candidate), or the restart length reaches 0.36 of the
+// iterations run.  The primal weight is smoothed at restarts:
+//     w <- exp(0.5 log(|dy|/|dx|) + 0.5 log w).
+// ---------------------------------------------------------------------------
+struct LpState {
+    double omega;        // primal weight
+    double kkt_last;     // weighted KKT at the last restart point
+    double kkt_prev;     // weighted KKT of the previous candidate
+    double rel_kkt;      // relative KKT of the better of (current, average)
+    double avg_div;      // divisor of the averages at the last check
+    long long iters;     // PDHG iterations since reset
+    long long accepted_iter;
+    int status;          // LPE_STATUS_*
+    int action;          // Action for k_check_apply
+    int restarts;
+    int source;          // 0: result from iterate, 1: from polish
+};
+
+enum Action { ACT_NONE = 0, ACT_RESTART_CUR = 1, ACT_RESTART_AVG = 2, ACT_ACCEPT_CUR = 3, ACT_ACCEPT_AVG = 4 };
+
+struct DecideArgs {
+    const double* rows;     // [2*CR_N][ld] reduced row values
+    const double* cols;     // [2*CC_N][ld]
+    LpState* st;
+    int* flags;
+    int* cnt;
+    double* omega64;
+    float* omega32;
+    double* inv_omega64;
+    float* inv_omega32;
+    const double* bnorm;    // [ld]
+    const double* cnorm;    // [ld]
+    int batch;
+    int ld;
+    int K;
+    int restarted;
+    int certify;
+    double eps_p, eps_d, eps_g;
+};
+
+__global__ void k_check_decide(DecideArgs a) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= a.batch) return;
+    LpState s = a.st[b];
+    s.action = ACT_NONE;
+    if (s.status != 0) { a.st[b] = s; return; }
+    s.iters += a.K;
+    const long ld = a.ld;
+    double om = s.omega;
+    double kkt[2] = {0.0, 0.0}, rel[2] = {0.0, 0.0};
+    bool pass[2] = {false, false};
+    const int nw = a.restarted ? 2 : 1;
+    for (int w = 0; w < nw; ++w) {
+        const double* R = a.rows + (long)w * CR_N * ld + b;
+        const double* C = a.cols + (long)w * CC_N * ld + b;
+        const double rp = sqrt(R[CR_RP2 * ld]);
+        const double rd = sqrt(C[CC_RD2 * ld]);
+        const double pobj = C[CC_POBJ * ld];
+        const double dobj = R[CR_BY * ld] + C[CC_BOUNDTERM * ld];
+        const double gap = fabs(pobj - dobj);
+        kkt[w] = sqrt(om * om * rp * rp + rd * rd / (om * om) + gap * gap);
+        double r1 = rp / (1.0 + a.bnorm[b]);
+        const double r2 = rd / (1.0 + a.cnorm[b]);
+        const double r3 = gap / (1.0 + fabs(pobj) + fabs(dobj));
+        r1 = r2 > r1 ? r2 : r1;
+        rel[w] = r3 > r1 ? r3 : r1;
+        const bool finite = R[CR_NONFIN * ld] == 0.0 && C[CC_NONFIN * ld] == 0.0;
+        double primal = fmax(R[CR_PERR * ld], C[CC_BVIOL * ld]);
+        primal = fmax(primal, 0.0);
+        double dual = fmax(R[CR_YPOS * ld], C[CC_DVIOL * ld]);
+        dual = fmax(dual, 0.0);
+        const double cg = (C[CC_BOUNDGAP * ld] + R[CR_ROWGAP * ld]) / fmax(1.0, fabs(pobj));
+        pass[w] = finite && isfinite(cg) && primal <= a.eps_p && dual <= a.eps_d && cg <= a.eps_g;
+    }
+    const int best = (nw == 2 && kkt[1] < kkt[0]) ? 1 : 0;
+    s.rel_kkt = rel[best];
+    s.avg_div = (double)(a.cnt[b] + a.K);
+    if (a.certify && (pass[0] || pass[1])) {
+        s.status = 1;
+        s.action = pass[0] ? ACT_ACCEPT_CUR : ACT_ACCEPT_AVG;
+        s.accepted_iter = s.iters;
+        a.flags[b] = 1;
+    } else if (a.restarted) {
+        const double kc = kkt[best];
+        bool restart = false;
+        if (!(s.kkt_last < INFINITY)) {
+            s.kkt_last = kc;
+        } else {
+            restart = kc <= 0.2 * s.kkt_last || (kc <= 0.8 * s.kkt_last && kc > s.kkt_prev) ||
+                      (double)(a.cnt[b] + a.K) >= 0.36 * (double)s.iters;
+        }
+        s.kkt_prev = kc;
+        if (restart) {
+            s.action = best ? ACT_RESTART_AVG : ACT_RESTART_CUR;
+            const double dx = sqrt(a.cols[(long)(best * CC_N + CC_DX2) * ld + b]);
+            const double dy = sqrt(a.rows[(long)(best * CR_N + CR_DY2) * ld + b]);
+            if (dx > 1e-10 && dy > 1e-10 && isfinite(dx) && isfinite(dy)) {
+                om = exp(0.5 * log(dy / dx) + 0.5 * log(om));
+                om = fmin(fmax(om, 1e-8), 1e8);
+            }
+            s.omega = om;
+            s.kkt_last = kc;
+            s.kkt_prev = INFINITY;
+            s.restarts += 1;
+            a.cnt[b] = 0;
+        } else {
+            a.cnt[b] += a.K;
+        }
+        a.omega64[b] = om;
+        a.omega32[b] = (float)om;
+        a.inv_omega64[b] = 1.0 / om;
+        a.inv_omega32[b] = (float)(1.0 / om);
+    }
+    a.st[b] = s;
+}
+
+// Apply restart / acceptance actions to one family of vectors.
+template <typename T>
+struct ApplyArgs {
+    int rows;
+    int ld;
+    int rows_per_block;
+    const LpState* st;
+    T* v;        // x or y
+    T* vsum;
+    T* vlast;
+    T* vbar;     // xbar, or nullptr for duals
+};
+
+template <typename T>
+__global__ void k_check_apply(ApplyArgs<T> a) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= a.ld) return;
+    const int act = a.st[b].action;
+    if (act == ACT_NONE) return;
+    const double inv = 1.0 / a.st[b].avg_div;
+    const long ld = a.ld;
+    const int r0 = blockIdx.y * a.rows_per_block;
+    const int r1 = min(a.rows, r0 + a.rows_per_block);
+    for (int r = r0; r < r1; ++r) {
+        const long i = r * ld + b;
+        T v = a.v[i];
+        if (act == ACT_RESTART_AVG || act == ACT_ACCEPT_AVG) {
+            v = (T)((double)a.vsum[i] * inv);
+            a.v[i] = v;
+        }
+        if (a.vbar) a.vbar[i] = v;
+        if (act == ACT_RESTART_AVG || act == ACT_RESTART_CUR) {
+            a.vlast[i] = v;
+            a.vsum[i] = T(0);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 certificate in original units: restates certify_blocks_device
+// (src/gpu_block_lp.py:52-132) and the host _certificate
+// (src/cpu_repeated_lp.py:150-178).
+// ---------------------------------------------------------------------------
+enum CertRowVal { XR_EQ = 0, XR_UB, XR_ROWGAP, XR_YUB, XR_NONFIN, XR_N };
+enum CertColVal { XC_LOW = 0, XC_UP, XC_RLOW, XC_RUP, XC_BOUNDGAP, XC_OBJ, XC_NONFIN_X, XC_NONFIN_R, XC_N };
+
+__device__ __forceinline__ double nmax(double acc, double o) {
+    return (o > acc || __builtin_isnan(o)) ? o : acc;
+}
+
+struct CertRowsArgs {
+    Shape sh;
+    Op<double> A;           // original FP64
+    const double* x;        // [n][ld]
+    const double* y;        // [m][ld]
+    const double* rhs;      // [m][ld]
+    int neq;
+    double* partial;        // [n_rb][XR_N][ld]
+};
+
+__global__ void __launch_bounds__(BLOCK) k_cert_rows(CertRowsArgs a) {
+    __shared__ double lds[WAVES_PER_BLOCK * XR_N * WAVE];
+    int tile, rb;
+    if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int b0 = tile * WAVE + lane;
+    const bool valid = b0 < a.sh.ld;
+    const long ld = a.sh.ld;
+    double v[XR_N][1] = {{0.0}, {-INFINITY}, {0.0}, {-INFINITY}, {0.0}};
+    if (valid) {
+        const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
+        for (int r = 0; r < a.sh.rpw; ++r) {
+            const int row = uniform(r0 + r);
+            if (row >= a.sh.rows) break;
+            double act[1];
+            row_dot<double, double, double, 1>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.x, ld, b0, row, act);
+            const double err = act[0] - a.rhs[row * ld + b0];
+            const double yv = a.y[row * ld + b0];
+            if (row < a.neq) {
+                v[XR_EQ][0] = nmax(v[XR_EQ][0], __builtin_fabs(err));
+            } else {
+                v[XR_UB][0] = nmax(v[XR_UB][0], err);
+                v[XR_ROWGAP][0] += __builtin_fabs(yv * (a.rhs[row * ld + b0] - act[0]));
+                v[XR_YUB][0] = nmax(v[XR_YUB][0], yv);
+            }
+            if (!__builtin_isfinite(act[0]) || !__builtin_isfinite(yv)) v[XR_NONFIN][0] += 1.0;
+        }
+    }
+    const int ops[XR_N] = {RMAX, RMAX, RSUM, RMAX, RSUM};
+    block_partials<XR_N, 1>(v, ops, a.partial, rb, ld, b0, lds, valid);
+}
+
+struct CertColsArgs {
+    Shape sh;
+    Op<double> AT;          // original FP64
+    const double* x;
+    const double* y;
+    const double* c;        // [n] or [n][ld]
+    const double* lower;
+    const double* upper;
+    int c_shared;
+    double* partial;        // [n_rb][XC_N][ld]
+};
+
+__global__ void __launch_bounds__(BLOCK) k_cert_cols(CertColsArgs a) {
+    __shared__ double lds[WAVES_PER_BLOCK * XC_N * WAVE];
+    int tile, rb;
+    if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int b0 = tile * WAVE + lane;
+    const bool valid = b0 < a.sh.ld;
+    const long ld = a.sh.ld;
+    double v[XC_N][1] = {{-INFINITY}, {-INFINITY}, {-INFINITY}, {-INFINITY}, {0.0}, {0.0}, {0.0}, {0.0}};
+    if (valid) {
+        const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
+        for (int r = 0; r < a.sh.rpw; ++r) {
+            const int col = uniform(r0 + r);
+            if (col >= a.sh.rows) break;
+            double aty[1];
+            row_dot<double, double, double, 1>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.y, ld, b0, col, aty);
+            const double cc = a.c_shared ? a.c[col] : a.c[col * ld + b0];
+            const double xv = a.x[col * ld + b0];
+            const double lo = a.lower[col * ld + b0], hi = a.upper[col * ld + b0];
+            const double rc = cc - aty[0];
+            v[XC_LOW][0] = nmax(v[XC_LOW][0], lo - xv);
+            v[XC_UP][0] = nmax(v[XC_UP][0], xv - hi);
+            v[XC_RLOW][0] = nmax(v[XC_RLOW][0], __builtin_isfinite(lo) ? 0.0 : rc);
+            v[XC_RUP][0] = nmax(v[XC_RUP][0], __builtin_isfinite(hi) ? 0.0 : -rc);
+            const double tgt = rc >= 0.0 ? lo : hi;
+            const double t2 = __builtin_isfinite(tgt) ? tgt : xv;
+            v[XC_BOUNDGAP][0] += __builtin_fabs(rc * (xv - t2));
+            v[XC_OBJ][0] += cc * xv;
+            if (!__builtin_isfinite(xv)) v[XC_NONFIN_X][0] += 1.0;
+            if (!__builtin_isfinite(rc)) v[XC_NONFIN_R][0] += 1.0;
+        }
+    }
+    const int ops[XC_N] = {RMAX, RMAX, RMAX, RMAX, RSUM, RSUM, RSUM, RSUM};
+    block_partials<XC_N, 1>(v, ops, a.partial, rb, ld, b0, lds, valid);
+}
+
+// metrics[b][LPE_NMETRIC] (batch-major, device) and pass flags.
+struct CertFinalArgs {
+    const double* rows;     // [XR_N][ld] reduced (nullptr when m == 0)
+    const double* cols;     // [XC_N][ld]
+    double* metrics;        // [B][8]
+    int* passed;            // [ld]
+    int batch, ld;
+    double eps_p, eps_d, eps_g;
+};
+
+__global__ void k_cert_final(CertFinalArgs a) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= a.batch) return;
+    const long ld = a.ld;
+    double row_eq = 0.0, row_ub = 0.0, row_gap = 0.0, dual_rows = 0.0, nonfin_rows = 0.0;
+    if (a.rows) {
+        row_eq = a.rows[XR_EQ * ld + b];
+        row_ub = a.rows[XR_UB * ld + b];
+        row_gap = a.rows[XR_ROWGAP * ld + b];
+        dual_rows = a.rows[XR_YUB * ld + b];
+        nonfin_rows = a.rows[XR_NONFIN * ld + b];
+        if (row_ub == -INFINITY) row_ub = 0.0;          // empty inequality group
+        if (dual_rows == -INFINITY) dual_rows = 0.0;
+    }
+    const double low = a.cols[XC_LOW * ld + b], up = a.cols[XC_UP * ld + b];
+    const double rlow = a.cols[XC_RLOW * ld + b], rup = a.cols[XC_RUP * ld + b];
+    const double bound_gap = a.cols[XC_BOUNDGAP * ld + b];
+    const double obj = a.cols[XC_OBJ * ld + b];
+    // primal = max(0, max(max(row_eq, row_ub), max(max(l-x), max(x-u))))
+    const double primal = nmax(0.0, nmax(nmax(row_eq, row_ub), nmax(low, up)));
+    const double dual = nmax(0.0, nmax(dual_rows, nmax(rlow, rup)));
+    const double gap = (bound_gap + row_gap) / fmax(1.0, fabs(obj));
+    const bool finite_primal = a.cols[XC_NONFIN_X * ld + b] == 0.0 && isfinite(obj) && nonfin_rows == 0.0;
+    const bool finite = finite_primal && a.cols[XC_NONFIN_R * ld + b] == 0.0;
+    const bool ok = finite && isfinite(gap) && primal <= a.eps_p && dual <= a.eps_d && gap <= a.eps_g;
+    double* m = a.metrics + (long)b * 8;
+    m[0] = primal; m[1] = dual; m[2] = gap; m[3] = obj; m[4] = ok ? 1.0 : 0.0;
+    m[5] = finite ? 1.0 : 0.0; m[6] = bound_gap; m[7] = row_gap;
+    if (a.passed) a.passed[b] = ok ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 active-set polish.
+//
+// With x from the iteration, variables within tol*max(1, u-l) of a bound are
+// fixed there (set N); the rest are free (F).  Inequality rows are active when
+// their dual is negative or their slack is small.  In the equilibrated space
+// (A^ = Dr A Dc) the polish solves two least-squares problems with the same
+// normal-equations operator M = R A^ D A^T R (R, D: 0/1 masks of active rows
+// and free columns) by conjugate gradients:
+//   primal: minimum-norm correction  A^_F dz = Dr (b - A x_N)  -> x_F += Dc dz
+//   dual:   min |D (c^ - A^T u)|, warm-started at u = y^        -> y = Dr u
+// A solution that identifies the optimal active set satisfies every KKT
+// condition to rounding; the FP64 certificate decides acceptance.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct PolishColsInitArgs {
+    Shape sh;
+    Op<double> AT;           // scaled FP64
+    const T* x;              // scaled iterate
+    const T* y;
+    const double* c;         // scaled c^ ([n] or [n][ld])
+    int c_shared;
+    const double* dc;
+    const double* lower;     // original [n][ld]
+    const double* upper;
+    const int* sel;          // [ld] selected LPs
+    double* xs;              // [n][ld] snapped original
+    uint8_t* fmask;          // [n][ld]
+    double tol, tol_far, tol_r;
+};
+
+// A variable is fixed at a bound when it lies within tol*max(1, u-l) of it, or
+// within tol_far*max(1, u-l) while its reduced cost r = c - A^T y points at
+// that bound by more than tol_r*(1 + |c|).
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_polish_cols_init(PolishColsInitArgs<T> a) {
+    int tile, rb;
+    if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int b0 = tile * WAVE + lane;
+    if (b0 >= a.sh.ld || !a.sel[b0]) return;
+    const long ld = a.sh.ld;
+    const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
+    for (int r = 0; r < a.sh.rpw; ++r) {
+        const int j = uniform(r0 + r);
+        if (j >= a.sh.rows) break;
+        const long i = j * ld + b0;
+        double aty[1];
+        row_dot<double, T, double, 1>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.y, ld, b0, j, aty);
+        const double d = a.dc[j];
+        const double cs = a.c_shared ? a.c[j] : a.c[i];
+        const double rc = (cs - aty[0]) / d;
+        const double co = cs / d;
+        const double thr = a.tol_r * (1.0 + fabs(co));
+        double xv = (double)a.x[i] * d;
+        const double lo = a.lower[i], hi = a.upper[i];
+        double w = hi - lo;
+        w = (isfinite(w) && w > 1.0) ? w : 1.0;
+        uint8_t free_ = 1;
+        if (xv - lo <= a.tol * w) { xv = lo; free_ = 0; }
+        else if (hi - xv <= a.tol * w) { xv = hi; free_ = 0; }
+        else if (rc > thr && xv - lo <= a.tol_far * w) { xv = lo; free_ = 0; }
+        else if (rc < -thr && hi - xv <= a.tol_far * w) { xv = hi; free_ = 0; }
+        a.xs[i] = xv;
+        a.fmask[i] = free_;
+    }
+}
+
+template <typename T>
+struct PolishRowsInitArgs {
+    Shape sh;
+    Op<double> A;            // original FP64 values
+    const double* xs;
+    const T* y;              // scaled dual iterate
+    const double* rhs;       // original [m][ld]
+    const double* dr;
+    const int* sel;
+    uint8_t* rmask;
+    double* rp; double* pp; double* wp;
+    double* wd;
+    int neq;
+    double tol_y, tol_s;
+    double* partial;         // [n_rb][1][ld]: sum rp^2
+};
+
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_polish_rows_init(PolishRowsInitArgs<T> a) {
+    __shared__ double lds[WAVES_PER_BLOCK * 1 * WAVE];
+    int tile, rb;
+    if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int b0 = tile * WAVE + lane;
+    const bool valid = b0 < a.sh.ld;
+    const long ld = a.sh.ld;
+    double v[1][1] = {{0.0}};
+    if (valid && a.sel[b0]) {
+        const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
+        for (int r = 0; r < a.sh.rpw; ++r) {
+            const int row = uniform(r0 + r);
+            if (row >= a.sh.rows) break;
+            double act[1];
+            row_dot<double, double, double, 1>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.xs, ld, b0, row, act);
+            const long i = row * ld + b0;
+            const double bb = a.rhs[i];
+            const double res = bb - act[0];
+            const double yv = (double)a.y[i];
+            uint8_t on = 1;
+            if (row >= a.neq) {
+                const double yo = yv * a.dr[row];
+                on = (yo < -a.tol_y || res <= a.tol_s * fmax(1.0, fabs(bb))) ? 1 : 0;
+            }
+            a.rmask[i] = on;
+            const double rv = on ? a.dr[row] * res : 0.0;
+            a.rp[i] = rv; a.pp[i] = rv; a.wp[i] = 0.0;
+            a.wd[i] = on ? yv : 0.0;
+            v[0][0] += rv * rv;
+        }
+    }
+    const int ops[1] = {RSUM};
+    block_partials<1, 1>(v, ops, a.partial, rb, ld, b0, lds, valid);
+}
+
+// q_j = fmask * (base_j - sum_i a^_ij p_i), with base = c^ (dual setup) or 0 and
+// sign flipped (CG products: q = D A^T p).  Two systems per launch.
+struct PolishColsArgs {
+    Shape sh;
+    Op<double> AT;           // scaled FP64
+    const double* p1; const double* p2;   // [m][ld]
+    double* q1; double* q2;               // [n][ld]
+    const double* c;         // scaled c^ for the dual setup, or nullptr
+    int c_shared;
+    const uint8_t* fmask;
+    const int* sel;
+};
+
+__global__ void __launch_bounds__(BLOCK) k_polish_cols(PolishColsArgs a) {
+    int tile, rb;
+    if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int b0 = tile * WAVE + lane;
+    if (b0 >= a.sh.ld || !a.sel[b0]) return;
+    const long ld = a.sh.ld;
+    const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
+    for (int r = 0; r < a.sh.rpw; ++r) {
+        const int col = uniform(r0 + r);
+        if (col >= a.sh.rows) break;
+        const long i = col * ld + b0;
+        const double f = a.fmask[i] ? 1.0 : 0.0;
+        double s1[1];
+        row_dot<double, double, double, 1>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.p1, ld, b0, col, s1);
+        if (a.c) {
+            const double cc = a.c_shared ? a.c[col] : a.c[i];
+            a.q1[i] = f * (cc - s1[0]);
+        } else {
+            a.q1[i] = f * s1[0];
+            if (a.p2) {
+                double s2[1];
+                row_dot<double, double, double, 1>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.p2, ld, b0, col, s2);
+                a.q2[i] = f * s2[0];
+            }
+        }
+    }
+}
+
+// s_i = rmask * sum_j a^_ij q_j for two systems; partial dots d1 = p1.s1,
+// d2 = p2.s2 (or |s1|^2 when p1 is nullptr, for the dual setup, which also
+// stores s1 into r1/pp1).
+struct PolishRowsArgs {
+    Shape sh;
+    Op<double> A;            // scaled FP64
+    const double* q1; const double* q2;
+    double* s1; double* s2;
+    const double* p1; const double* p2;
+    double* setup_r; double* setup_p;     // dual setup: r = p = s1
+    const uint8_t* rmask;
+    const int* sel;
+    double* partial;         // [n_rb][2][ld]
+};
+
+__global__ void __launch_bounds__(BLOCK) k_polish_rows(PolishRowsArgs a) {
+    __shared__ double lds[WAVES_PER_BLOCK * 2 * WAVE];
+    int tile, rb;
+    if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int b0 = tile * WAVE + lane;
+    const bool valid = b0 < a.sh.ld;
+    const long ld = a.sh.ld;
+    double v[2][1] = {{0.0}, {0.0}};
+    if (valid && a.sel[b0]) {
+        const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
+        for (int r = 0; r < a.sh.rpw; ++r) {
+            const int row = uniform(r0 + r);
+            if (row >= a.sh.rows) break;
+            const long i = row * ld + b0;
+            const double on = a.rmask[i] ? 1.0 : 0.0;
+            double t1[1];
+            row_dot<double, double, double, 1>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.q1, ld, b0, row, t1);
+            const double sv1 = on * t1[0];
+            if (a.setup_r) {
+                a.setup_r[i] = sv1; a.setup_p[i] = sv1;
+                v[0][0] += sv1 * sv1;
+            } else {
+                a.s1[i] = sv1;
+                v[0][0] += a.p1[i] * sv1;
+                double t2[1];
+                row_dot<double, double, double, 1>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.q2, ld, b0, row, t2);
+                const double sv2 = on * t2[0];
+                a.s2[i] = sv2;
+                v[1][0] += a.p2[i] * sv2;
+            }
+        }
+    }
+    const int ops[2] = {RSUM, RSUM};
+    block_partials<2, 1>(v, ops, a.partial, rb, ld, b0, lds, valid);
+}
+
+// Per-LP CG scalars.  mode 0: init rho from reduced [2][ld] (rho_p, rho_d);
+// mode 1: alpha = rho / pAp; mode 2: beta = rho_new / rho, rho = rho_new.
+struct CgScalarArgs {
+    const double* red;      // [2][ld]
+    double* rho;            // [2][ld]
+    double* alpha;          // [2][ld]
+    double* beta;           // [2][ld]
+    const int* sel;
+    int batch, ld, mode;
+};
+
+__global__ void k_cg_scalars(CgScalarArgs a) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= a.batch || !a.sel[b]) return;
+    for (int s = 0; s < 2; ++s) {
+        const long i = (long)s * a.ld + b;
+        const double v = a.red[i];
+        if (a.mode == 0) {
+            a.rho[i] = v;
+        } else if (a.mode == 1) {
+            const double r = a.rho[i];
+            a.alpha[i] = (v > 0.0 && isfinite(v) && r > 0.0) ? r / v : 0.0;
+        } else {
+            const double r = a.rho[i];
+            a.beta[i] = (r > 0.0 && isfinite(v)) ? v / r : 0.0;
+            a.rho[i] = v;
+        }
+    }
+}
+
+// w += alpha p; r -= alpha s (two systems); partial |r|^2.
+struct CgUpdateArgs {
+    Shape sh;
+    double* w1; double* r1; const double* p1; const double* s1;
+    double* w2; double* r2; const double* p2; const double* s2;
+    const double* alpha;    // [2][ld]
+    const int* sel;
+    double* partial;        // [n_rb][2][ld]
+};
+
+__global__ void __launch_bounds__(BLOCK) k_cg_update(CgUpdateArgs a) {
+    __shared__ double lds[WAVES_PER_BLOCK * 2 * WAVE];
+    int tile, rb;
+    if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int b0 = tile * WAVE + lane;
+    const bool valid = b0 < a.sh.ld;
+    const long ld = a.sh.ld;
+    double v[2][1] = {{0.0}, {0.0}};
+    if (valid && a.sel[b0]) {
+        const double al1 = a.alpha[b0], al2 = a.alpha[ld + b0];
+        const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
+        for (int r = 0; r < a.sh.rpw; ++r) {
+            const int row = r0 + r;
+            if (row >= a.sh.rows) break;
+            const long i = row * ld + b0;
+            a.w1[i] = a.w1[i] + al1 * a.p1[i];
+            const double n1 = a.r1[i] - al1 * a.s1[i];
+            a.r1[i] = n1;
+            v[0][0] += n1 * n1;
+            a.w2[i] = a.w2[i] + al2 * a.p2[i];
+            const double n2 = a.r2[i] - al2 * a.s2[i];
+            a.r2[i] = n2;
+            v[1][0] += n2 * n2;
+        }
+    }
+    const int ops[2] = {RSUM, RSUM};
+    block_partials<2, 1>(v, ops, a.partial, rb, ld, b0, lds, valid);
+}
+
+// p = r + beta p (two systems).
+struct CgDirArgs {
+    int rows, ld, rows_per_block;
+    double* p1; const double* r1; double* p2; const double* r2;
+    const double* beta;
+    const int* sel;
+};
+
+__global__ void k_cg_direction(CgDirArgs a) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= a.ld || !a.sel[b]) return;
+    const long ld = a.ld;
+    const double be1 = a.beta[b], be2 = a.beta[ld + b];
+    const int r0 = blockIdx.y * a.rows_per_block;
+    const int r1 = min(a.rows, r0 + a.rows_per_block);
+    for (int r = r0; r < r1; ++r) {
+        const long i = r * ld + b;
+        a.p1[i] = a.r1[i] + be1 * a.p1[i];
+        a.p2[i] = a.r2[i] + be2 * a.p2[i];
+    }
+}
+
+// x = clip(xs + Dc * fmask * A^T w_p, l, u) (original units).
+struct PolishFinalColsArgs {
+    Shape sh;
+    Op<double> AT;          // scaled FP64
+    const double* wp;
+    const double* xs;
+    const double* dc;
+    const double* lower; const double* upper;
+    const uint8_t* fmask;
+    const int* sel;
+    double* xout;
+};
+
+__global__ void __launch_bounds__(BLOCK) k_polish_final_cols(PolishFinalColsArgs a) {
+    int tile, rb;
+    if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int b0 = tile * WAVE + lane;
+    if (b0 >= a.sh.ld || !a.sel[b0]) return;
+    const long ld = a.sh.ld;
+    const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
+    for (int r = 0; r < a.sh.rpw; ++r) {
+        const int col = uniform(r0 + r);
+        if (col >= a.sh.rows) break;
+        const long i = col * ld + b0;
+        double xv = a.xs[i];
+        if (a.fmask[i]) {
+            double s[1];
+            row_dot<double, double, double, 1>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.wp, ld, b0, col, s);
+            xv = xv + a.dc[col] * s[0];
+            xv = fmin(fmax(xv, a.lower[i]), a.upper[i]);
+        }
+        a.xout[i] = xv;
+    }
+}
+
+// y = rmask * Dr * w_d, inequality rows clipped to <= 0.
+struct PolishFinalRowsArgs {
+    int rows, ld, rows_per_block, neq;
+    const double* wd;
+    const double* dr;
+    const uint8_t* rmask;
+    const int* sel;
+    double* yout;
+};
+
+__global__ void k_polish_final_rows(PolishFinalRowsArgs a) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= a.ld || !a.sel[b]) return;
+    const long ld = a.ld;
+    const int r0 = blockIdx.y * a.rows_per_block;
+    const int r1 = min(a.rows, r0 + a.rows_per_block);
+    for (int r = r0; r < r1; ++r) {
+        const long i = r * ld + b;
+        double yv = a.rmask[i] ? a.dr[r] * a.wd[i] : 0.0;
+        if (r >= a.neq) yv = fmin(yv, 0.0);
+        a.yout[i] = yv;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Layout conversion between the caller's batch-major [B][len] FP64 arrays and
+// the internal batch-minor [len][ld] arrays, with optional diagonal scaling:
+// mode 0: copy, 1: multiply by scale[j], 2: divide by scale[j].
+// ---------------------------------------------------------------------------
+template <typename TO>
+__global__ void __launch_bounds__(256) k_to_internal(const double* __restrict__ in, TO* __restrict__ out,
+                                                     int len, int batch, int ld, const double* __restrict__ scale,
+                                                     int mode, double fill) {
+    __shared__ double tile[64][65];
+    const int j0 = blockIdx.x * 64, b0 = blockIdx.y * 64;
+    const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
+    for (int k = ty; k < 64; k += 4) {
+        const int b = b0 + k, j = j0 + tx;
+        double v = fill;
+        if (b < batch && j < len) v = in[(long)b * len + j];
+        tile[k][tx] = v;
+    }
+    __syncthreads();
+    for (int k = ty; k < 64; k += 4) {
+        const int j = j0 + k, b = b0 + tx;
+        if (j < len && b < ld) {
+            double v = tile[tx][k];
+            if (b < batch) {
+                if (mode == 1) v = v * scale[j];
+                else if (mode == 2) v = v / scale[j];
+            }
+            out[(long)j * ld + b] = (TO)v;
+        }
+    }
+}
+
+template <typename TI>
+__global__ void __launch_bounds__(256) k_to_external(const TI* __restrict__ in, double* __restrict__ out,
+                                                     int len, int batch, int ld, const double* __restrict__ scale,
+                                                     int mode) {
+    __shared__ double tile[64][65];
+    const int j0 = blockIdx.x * 64, b0 = blockIdx.y * 64;
+    const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
+    for (int k = ty; k < 64; k += 4) {
+        const int j = j0 + k, b = b0 + tx;
+        double v = 0.0;
+        if (j < len && b < ld) v = (double)in[(long)j * ld + b];
+        tile[k][tx] = v;
+    }
+    __syncthreads();
+    for (int k = ty; k < 64; k += 4) {
+        const int b = b0 + k, j = j0 + tx;
+        if (b < batch && j < len) {
+            double v = tile[tx][k];
+            if (mode == 1) v = v * scale[j];
+            else if (mode == 2) v = v / scale[j];
+            out[(long)b * len + j] = v;
+        }
+    }
+}
+
+// Elementwise over [rows][ld]: dst = src * scale[row] (mode 1) or copy, per
+// selected LP (sel == nullptr: all).  TI -> double.
+template <typename TI>
+__global__ void k_scale_copy(const TI* __restrict__ src, double* __restrict__ dst, int rows, int ld,
+                             int rows_per_block, const double* __restrict__ scale, const int* __restrict__ sel,
+                             int sel_value) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= ld) return;
+    if (sel && sel[b] != sel_value) return;
+    const long l = ld;
+    const int r0 = blockIdx.y * rows_per_block;
+    const int r1 = min(rows, r0 + rows_per_block);
+    for (int r = r0; r < r1; ++r) {
+        double v = (double)src[r * l + b];
+        if (scale) v = v * scale[r];
+        dst[r * l + b] = v;
+    }
+}
+
+// Generic batched SpMV: out[r][b] = sum a_rk in[idx][b].
+template <typename TV, typename TA>
+struct SpmvArgs {
+    Shape sh;
+    Op<TV> A;
+    const TA* x;
+    TA* y;
+};
+
+template <typename TV, typename TA>
+__global__ void __launch_bounds__(BLOCK) k_spmv(SpmvArgs<TV, TA> a) {
+    int tile, rb;
+    if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int b0 = tile * WAVE + lane;
+    if (b0 >= a.sh.ld) return;
+    const long ld = a.sh.ld;
+    const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
+    for (int r = 0; r < a.sh.rpw; ++r) {
+        const int row = uniform(r0 + r);
+        if (row >= a.sh.rows) break;
+        TA s[1];
+        row_dot<TV, TA, TA, 1>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.x, ld, b0, row, s);
+        a.y[row * ld + b0] = s[0];
+    }
+}
+
+// Input validation of the caller's batch-major arrays (src/gpu_pdhg_workspace.py:185-207,
+// :230-244): counts violations into err[0..3].
+__global__ void k_validate(const double* __restrict__ c, long nc, const double* __restrict__ rhs, long nr,
+                           const double* __restrict__ lower, const double* __restrict__ upper, long nb,
+                           const double* __restrict__ var, long nv, int* err) {
+    const long stride = (long)gridDim.x * blockDim.x;
+    const long t0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int e_c = 0, e_r = 0, e_b = 0, e_v = 0;
+    for (long i = t0; i < nc; i += stride) e_c += !isfinite(c[i]);
+    for (long i = t0; i < nr; i += stride) e_r += !isfinite(rhs[i]);
+    for (long i = t0; i < nb; i += stride) {
+        const double lo = lower[i], hi = upper[i];
+        e_b += (isnan(lo) || isnan(hi) || lo > hi || lo == INFINITY || hi == -INFINITY);
+    }
+    for (long i = t0; i < nv; i += stride) e_v += !isfinite(var[i]);
+    if (e_c) atomicAdd(err + 0, e_c);
+    if (e_r) atomicAdd(err + 1, e_r);
+    if (e_b) atomicAdd(err + 2, e_b);
+    if (e_v) atomicAdd(err + 3, e_v);
+}
+
+// Per-LP values of the variable coefficients, scaled: out[s][b] = in * dr[row_s] * dc[col_s].
+template <typename TO>
+__global__ void k_var_scale(const double* __restrict__ in, TO* __restrict__ out, int nvar, int ld,
+                            const int* __restrict__ vrow, const int* __restrict__ vcol,
+                            const double* __restrict__ dr, const double* __restrict__ dc) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (b >= ld || s >= nvar) return;
+    const double v = in[(long)s * ld + b];
+    out[(long)s * ld + b] = (TO)(v * dr[vrow[s]] * dc[vcol[s]]);
+}
+
+// Fixed-metric per-LP diagonal metric (src/gpu_pdhg_corrector.py:185-196):
+// out[r][b] = (sum_k |a_rk| > 0 ? eta / sum : 1) * mul   (mul = w for sigma,
+// 1/w for tau, applied as in the reference: base * w or base / w).
+struct RefMetricArgs {
+    Shape sh;
+    Op<double> A;
+    double* out;
+    double eta;
+    double weight;
+    int divide;      // 0: base * weight, 1: base / weight
+};
+
+__global__ void __launch_bounds__(BLOCK) k_ref_metric(RefMetricArgs a) {
+    int tile, rb;
+    if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int b0 = tile * WAVE + lane;
+    if (b0 >= a.sh.ld) return;
+    const long ld = a.sh.ld;
+    const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
+    for (int r = 0; r < a.sh.rpw; ++r) {
+        const int row = uniform(r0 + r);
+        if (row >= a.sh.rows) break;
+        double s = 0.0;
+        const int k0 = a.A.ptr[row], k1 = a.A.ptr[row + 1];
+        for (int k = k0; k < k1; ++k) {
+            const int sl = a.A.slot[k];
+            const double v = sl < 0 ? a.A.val[k] : a.A.var[(long)sl * ld + b0];
+            s = s + __builtin_fabs(v);
+        }
+        const double base = s > 0.0 ? a.eta / s : 1.0;
+        a.out[row * ld + b0] = a.divide ? base / a.weight : base * a.weight;
+    }
+}
+
+// Per-LP 2-norm of an internal [rows][ld] FP64 vector (or of a shared [rows]
+// vector when shared != 0).
+__global__ void k_norms(const double* __restrict__ v, int rows, int ld, int shared, double* __restrict__ out) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= ld) return;
+    double s = 0.0;
+    for (int r = 0; r < rows; ++r) {
+        const double x = shared ? v[r] : v[(long)r * ld + b];
+        s += x * x;
+    }
+    out[b] = sqrt(s);
+}
+
+// Project a starting point: x = clip(x, l, u) (scaled), y <= 0 on inequality rows.
+template <typename T>
+__global__ void k_project(T* __restrict__ x, T* __restrict__ xbar, T* __restrict__ xlast, T* __restrict__ xsum,
+                          const T* __restrict__ lower, const T* __restrict__ upper, int rows, int ld,
+                          int rows_per_block, int is_dual, int neq) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= ld) return;
+    const long l = ld;
+    const int r0 = blockIdx.y * rows_per_block;
+    const int r1 = min(rows, r0 + rows_per_block);
+    for (int r = r0; r < r1; ++r) {
+        const long i = r * l + b;
+        T v = x[i];
+        if (is_dual) {
+            if (r >= neq) v = __builtin_fmin(v, T(0));
+        } else {
+            v = __builtin_fmin(__builtin_fmax(v, lower[i]), upper[i]);
+        }
+        x[i] = v;
+        if (xbar) xbar[i] = v;
+        if (xlast) xlast[i] = v;
+        if (xsum) xsum[i] = T(0);
+    }
+}
+
+// Masked copy: dst[r][b] = src[r][b] for LPs with mask[b] == value.
+template <typename T>
+__global__ void k_masked_copy_t(const T* __restrict__ src, T* __restrict__ dst, int rows, int ld,
+                                int rows_per_block, const int* __restrict__ mask, int value) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= ld || mask[b] != value) return;
+    const long l = ld;
+    const int r0 = blockIdx.y * rows_per_block;
+    const int r1 = min(rows, r0 + rows_per_block);
+    for (int r = r0; r < r1; ++r) dst[r * l + b] = src[r * l + b];
+}
+
+__global__ void k_masked_copy(const double* __restrict__ src, double* __restrict__ dst, int rows, int ld,
+                              int rows_per_block, const int* __restrict__ mask, int value) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= ld || mask[b] != value) return;
+    const long l = ld;
+    const int r0 = blockIdx.y * rows_per_block;
+    const int r1 = min(rows, r0 + rows_per_block);
+    for (int r = r0; r < r1; ++r) dst[r * l + b] = src[r * l + b];
+}
+
+template <typename TI, typename TO>
+__global__ void k_convert(const TI* __restrict__ in, TO* __restrict__ out, long count) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) out[i] = (TO)in[i];
+}
+
+// Mark polished LPs optimal and frozen.
+__global__ void k_accept_polished(LpState* st, int* flags, int* source, const int* sel, int ld) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= ld || !sel[b]) return;
+    st[b].status = 1;
+    st[b].source = 1;
+    st[b].accepted_iter = st[b].iters;
+    flags[b] = 1;
+    source[b] = 1;
+}
+
+}  // namespace lpe

@@ -1,0 +1,140 @@
+"""The reference's LP backend surfaces served by the MI355X engine.
+
+* ``solve(c, **linprog_kwargs)`` - the per-LP backend called by the
+  community solver (src/community_solver.py:1200-1205): returns
+  ``SimpleNamespace(success, x, fun, message, ...)``; failure is
+  ``success=False`` (the caller aborts that environment's step).
+* ``solve_batch(requests, environment_ids=None)`` - the batched backend
+  (src/cpu_repeated_lp.py:783-818): one unique stable environment ID per
+  request, calls serialised, results in request order with ``diagnostics``
+  holding the certificate (``certificate_passed``, ``primal_residual``,
+  ``dual_violation``, ``relative_kkt_gap``, ``stage``, ...).
+
+Requests with the same sparsity pattern, equality count and stage are solved
+as one shared-pattern batch; per-LP coefficient differences become per-LP
+matrix entries.  Every answer is the engine's FP64 certificate; an LP that
+does not certify within the budget is reported ``success=False`` (no CPU
+fallback).  With ``warm_start`` each environment's previous primal/dual of
+the same stage seeds its next solve (src/gpu_resident_reduced_pdhg.py:104-118).
+"""
+
+from __future__ import annotations
+
+import time
+from types import SimpleNamespace
+
+import numpy as np
+import torch
+
+from . import _lib
+from .corrector import shared_pattern
+from .engine import BatchedLpEngine, SolveOptions
+from .requests import normalize, pattern_key
+
+
+class GpuLpBackend:
+    name = "mi355x_batched_lp"
+
+    def __init__(self, *, precision: str = "fp32", polish: bool = True, max_iterations: int = 20000,
+                 check_interval: int = 64, warm_start: bool = True, launch: tuple | None = None):
+        self.precision = precision
+        self.method = f"restarted_averaged_pdhg_{precision}" + ("_polish" if polish else "")
+        self.options = SolveOptions(max_iterations=max_iterations, check_interval=check_interval, polish=polish)
+        if launch:
+            self.options.rows_per_wave, self.options.lps_per_lane = launch
+        self.warm_start = bool(warm_start)
+        self._engines: dict = {}
+        self._warm: dict = {}
+        self.history: list[dict] = []
+        self.closed = False
+
+    # ---------------------------------------------------------------- engines
+    def _engine_for(self, key, template, varying, batch, shared_c):
+        cache_key = (key, tuple(varying.tolist()), batch, shared_c)
+        eng = self._engines.get(cache_key)
+        if eng is None:
+            a, neq = template
+            eng = BatchedLpEngine(a, neq, batch, var_positions=varying, precision=self.precision,
+                                  shared_c=shared_c)
+            self._engines[cache_key] = eng
+        return eng
+
+    def _solve_group(self, problems, env_ids, stage):
+        a0, _, _, _, c0, neq = problems[0]
+        template, varying, values = shared_pattern([p[0] for p in problems])
+        B = len(problems)
+        shared_c = all(np.array_equal(p[4], c0) for p in problems)
+        key = pattern_key(template, neq)
+        eng = self._engine_for(key, (template, neq), varying, B, shared_c)
+        c = c0 if shared_c else np.stack([p[4] for p in problems])
+        eng.update(c, np.stack([p[1] for p in problems]), np.stack([p[2] for p in problems]),
+                   np.stack([p[3] for p in problems]), var_values=values if varying.size else None)
+        x0 = y0 = None
+        if self.warm_start:
+            prev = [self._warm.get((e, stage, key)) for e in env_ids]
+            if all(p is not None for p in prev):
+                x0 = np.stack([p[0] for p in prev]); y0 = np.stack([p[1] for p in prev])
+        eng.reset(x0, y0)
+        info = eng.solve(self.options)
+        x, y, status = eng.state()
+        metrics = eng.certify(x, y).cpu().numpy()
+        x, y = x.cpu().numpy(), y.cpu().numpy()
+        out = []
+        for b, (p, e) in enumerate(zip(problems, env_ids)):
+            ok = bool(status[b] == _lib.STATUS_OPTIMAL and metrics[b, 4] == 1.0)
+            diag = dict(certificate_passed=ok, primal_residual=float(metrics[b, 0]),
+                        dual_violation=float(metrics[b, 1]), relative_kkt_gap=float(metrics[b, 2]),
+                        objective=float(metrics[b, 3]), status=_lib.STATUS_NAMES[int(status[b])],
+                        stage=stage, iterations=int(info["iterations"]), warm_started=x0 is not None,
+                        cpu_solver_runs=0, numerical_retry_count=0)
+            if ok and self.warm_start:
+                self._warm[(e, stage, key)] = (x[b].copy(), y[b].copy())
+            out.append(SimpleNamespace(success=ok, x=x[b].copy() if ok else None,
+                                       fun=float(metrics[b, 3]) if ok else None, nit=int(info["iterations"]),
+                                       diagnostics=diag,
+                                       message="optimal" if ok else "original LP certificate failed: "
+                                       + diag["status"]))
+        return out, info
+
+    # ------------------------------------------------------------- interfaces
+    def solve_batch(self, requests, *, environment_ids=None):
+        if self.closed:
+            raise RuntimeError("The batched LP backend is closed")
+        requests = list(requests)
+        if environment_ids is None:
+            environment_ids = list(range(len(requests)))
+        environment_ids = list(environment_ids)
+        if len(environment_ids) != len(requests) or len(set(environment_ids)) != len(environment_ids):
+            raise ValueError("One unique stable environment ID is required per request")
+        started = time.perf_counter()
+        problems = [normalize(c, kw) for c, kw in requests]      # validate everything first
+        stages = [kw.get("_stage") for _, kw in requests]
+        groups: dict = {}
+        for i, (p, st) in enumerate(zip(problems, stages)):
+            groups.setdefault((p[0].shape, p[5], st), []).append(i)
+        results = [None] * len(requests)
+        infos = []
+        for (_, _, stage), idx in groups.items():
+            res, info = self._solve_group([problems[i] for i in idx], [environment_ids[i] for i in idx], stage)
+            infos.append(info)
+            for i, r in zip(idx, res):
+                results[i] = r
+        self.history.append(dict(batch=len(requests), groups=len(groups), cpu_lp_calls=0,
+                                 seconds=time.perf_counter() - started, solve_info=infos,
+                                 rows=[r.diagnostics for r in results]))
+        return results
+
+    def solve(self, c, **kwargs):
+        """Per-LP backend (src/community_solver.py:1200-1205)."""
+        return self.solve_batch([(c, kwargs)], environment_ids=[kwargs.pop("_environment_id", 0)])[0]
+
+    def clear_models(self):
+        for eng in self._engines.values():
+            eng.close()
+        self._engines.clear()
+        self._warm.clear()
+
+    def close(self):
+        if not self.closed:
+            self.clear_models()
+            self.closed = True

@@ -1,0 +1,129 @@
+"""Device-resident dFBA rollout of many environments on the batched engine.
+
+Each step mirrors the reference's dFBA step for the cooperative consortium
+(src/dfba_simulator.py:1618 step -> :646-666 uptake bounds -> community LP ->
+:1326-1435 Euler medium update), restated on synthetic data:
+
+* uptake bounds  lb = max(-Vmax*C/(Km+C), -1000) on each species' medium
+  exchanges (src/dfba_simulator.py:646-666);
+* medium rows    sum_s -biomass_s * v_ex(s,i) <= C_i/dt
+  (src/community_solver.py:1120-1149); growth rows t - mu_k <= 0 (:1114-1119);
+* Euler update   C_i <- max(0, C_i + sum_s v_ex(s,i) * biomass_s * dt)
+  (src/dfba_simulator.py:1352,1425-1427);
+  biomass_s <- biomass_s * (1 + GROWTH_SCALE * v_bio_s * dt).
+
+All per-environment state is a torch tensor on the GPU; the LPs of one step
+are one batched solve.  Warm starts: "previous" (the last step's primal and
+dual), "none" (cold) or a callable returning (x0, y0) (the learned model).
+"""
+
+from __future__ import annotations
+
+import time
+from typing import Callable, Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+from .engine import BatchedLpEngine, SolveOptions
+from .workloads import BOUND, Consortium, consortium
+
+GROWTH_SCALE = 1e-4
+
+
+class ConsortiumRollout:
+    def __init__(self, model: Optional[Consortium] = None, n_env: int = 4096, *, seed: int = 2,
+                 precision: str = "fp32", options: Optional[SolveOptions] = None,
+                 warm_start="previous", dt: float = 1.0, conc_range=(1e-2, 10.0),
+                 device: Optional[torch.device] = None):
+        self.model = model if model is not None else consortium(seed)
+        md = self.model
+        self.n_env = int(n_env)
+        self.dt = float(dt)
+        self.device = device or torch.device("cuda", torch.cuda.current_device())
+        self.options = options or SolveOptions()
+        self.warm_start = warm_start
+        self.engine = BatchedLpEngine(md.A, md.neq, self.n_env, var_positions=md.coupling_nnz.ravel(),
+                                      precision=precision, shared_c=True, device=self.device)
+        rng = np.random.default_rng(seed + 7)
+        lo, hi = conc_range
+        dev = self.device
+        self.conc = torch.as_tensor(np.exp(rng.uniform(np.log(lo), np.log(hi), size=(self.n_env, md.medium_cols.shape[1]))),
+                                    device=dev)
+        self.biomass = torch.as_tensor(rng.uniform(0.05, 0.5, size=(self.n_env, md.medium_cols.shape[0])), device=dev)
+        self._lower0 = torch.as_tensor(md.lower, device=dev)
+        self._upper0 = torch.as_tensor(md.upper, device=dev)
+        self._rhs_species = torch.as_tensor(md.rhs_species, device=dev)
+        self._medium_cols = torch.as_tensor(md.medium_cols, device=dev, dtype=torch.long)
+        self._biomass_cols = torch.as_tensor(md.biomass_cols, device=dev, dtype=torch.long)
+        self._vmax = torch.as_tensor(md.vmax, device=dev)
+        self._km = torch.as_tensor(md.km, device=dev)
+        self.c = md.c.copy()
+        self.x = None
+        self.y = None
+        self.step_index = 0
+        self.log: list[dict] = []
+
+    def lp_inputs(self, conc=None, biomass=None):
+        """(rhs [B,m], lower [B,n], upper [B,n], coupling values [B,3*K]) on the device."""
+        md = self.model
+        conc = self.conc if conc is None else conc
+        biomass = self.biomass if biomass is None else biomass
+        B = conc.shape[0]
+        lower = self._lower0.expand(B, -1).clone()
+        for k in range(md.medium_cols.shape[0]):
+            mm = -self._vmax[k] * conc / (self._km[k] + conc)
+            lower[:, self._medium_cols[k]] = torch.clamp(mm, min=-BOUND)
+        upper = self._upper0.expand(B, -1).contiguous()
+        rhs = torch.zeros((B, md.shape[0]), dtype=torch.float64, device=self.device)
+        rhs[:, :md.neq] = self._rhs_species
+        rhs[:, md.neq:md.neq + conc.shape[1]] = conc / self.dt
+        coupling = -torch.clamp(biomass, min=1e-12)[:, :, None].expand(-1, -1, conc.shape[1]).reshape(B, -1)
+        return rhs, lower, upper, coupling.contiguous()
+
+    def euler(self, x: torch.Tensor, mask: Optional[torch.Tensor] = None):
+        """Medium and biomass update from the solved fluxes (skipped for LPs
+        that did not certify, whose environment keeps its state)."""
+        md = self.model
+        v_ex = x[:, self._medium_cols]                       # [B, 3, K]
+        delta = (v_ex * self.biomass[:, :, None]).sum(dim=1) * self.dt
+        conc = torch.clamp(self.conc + delta, min=0.0)
+        mu = x[:, self._biomass_cols] * GROWTH_SCALE
+        biomass = self.biomass * (1.0 + mu * self.dt)
+        if mask is not None:
+            conc = torch.where(mask[:, None], conc, self.conc)
+            biomass = torch.where(mask[:, None], biomass, self.biomass)
+        self.conc, self.biomass = conc, biomass
+
+    def step(self, *, advance: bool = True) -> dict:
+        rhs, lower, upper, coupling = self.lp_inputs()
+        eng = self.engine
+        t0 = time.perf_counter()
+        eng.update(self.c, rhs, lower, upper, var_values=coupling)
+        x0 = y0 = None
+        if callable(self.warm_start):
+            x0, y0 = self.warm_start(self.conc, lower, self.biomass)
+        elif self.warm_start == "previous" and self.x is not None:
+            x0, y0 = self.x, self.y
+        eng.reset(x0, y0)
+        info = eng.solve(self.options)
+        x, y, status = eng.state()
+        certified = status == _lib.STATUS_OPTIMAL
+        torch.cuda.synchronize()
+        info["wall_seconds"] = time.perf_counter() - t0
+        info["step"] = self.step_index
+        self.log.append(info)
+        if advance:
+            ok = torch.as_tensor(certified, device=self.device)
+            self.x = torch.where(ok[:, None], x, self.x) if self.x is not None else x
+            self.y = torch.where(ok[:, None], y, self.y) if self.y is not None else y
+            self.euler(x, ok)
+            self.step_index += 1
+        return dict(info, certified_mask=certified, x=x, y=y)
+
+    def snapshot(self):
+        return self.conc.clone(), self.biomass.clone()
+
+    def restore(self, snap):
+        self.conc, self.biomass = snap[0].clone(), snap[1].clone()

@@ -1,0 +1,22 @@
+"""CPU oracle of the batched LP engine - TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this package, and only as the checker: the engine in
+``co-cultivation-metabolic-atlas_amd/lpe`` never calls it.
+
+Modules (each function cites the reference file:line it restates):
+
+* ``spmv``        - CSR products in the engine's summation order.
+* ``certificate`` - the reference's original-LP certificate
+  (src/gpu_block_lp.py:52-132, src/cpu_repeated_lp.py:150-178).
+* ``pdhg``        - fixed-metric PDHG of the reference
+  (src/gpu_pdhg_corrector.py:185-196,296-358; tests/test_gpu_pdhg_workspace.py:35-50)
+  and the restarted-averaged PDHG the engine runs (published PDLP method; the
+  reference has none).
+* ``polish``      - the FP64 active-set polish (no reference counterpart).
+* ``highs``       - HiGHS FP64 through SciPy, the reference's exact CPU path
+  (src/cpu_repeated_lp.py:287-296, src/offline_scipy_teacher.py:32-40).
+* ``mlp``         - the warm-start network's forward/backward in FP64.
+
+Parity status of each function is recorded in its docstring ("pinned by").
+"""

@@ -1,0 +1,150 @@
+"""GPU parity of the restarted-averaged PDHG, polish and certificate.
+
+The oracle is HiGHS FP64 (the reference's exact CPU path) plus the CPU
+restatements in oracle/.  Bar: every certified LP passes the reference's
+certificate (primal <= 1e-5, dual <= 1e-7, relative KKT gap <= 1e-7,
+src/gpu_block_lp.py:84) and is within 1e-6 relative objective error of HiGHS.
+"""
+
+import numpy as np
+import pytest
+
+from lpe import workloads
+from oracle import highs, certificate as cert, pdhg as opdhg
+
+pytestmark = pytest.mark.gpu
+
+REL_OBJ_TOL = 1e-6
+
+
+def _engine(A, neq, batch, **kw):
+    from lpe.engine import BatchedLpEngine
+    return BatchedLpEngine(A, neq, batch, **kw)
+
+
+def _uptake(batch=8, m=120, n=180, seed=3):
+    return workloads.uptake_batch(batch=batch, seed=seed, m=m, n=n)
+
+
+def _load_uptake(eng, ub):
+    eng.update(ub.c, np.tile(ub.rhs, (ub.batch, 1)), ub.lower, ub.upper)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_restarted_iterates_match_oracle_before_restarts(device, precision):
+    ub = _uptake(batch=3)
+    eng = _engine(ub.S, ub.neq, ub.batch, precision=precision, shared_c=True)
+    _load_uptake(eng, ub)
+    eng.reset()
+    eng.run(50)
+    x, y, _ = eng.state()
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    for b in range(ub.batch):
+        r = opdhg.restarted_pdhg(*ub.problem(b), iterations=50, check_interval=50, precision=precision,
+                                 certify=False)
+        np.testing.assert_allclose(x.cpu().numpy()[b], r["x"], rtol=tol, atol=tol * 1e3)
+        np.testing.assert_allclose(y.cpu().numpy()[b], r["y"], rtol=tol, atol=tol * 1e3)
+
+
+def test_restarted_fp64_bit_exact_single_chunk(device):
+    ub = _uptake(batch=2)
+    eng = _engine(ub.S, ub.neq, ub.batch, precision="fp64", shared_c=True)
+    _load_uptake(eng, ub)
+    eng.reset()
+    eng.run(30, chunk=10, use_graph=True)
+    gx, gy, _ = eng.state()
+    eng.reset()
+    eng.run(30, use_graph=False)
+    lx, ly, _ = eng.state()
+    assert bool((gx == lx).all()) and bool((gy == ly).all())
+    r = opdhg.restarted_pdhg(*ub.problem(0), iterations=30, check_interval=30, certify=False)
+    np.testing.assert_array_equal(r["x_scaled"], r["x_scaled"])  # oracle determinism
+    np.testing.assert_allclose(gx.cpu().numpy()[0], r["x"], rtol=1e-13, atol=1e-10)
+
+
+@pytest.mark.parametrize("precision,polish", [("fp64", False), ("fp64", True), ("fp32", True)])
+def test_solve_small_batch_against_highs(device, precision, polish):
+    ub = _uptake(batch=8)
+    eng = _engine(ub.S, ub.neq, ub.batch, precision=precision, shared_c=True)
+    _load_uptake(eng, ub)
+    eng.reset()
+    info = eng.solve(max_iterations=40000, check_interval=64, polish=polish)
+    x, y, status = eng.state()
+    x, y = x.cpu().numpy(), y.cpu().numpy()
+    assert info["certified"] == ub.batch, info
+    for b in range(ub.batch):
+        p = ub.problem(b)
+        ref = highs.solve(*p)
+        c = cert.certificate(*p, x[b], y[b])
+        assert status[b] == 1
+        assert c["certificate_passed"], c
+        assert cert.relative_objective_error(c["objective"], ref.fun) <= REL_OBJ_TOL
+
+
+def test_species_lp_full_size_fp32_polish(device):
+    p = workloads.species_lp(0)
+    A, rhs, lo, hi, c, neq = p
+    eng = _engine(A, neq, 2, precision="fp32", shared_c=True)
+    eng.update(c, np.tile(rhs, (2, 1)), np.tile(lo, (2, 1)), np.tile(hi, (2, 1)))
+    eng.reset()
+    info = eng.solve(max_iterations=20000, check_interval=64, polish=True)
+    x, y, status = eng.state()
+    ref = highs.solve(*p)
+    for b in range(2):
+        cc = cert.certificate(*p, x.cpu().numpy()[b], y.cpu().numpy()[b])
+        assert cc["certificate_passed"], (cc, info)
+        assert cert.relative_objective_error(cc["objective"], ref.fun) <= REL_OBJ_TOL
+
+
+def test_consortium_batch_per_lp_coefficients(device):
+    model = workloads.consortium(seed=4, shapes=((120, 180), (150, 220), (90, 130)), n_medium=20)
+    cb = workloads.consortium_batch(batch=6, seed=4, model=model)
+    md = cb.model
+    eng = _engine(md.A, md.neq, cb.batch, precision="fp32", shared_c=True,
+                  var_positions=md.coupling_nnz.ravel())
+    eng.update(md.c, cb.rhs(), cb.lower(), cb.upper(), var_values=cb.coupling_values().reshape(cb.batch, -1))
+    eng.reset()
+    info = eng.solve(max_iterations=40000, check_interval=64, polish=True)
+    x, y, status = eng.state()
+    assert info["certified"] == cb.batch, info
+    for b in range(cb.batch):
+        p = cb.problem(b)
+        ref = highs.solve(*p)
+        c = cert.certificate(*p, x.cpu().numpy()[b], y.cpu().numpy()[b])
+        assert c["certificate_passed"], c
+        assert cert.relative_objective_error(c["objective"], ref.fun) <= REL_OBJ_TOL
+
+
+def test_device_certificate_matches_oracle(device):
+    ub = _uptake(batch=4)
+    eng = _engine(ub.S, ub.neq, ub.batch, shared_c=True)
+    _load_uptake(eng, ub)
+    xs, ys = [], []
+    for b in range(ub.batch):
+        r = highs.solve(*ub.problem(b))
+        xs.append(r.x); ys.append(r.y)
+    xs, ys = np.array(xs), np.array(ys)
+    xs[1, 5] += 1e-3  # breaks only LP 1 (tests/test_gpu_block_lp.py:51-52)
+    metrics = eng.certify(xs, ys).cpu().numpy()
+    for b in range(ub.batch):
+        ref = cert.certificate(*ub.problem(b), xs[b], ys[b])
+        assert bool(metrics[b, 4]) == ref["certificate_passed"]
+        assert metrics[b, 0] == ref["primal_residual"]
+        assert metrics[b, 1] == ref["dual_violation"]
+        np.testing.assert_allclose(metrics[b, 2], ref["relative_kkt_gap"], rtol=1e-10, atol=1e-18)
+        np.testing.assert_allclose(metrics[b, 3], ref["objective"], rtol=1e-13)
+    assert metrics[0, 4] == 1.0 and metrics[1, 4] == 0.0
+
+
+def test_nonfinite_and_infeasible_fail_closed(device):
+    from scipy.sparse import csr_matrix
+    # x <= 0 and x >= 1 (tests/test_gpu_pdhg_corrector.py:98-112)
+    A = csr_matrix(np.array([[1.0], [-1.0]]))
+    eng = _engine(A, 0, 1, shared_c=True)
+    eng.update(np.zeros(1), np.array([[0.0, -1.0]]), np.full((1, 1), -np.inf), np.full((1, 1), np.inf))
+    eng.reset()
+    info = eng.solve(max_iterations=640, check_interval=64, polish=True)
+    _, _, status = eng.state()
+    assert info["certified"] == 0 and status[0] != 1
+    m = eng.certify(np.full((1, 1), np.nan), np.full((1, 2), -1.0)).cpu().numpy()
+    assert m[0, 4] == 0.0
