@@ -11,9 +11,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -126,7 +128,7 @@ struct lpe_handle {
     double *d_rd = nullptr, *d_pd = nullptr, *d_wd = nullptr;
     double *d_q1 = nullptr, *d_q2 = nullptr, *d_s1 = nullptr, *d_s2 = nullptr;
     uint8_t *d_fmask = nullptr, *d_rmask = nullptr;
-    double *d_rho = nullptr, *d_alpha = nullptr, *d_beta = nullptr;
+    double *d_rho = nullptr, *d_alpha = nullptr, *d_beta = nullptr, *d_invdiag = nullptr;
     double *d_xpol = nullptr, *d_ypol = nullptr, *d_xres = nullptr, *d_yres = nullptr;
     double* d_metrics = nullptr;
     double *d_xin = nullptr, *d_yin = nullptr;    // user iterates for lpe_certify / lpe_spmv
@@ -141,6 +143,11 @@ struct lpe_handle {
     hipStream_t s = nullptr;
     hipEvent_t ev_sync = nullptr, ev_a = nullptr, ev_b = nullptr;
     hipGraphExec_t graph = nullptr;
+    std::map<int, hipGraphExec_t> cg_graphs;
+    // compaction buffers of the polish
+    int* d_perm = nullptr; int* d_csel = nullptr;
+    double *d_clo = nullptr, *d_cup = nullptr, *d_crhs = nullptr, *d_cc = nullptr, *d_ccs = nullptr;
+    double *d_cvar_o = nullptr, *d_cvar_s = nullptr, *d_cx = nullptr, *d_cy = nullptr;
     int graph_K = -1, graph_rpw = -1, graph_lpl = -1;
 
     int rpw = 1, lpl = 1;
@@ -359,13 +366,19 @@ int run_iterations(lpe_handle* h, long iters, int chunk, int use_graph) {
     return LPE_OK;
 }
 
-void reduce(lpe_handle* h, const double* partial, double* out, const int* ops, int nv, int n_rb) {
-    hipLaunchKernelGGL(k_reduce_partials, dim3(ceil_div(h->ld, 256), nv), dim3(256), 0, h->s, partial, out, ops,
-                       nv, n_rb, h->ld);
+void reduce_ld(lpe_handle* h, const double* partial, double* out, const int* ops, int nv, int n_rb, int ld) {
+    hipLaunchKernelGGL(k_reduce_partials, dim3(ceil_div(ld, 64), nv), dim3(256), 0, h->s, partial, out, ops,
+                       nv, n_rb, ld);
 }
 
-constexpr int CHECK_RPW = 4;
-constexpr int POLISH_RPW = 4;
+void reduce(lpe_handle* h, const double* partial, double* out, const int* ops, int nv, int n_rb) {
+    reduce_ld(h, partial, out, ops, nv, n_rb, h->ld);
+}
+
+// Rows per wave of the check / certificate passes: 128 rows per block keeps
+// the per-block partial sums (NV doubles per LP and block) small.
+constexpr int CHECK_RPW = 32;
+constexpr int POLISH_RPW = 16;
 
 // Device check: KKT / certificate values of (current, average), decisions, actions.
 template <typename T>
@@ -424,29 +437,72 @@ int launch_check(lpe_handle* h, int K, int certify, double ep, double ed, double
     return launch_check_t<double>(h, K, certify, ep, ed, eg);
 }
 
-// FP64 certificate of internal original-unit (x, y) into d_metrics / d_passed.
-int launch_certificate(lpe_handle* h, const double* x, const double* y, double ep, double ed, double eg) {
+// A set of LP columns: the whole batch, or the selected LPs gathered into a
+// dense prefix of width ld (the polish's compacted view).
+struct View {
+    int ld;
+    const double* c;       // original c ([n] shared or [n][ld])
+    const double* c_s;     // scaled c^
+    const double* rhs;     // original [m][ld]
+    const double* lo;      // original [n][ld]
+    const double* up;
+    const double* var_o;   // per-LP coefficients [nvar][ld], original
+    const double* var_s;   // scaled
+    const int* sel;        // [ld]
+};
+
+View full_view(lpe_handle* h) {
+    View v;
+    v.ld = h->ld; v.c = h->d_c_o; v.c_s = h->d_c_s; v.rhs = h->d_rhs_o; v.lo = h->d_lo_o; v.up = h->d_up_o;
+    v.var_o = h->d_var_o; v.var_s = h->d_var_s; v.sel = h->d_sel;
+    return v;
+}
+
+Op<double> op_orig_v(lpe_handle* h, bool transpose, const View& v) {
+    Op<double> o = op_orig(h, transpose);
+    o.var = v.var_o;
+    return o;
+}
+
+Op<double> op_scaled_v(lpe_handle* h, bool transpose, const View& v) {
+    Op<double> o = op_scaled(h, transpose);
+    o.var = v.var_s;
+    return o;
+}
+
+// FP64 certificate of original-unit (x, y) of a view into d_metrics / d_passed
+// (entries [0, ld) of the view's LP order).
+int launch_certificate_v(lpe_handle* h, const View& v, const double* x, const double* y, double ep, double ed,
+                         double eg) {
     int grid_r = 0, grid_c = 0;
-    Shape sr = make_shape(std::max(h->m, 1), h->ld, h->B, CHECK_RPW, 1, &grid_r);
-    Shape sc = make_shape(h->n, h->ld, h->B, CHECK_RPW, 1, &grid_c);
+    Shape sr = make_shape(std::max(h->m, 1), v.ld, v.ld, CHECK_RPW, 1, &grid_r);
+    Shape sc = make_shape(h->n, v.ld, v.ld, CHECK_RPW, 1, &grid_c);
     if (h->m > 0) {
         CertRowsArgs r;
-        r.sh = sr; r.A = op_orig(h, false); r.x = x; r.y = y; r.rhs = h->d_rhs_o; r.neq = h->neq;
+        r.sh = sr; r.A = op_orig_v(h, false, v); r.x = x; r.y = y; r.rhs = v.rhs; r.neq = h->neq;
         r.partial = h->d_partial;
         hipLaunchKernelGGL(k_cert_rows, dim3(grid_r), dim3(BLOCK), 0, h->s, r);
-        reduce(h, h->d_partial, h->d_red_a, h->d_ops_xr, XR_N, sr.n_rb);
+        reduce_ld(h, h->d_partial, h->d_red_a, h->d_ops_xr, XR_N, sr.n_rb, v.ld);
     }
     CertColsArgs c;
-    c.sh = sc; c.AT = op_orig(h, true); c.x = x; c.y = y; c.c = h->d_c_o; c.lower = h->d_lo_o;
-    c.upper = h->d_up_o; c.c_shared = h->shared_c; c.partial = h->d_partial;
+    c.sh = sc; c.AT = op_orig_v(h, true, v); c.x = x; c.y = y; c.c = v.c; c.lower = v.lo;
+    c.upper = v.up; c.c_shared = h->shared_c; c.partial = h->d_partial;
     hipLaunchKernelGGL(k_cert_cols, dim3(grid_c), dim3(BLOCK), 0, h->s, c);
-    reduce(h, h->d_partial, h->d_red_b, h->d_ops_xc, XC_N, sc.n_rb);
+    reduce_ld(h, h->d_partial, h->d_red_b, h->d_ops_xc, XC_N, sc.n_rb, v.ld);
     CertFinalArgs f;
     f.rows = h->m > 0 ? h->d_red_a : nullptr; f.cols = h->d_red_b; f.metrics = h->d_metrics;
-    f.passed = h->d_passed; f.batch = h->B; f.ld = h->ld; f.eps_p = ep; f.eps_d = ed; f.eps_g = eg;
-    hipLaunchKernelGGL(k_cert_final, dim3(ceil_div(h->B, 256)), dim3(256), 0, h->s, f);
+    f.passed = h->d_passed; f.batch = std::min(v.ld, h->ld); f.ld = v.ld; f.eps_p = ep; f.eps_d = ed; f.eps_g = eg;
+    hipLaunchKernelGGL(k_cert_final, dim3(ceil_div(f.batch, 256)), dim3(256), 0, h->s, f);
     HIPCHK(hipGetLastError());
     return LPE_OK;
+}
+
+int launch_certificate(lpe_handle* h, const double* x, const double* y, double ep, double ed, double eg) {
+    View v = full_view(h);
+    int grid_r = 0, grid_c = 0;
+    (void)grid_r; (void)grid_c;
+    // the full view covers ld LPs; metrics of padding LPs are not read
+    return launch_certificate_v(h, v, x, y, ep, ed, eg);
 }
 
 // Original-unit FP64 copy of the current iterate into (xo, yo).
@@ -481,99 +537,255 @@ int build_results(lpe_handle* h) {
     return LPE_OK;
 }
 
-// One polish attempt for the LPs in h->d_sel (device, already uploaded).
+constexpr int CG_BLOCK = 25;
+
+void cg_steps(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, int grid_r, int grid_c, int steps) {
+    CgScalarArgs cs;
+    cs.red = h->d_red_a; cs.rho = h->d_rho; cs.alpha = h->d_alpha; cs.beta = h->d_beta; cs.sel = v.sel;
+    cs.batch = v.ld; cs.ld = v.ld;
+    for (int k = 0; k < steps; ++k) {
+        PolishColsArgs c1;
+        c1.sh = sc; c1.AT = op_scaled_v(h, true, v); c1.p1 = h->d_pp; c1.p2 = h->d_pd; c1.q1 = h->d_q1;
+        c1.q2 = h->d_q2; c1.c = nullptr; c1.c_shared = 0; c1.fmask = h->d_fmask; c1.sel = v.sel;
+        hipLaunchKernelGGL(k_polish_cols, dim3(grid_c), dim3(BLOCK), 0, h->s, c1);
+        PolishRowsArgs c2;
+        c2.sh = sr; c2.A = op_scaled_v(h, false, v); c2.q1 = h->d_q1; c2.q2 = h->d_q2; c2.s1 = h->d_s1; c2.s2 = h->d_s2;
+        c2.p1 = h->d_pp; c2.p2 = h->d_pd; c2.setup_r = nullptr; c2.setup_p = nullptr; c2.invdiag = h->d_invdiag;
+        c2.rmask = h->d_rmask; c2.sel = v.sel; c2.partial = h->d_partial;
+        hipLaunchKernelGGL(k_polish_rows, dim3(grid_r), dim3(BLOCK), 0, h->s, c2);
+        reduce_ld(h, h->d_partial, h->d_red_a, h->d_ops_sum, 2, sr.n_rb, v.ld);
+        cs.mode = 1;
+        hipLaunchKernelGGL(k_cg_scalars, dim3(ceil_div(v.ld, 256)), dim3(256), 0, h->s, cs);
+        CgUpdateArgs u;
+        u.sh = sr; u.w1 = h->d_wp; u.r1 = h->d_rp_; u.p1 = h->d_pp; u.s1 = h->d_s1;
+        u.w2 = h->d_wd; u.r2 = h->d_rd; u.p2 = h->d_pd; u.s2 = h->d_s2; u.alpha = h->d_alpha;
+        u.invdiag = h->d_invdiag; u.sel = v.sel; u.partial = h->d_partial;
+        hipLaunchKernelGGL(k_cg_update, dim3(grid_r), dim3(BLOCK), 0, h->s, u);
+        reduce_ld(h, h->d_partial, h->d_red_a, h->d_ops_sum, 2, sr.n_rb, v.ld);
+        cs.mode = 2;
+        hipLaunchKernelGGL(k_cg_scalars, dim3(ceil_div(v.ld, 256)), dim3(256), 0, h->s, cs);
+        CgDirArgs dd;
+        dd.rows = h->m; dd.ld = v.ld; dd.rows_per_block = ROWS_PER_BLOCK_EW; dd.p1 = h->d_pp; dd.r1 = h->d_rp_;
+        dd.p2 = h->d_pd; dd.r2 = h->d_rd; dd.beta = h->d_beta; dd.invdiag = h->d_invdiag; dd.sel = v.sel;
+        hipLaunchKernelGGL(k_cg_direction, ew_grid(h->m, v.ld), dim3(256), 0, h->s, dd);
+    }
+}
+
+// CG_BLOCK preconditioned-CG steps replayed from a hipGraph captured once per
+// view width (the kernels read the selection and all vectors from fixed
+// buffers, so the graph is reused across attempts).
+int cg_block(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, int grid_r, int grid_c) {
+    const int key = v.ld + (v.sel == h->d_sel ? 0 : 1 << 30);
+    hipGraphExec_t& g_exec = h->cg_graphs[key];
+    if (!g_exec) {
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(h->s, hipStreamCaptureModeThreadLocal));
+        cg_steps(h, v, sr, sc, grid_r, grid_c, CG_BLOCK);
+        HIPCHK(hipStreamEndCapture(h->s, &g));
+        hipError_t e = hipGraphInstantiate(&g_exec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) { g_exec = nullptr; return fail(LPE_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)); }
+    }
+    HIPCHK(hipGraphLaunch(g_exec, h->s));
+    return LPE_OK;
+}
+
+// One polish round over a view.  Input: T iterates in scaled units
+// (orig == 0) or FP64 (x, y) in original units (orig == 1, the previous
+// round's result), both in the view's column order.  CG runs in blocks of
+// CG_BLOCK steps and stops once every selected LP's preconditioned residual
+// has dropped by 1e-10 (rho by 1e-20).  Output: d_xpol / d_ypol (view order).
 template <typename T>
-int polish_t(lpe_handle* h, int cg_iters, double tol) {
-    Iter<T>& it = iter_of<T>(h);
-    const long ld = h->ld;
+int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const T* yin, int orig, int cg_iters,
+             double tol) {
+    const long ld = v.ld;
     int grid_r = 0, grid_c = 0;
-    Shape sr = make_shape(std::max(h->m, 1), h->ld, h->B, POLISH_RPW, 1, &grid_r);
-    Shape sc = make_shape(h->n, h->ld, h->B, POLISH_RPW, 1, &grid_c);
+    Shape sr = make_shape(std::max(h->m, 1), v.ld, v.ld, POLISH_RPW, 1, &grid_r);
+    Shape sc = make_shape(h->n, v.ld, v.ld, POLISH_RPW, 1, &grid_c);
     {
         PolishColsInitArgs<T> a;
-        a.sh = sc; a.AT = op_scaled(h, true); a.x = it.x; a.y = it.y; a.c = h->d_c_s; a.c_shared = h->shared_c;
-        a.dc = h->d_dc; a.lower = h->d_lo_o; a.upper = h->d_up_o; a.sel = h->d_sel; a.xs = h->d_xs;
+        a.sh = sc; a.AT = orig ? op_orig_v(h, true, v) : op_scaled_v(h, true, v); a.x = xin; a.y = yin;
+        a.c = orig ? v.c : v.c_s; a.c_shared = h->shared_c; a.orig = orig;
+        a.dc = h->d_dc; a.lower = v.lo; a.upper = v.up; a.sel = v.sel; a.xs = h->d_xs;
         a.fmask = h->d_fmask; a.tol = tol; a.tol_far = 1e-2; a.tol_r = 1e-5;
         hipLaunchKernelGGL((k_polish_cols_init<T>), dim3(grid_c), dim3(BLOCK), 0, h->s, a);
     }
     if (h->m > 0) {
         PolishRowsInitArgs<T> a;
-        a.sh = sr; a.A = op_orig(h, false); a.xs = h->d_xs; a.y = it.y; a.rhs = h->d_rhs_o; a.dr = h->d_dr;
-        a.sel = h->d_sel; a.rmask = h->d_rmask; a.rp = h->d_rp_; a.pp = h->d_pp; a.wp = h->d_wp; a.wd = h->d_wd;
+        a.sh = sr; a.A = op_orig_v(h, false, v); a.As = op_scaled_v(h, false, v); a.xs = h->d_xs; a.y = yin;
+        a.rhs = v.rhs; a.dr = h->d_dr; a.orig = orig; a.fmask = h->d_fmask; a.invdiag = h->d_invdiag;
+        a.sel = v.sel; a.rmask = h->d_rmask; a.rp = h->d_rp_; a.pp = h->d_pp; a.wp = h->d_wp; a.wd = h->d_wd;
         a.neq = h->neq; a.tol_y = 1e-9; a.tol_s = 1e-7; a.partial = h->d_partial;
         hipLaunchKernelGGL((k_polish_rows_init<T>), dim3(grid_r), dim3(BLOCK), 0, h->s, a);
-        // rho_p into red_a[0]
-        reduce(h, h->d_partial, h->d_red_a, h->d_ops_sum, 1, sr.n_rb);
-        // dual setup: q1 = D (c^ - A^T wd); r_d = p_d = R A q1
+        reduce_ld(h, h->d_partial, h->d_red_a, h->d_ops_sum, 1, sr.n_rb, v.ld);
+        // dual setup: q1 = D (c^ - A^T wd); r_d = R A q1, p_d = r_d / diag
         PolishColsArgs pc;
-        pc.sh = sc; pc.AT = op_scaled(h, true); pc.p1 = h->d_wd; pc.p2 = nullptr; pc.q1 = h->d_q1; pc.q2 = nullptr;
-        pc.c = h->d_c_s; pc.c_shared = h->shared_c; pc.fmask = h->d_fmask; pc.sel = h->d_sel;
+        pc.sh = sc; pc.AT = op_scaled_v(h, true, v); pc.p1 = h->d_wd; pc.p2 = nullptr; pc.q1 = h->d_q1; pc.q2 = nullptr;
+        pc.c = v.c_s; pc.c_shared = h->shared_c; pc.fmask = h->d_fmask; pc.sel = v.sel;
         hipLaunchKernelGGL(k_polish_cols, dim3(grid_c), dim3(BLOCK), 0, h->s, pc);
         PolishRowsArgs pr;
-        pr.sh = sr; pr.A = op_scaled(h, false); pr.q1 = h->d_q1; pr.q2 = nullptr; pr.s1 = nullptr; pr.s2 = nullptr;
-        pr.p1 = nullptr; pr.p2 = nullptr; pr.setup_r = h->d_rd; pr.setup_p = h->d_pd; pr.rmask = h->d_rmask;
-        pr.sel = h->d_sel; pr.partial = h->d_partial;
+        pr.sh = sr; pr.A = op_scaled_v(h, false, v); pr.q1 = h->d_q1; pr.q2 = nullptr; pr.s1 = nullptr; pr.s2 = nullptr;
+        pr.p1 = nullptr; pr.p2 = nullptr; pr.setup_r = h->d_rd; pr.setup_p = h->d_pd; pr.invdiag = h->d_invdiag;
+        pr.rmask = h->d_rmask; pr.sel = v.sel; pr.partial = h->d_partial;
         hipLaunchKernelGGL(k_polish_rows, dim3(grid_r), dim3(BLOCK), 0, h->s, pr);
-        // rho_d into red_a[ld]: reduce 1 value (the setup writes dot into slot 0)
-        reduce(h, h->d_partial, h->d_red_a + ld, h->d_ops_sum, 1, sr.n_rb);
+        reduce_ld(h, h->d_partial, h->d_red_a + ld, h->d_ops_sum, 1, sr.n_rb, v.ld);
         CgScalarArgs cs;
-        cs.red = h->d_red_a; cs.rho = h->d_rho; cs.alpha = h->d_alpha; cs.beta = h->d_beta; cs.sel = h->d_sel;
-        cs.batch = h->B; cs.ld = h->ld; cs.mode = 0;
-        hipLaunchKernelGGL(k_cg_scalars, dim3(ceil_div(h->B, 256)), dim3(256), 0, h->s, cs);
-        for (int k = 0; k < cg_iters; ++k) {
-            PolishColsArgs c1;
-            c1.sh = sc; c1.AT = op_scaled(h, true); c1.p1 = h->d_pp; c1.p2 = h->d_pd; c1.q1 = h->d_q1;
-            c1.q2 = h->d_q2; c1.c = nullptr; c1.c_shared = 0; c1.fmask = h->d_fmask; c1.sel = h->d_sel;
-            hipLaunchKernelGGL(k_polish_cols, dim3(grid_c), dim3(BLOCK), 0, h->s, c1);
-            PolishRowsArgs c2;
-            c2.sh = sr; c2.A = op_scaled(h, false); c2.q1 = h->d_q1; c2.q2 = h->d_q2; c2.s1 = h->d_s1; c2.s2 = h->d_s2;
-            c2.p1 = h->d_pp; c2.p2 = h->d_pd; c2.setup_r = nullptr; c2.setup_p = nullptr; c2.rmask = h->d_rmask;
-            c2.sel = h->d_sel; c2.partial = h->d_partial;
-            hipLaunchKernelGGL(k_polish_rows, dim3(grid_r), dim3(BLOCK), 0, h->s, c2);
-            reduce(h, h->d_partial, h->d_red_a, h->d_ops_sum, 2, sr.n_rb);
-            cs.mode = 1;
-            hipLaunchKernelGGL(k_cg_scalars, dim3(ceil_div(h->B, 256)), dim3(256), 0, h->s, cs);
-            CgUpdateArgs u;
-            u.sh = sr; u.w1 = h->d_wp; u.r1 = h->d_rp_; u.p1 = h->d_pp; u.s1 = h->d_s1;
-            u.w2 = h->d_wd; u.r2 = h->d_rd; u.p2 = h->d_pd; u.s2 = h->d_s2; u.alpha = h->d_alpha; u.sel = h->d_sel;
-            u.partial = h->d_partial;
-            hipLaunchKernelGGL(k_cg_update, dim3(grid_r), dim3(BLOCK), 0, h->s, u);
-            reduce(h, h->d_partial, h->d_red_a, h->d_ops_sum, 2, sr.n_rb);
-            cs.mode = 2;
-            hipLaunchKernelGGL(k_cg_scalars, dim3(ceil_div(h->B, 256)), dim3(256), 0, h->s, cs);
-            CgDirArgs dd;
-            dd.rows = h->m; dd.ld = h->ld; dd.rows_per_block = ROWS_PER_BLOCK_EW; dd.p1 = h->d_pp; dd.r1 = h->d_rp_;
-            dd.p2 = h->d_pd; dd.r2 = h->d_rd; dd.beta = h->d_beta; dd.sel = h->d_sel;
-            hipLaunchKernelGGL(k_cg_direction, ew_grid(h->m, h->ld), dim3(256), 0, h->s, dd);
+        cs.red = h->d_red_a; cs.rho = h->d_rho; cs.alpha = h->d_alpha; cs.beta = h->d_beta; cs.sel = v.sel;
+        cs.batch = v.ld; cs.ld = v.ld; cs.mode = 0;
+        hipLaunchKernelGGL(k_cg_scalars, dim3(ceil_div(v.ld, 256)), dim3(256), 0, h->s, cs);
+        std::vector<double> rho0(2 * ld), rho(2 * ld);
+        HIPCHK(hipMemcpyAsync(rho0.data(), h->d_rho, sizeof(double) * 2 * ld, hipMemcpyDeviceToHost, h->s));
+        HIPCHK(hipStreamSynchronize(h->s));
+        const int blocks = (cg_iters + CG_BLOCK - 1) / CG_BLOCK;
+        for (int k = 0; k < blocks; ++k) {
+            if (k > 0) {
+                HIPCHK(hipMemcpyAsync(rho.data(), h->d_rho, sizeof(double) * 2 * ld, hipMemcpyDeviceToHost, h->s));
+                HIPCHK(hipStreamSynchronize(h->s));
+                bool converged = true;
+                for (long b = 0; b < ld && converged; ++b) {
+                    if (!hsel[b]) continue;
+                    for (int q = 0; q < 2; ++q)
+                        if (!(rho[q * ld + b] <= 1e-20 * rho0[q * ld + b])) converged = false;
+                }
+                if (converged) break;
+            }
+            CHK(cg_block(h, v, sr, sc, grid_r, grid_c));
         }
     }
     {
         PolishFinalColsArgs f;
-        f.sh = sc; f.AT = op_scaled(h, true); f.wp = h->d_wp; f.xs = h->d_xs; f.dc = h->d_dc; f.lower = h->d_lo_o;
-        f.upper = h->d_up_o; f.fmask = h->d_fmask; f.sel = h->d_sel; f.xout = h->d_xpol;
-        if (h->m == 0) {
-            // no rows: x is the snapped point
-            f.fmask = h->d_fmask;
-        }
+        f.sh = sc; f.AT = op_scaled_v(h, true, v); f.wp = h->d_wp; f.xs = h->d_xs; f.dc = h->d_dc; f.lower = v.lo;
+        f.upper = v.up; f.fmask = h->d_fmask; f.sel = v.sel; f.xout = h->d_xpol;
         hipLaunchKernelGGL(k_polish_final_cols, dim3(grid_c), dim3(BLOCK), 0, h->s, f);
     }
     if (h->m > 0) {
         PolishFinalRowsArgs f;
-        f.rows = h->m; f.ld = h->ld; f.rows_per_block = ROWS_PER_BLOCK_EW; f.neq = h->neq; f.wd = h->d_wd;
-        f.dr = h->d_dr; f.rmask = h->d_rmask; f.sel = h->d_sel; f.yout = h->d_ypol;
-        hipLaunchKernelGGL(k_polish_final_rows, ew_grid(h->m, h->ld), dim3(256), 0, h->s, f);
+        f.rows = h->m; f.ld = v.ld; f.rows_per_block = ROWS_PER_BLOCK_EW; f.neq = h->neq; f.wd = h->d_wd;
+        f.dr = h->d_dr; f.rmask = h->d_rmask; f.sel = v.sel; f.yout = h->d_ypol;
+        hipLaunchKernelGGL(k_polish_final_rows, ew_grid(h->m, v.ld), dim3(256), 0, h->s, f);
     }
     HIPCHK(hipGetLastError());
     return LPE_OK;
 }
 
-int polish_selected(lpe_handle* h, int cg_iters, double tol) {
-    if (h->m == 0) {
-        // box-only LPs: wp == 0, so the polish is the snapped point
-        HIPCHK(hipMemsetAsync(h->d_wp, 0, sizeof(double) * 1, h->s));
+template <typename T>
+void gather_cols(lpe_handle* h, const T* src, T* dst, int rows, int ldp) {
+    if (rows <= 0) return;
+    hipLaunchKernelGGL((k_gather_cols<T>), ew_grid(rows, ldp), dim3(256), 0, h->s, src, dst, rows, h->ld, ldp,
+                       (const int*)h->d_perm, ROWS_PER_BLOCK_EW);
+}
+
+template <typename T>
+int polish_rounds(lpe_handle* h, const View& v, std::vector<int>& hsel, const T* xin, const T* yin, int cg_iters,
+                  double tol, double ep, double ed, double eg, std::vector<int>& passed) {
+    const int ld = v.ld;
+    CHK(polish_t<T>(h, v, hsel.data(), xin, yin, 0, cg_iters, tol));
+    CHK(launch_certificate_v(h, v, h->d_xpol, h->d_ypol, ep, ed, eg));
+    std::vector<double> met((size_t)8 * ld);
+    passed.assign(ld, 0);
+    HIPCHK(hipMemcpyAsync(passed.data(), h->d_passed, sizeof(int) * ld, hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipMemcpyAsync(met.data(), h->d_metrics, sizeof(double) * 8 * ld, hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    // round 2 (iterative refinement from round 1's FP64 result) only where round
+    // 1 found the optimal face (small gap) but the CG residuals were above the gates
+    std::vector<int> sel2(ld, 0);
+    int again = 0;
+    for (int b = 0; b < ld; ++b) {
+        const bool close = met[(size_t)b * 8 + 5] == 1.0 && met[(size_t)b * 8 + 2] <= 1e-4;
+        sel2[b] = (hsel[b] && !passed[b] && close) ? 1 : 0;
+        again += sel2[b];
     }
-    if (h->method != LPE_METHOD_FIXED && h->precision == LPE_PREC_FP32) return polish_t<float>(h, cg_iters, tol);
-    return polish_t<double>(h, cg_iters, tol);
+    if (again > 0 && h->m > 0) {
+        int* dsel = const_cast<int*>(v.sel);
+        HIPCHK(hipMemcpyAsync(dsel, sel2.data(), sizeof(int) * ld, hipMemcpyHostToDevice, h->s));
+        CHK(polish_t<double>(h, v, sel2.data(), h->d_xpol, h->d_ypol, 1, cg_iters, tol));
+        CHK(launch_certificate_v(h, v, h->d_xpol, h->d_ypol, ep, ed, eg));
+        std::vector<int> p2(ld, 0);
+        HIPCHK(hipMemcpyAsync(p2.data(), h->d_passed, sizeof(int) * ld, hipMemcpyDeviceToHost, h->s));
+        HIPCHK(hipMemcpyAsync(dsel, hsel.data(), sizeof(int) * ld, hipMemcpyHostToDevice, h->s));
+        HIPCHK(hipStreamSynchronize(h->s));
+        for (int b = 0; b < ld; ++b)
+            if (sel2[b]) passed[b] = p2[b];
+    }
+    for (int b = 0; b < ld; ++b) passed[b] = passed[b] && hsel[b];
+    return LPE_OK;
+}
+
+// Polish the LPs selected in h->h_sel.  Selections of at most half the batch
+// are gathered into a dense prefix of the compaction buffers first, so the
+// cost follows the number of LPs polished.  On return h->h_sel marks the LPs
+// that passed the certificate, their results are in d_xres / d_yres, and
+// d_sel mirrors h->h_sel.
+int polish_selected(lpe_handle* h, int cg_iters, double tol, double ep, double ed, double eg) {
+    const bool f32 = h->method != LPE_METHOD_FIXED && h->precision == LPE_PREC_FP32;
+    std::vector<int> idx;
+    for (int b = 0; b < h->B; ++b)
+        if (h->h_sel[b]) idx.push_back(b);
+    const int P = (int)idx.size();
+    std::vector<int> passed;
+    if (P == 0) return LPE_OK;
+    if (2 * P > h->ld) {
+        View v = full_view(h);
+        std::vector<int> hsel(h->h_sel, h->h_sel + h->ld);
+        HIPCHK(hipMemcpyAsync(h->d_sel, h->h_sel, sizeof(int) * h->ld, hipMemcpyHostToDevice, h->s));
+        if (f32) CHK(polish_rounds<float>(h, v, hsel, h->it32.x, h->it32.y, cg_iters, tol, ep, ed, eg, passed));
+        else CHK(polish_rounds<double>(h, v, hsel, h->it64.x, h->it64.y, cg_iters, tol, ep, ed, eg, passed));
+        for (int b = 0; b < h->ld; ++b) h->h_sel[b] = passed[b];
+        HIPCHK(hipMemcpyAsync(h->d_sel, h->h_sel, sizeof(int) * h->ld, hipMemcpyHostToDevice, h->s));
+        hipLaunchKernelGGL(k_masked_copy, ew_grid(h->n, h->ld), dim3(256), 0, h->s, (const double*)h->d_xpol,
+                           h->d_xres, h->n, h->ld, ROWS_PER_BLOCK_EW, (const int*)h->d_sel, 1);
+        if (h->m > 0)
+            hipLaunchKernelGGL(k_masked_copy, ew_grid(h->m, h->ld), dim3(256), 0, h->s, (const double*)h->d_ypol,
+                               h->d_yres, h->m, h->ld, ROWS_PER_BLOCK_EW, (const int*)h->d_sel, 1);
+        HIPCHK(hipGetLastError());
+        return LPE_OK;
+    }
+    // compacted view: power-of-two width buckets keep the captured CG graphs few
+    int ldp = 64;
+    while (ldp < P) ldp *= 2;
+    std::vector<int> perm(ldp, -1), hsel(ldp, 0);
+    for (int j = 0; j < P; ++j) { perm[j] = idx[j]; hsel[j] = 1; }
+    HIPCHK(hipMemcpyAsync(h->d_perm, perm.data(), sizeof(int) * ldp, hipMemcpyHostToDevice, h->s));
+    HIPCHK(hipMemcpyAsync(h->d_csel, hsel.data(), sizeof(int) * ldp, hipMemcpyHostToDevice, h->s));
+    const int m = h->m, n = h->n;
+    gather_cols<double>(h, h->d_lo_o, h->d_clo, n, ldp);
+    gather_cols<double>(h, h->d_up_o, h->d_cup, n, ldp);
+    gather_cols<double>(h, h->d_rhs_o, h->d_crhs, m, ldp);
+    if (!h->shared_c) {
+        gather_cols<double>(h, h->d_c_o, h->d_cc, n, ldp);
+        gather_cols<double>(h, h->d_c_s, h->d_ccs, n, ldp);
+    }
+    if (h->nvar > 0) {
+        gather_cols<double>(h, h->d_var_o, h->d_cvar_o, h->nvar, ldp);
+        gather_cols<double>(h, h->d_var_s, h->d_cvar_s, h->nvar, ldp);
+    }
+    View v;
+    v.ld = ldp; v.c = h->shared_c ? h->d_c_o : h->d_cc; v.c_s = h->shared_c ? h->d_c_s : h->d_ccs;
+    v.rhs = h->d_crhs; v.lo = h->d_clo; v.up = h->d_cup;
+    v.var_o = h->nvar > 0 ? h->d_cvar_o : h->d_var_o; v.var_s = h->nvar > 0 ? h->d_cvar_s : h->d_var_s;
+    v.sel = h->d_csel;
+    if (f32) {
+        float* cx = reinterpret_cast<float*>(h->d_cx);
+        float* cy = reinterpret_cast<float*>(h->d_cy);
+        gather_cols<float>(h, h->it32.x, cx, n, ldp);
+        gather_cols<float>(h, h->it32.y, cy, m, ldp);
+        CHK(polish_rounds<float>(h, v, hsel, cx, cy, cg_iters, tol, ep, ed, eg, passed));
+    } else {
+        gather_cols<double>(h, h->it64.x, h->d_cx, n, ldp);
+        gather_cols<double>(h, h->it64.y, h->d_cy, m, ldp);
+        CHK(polish_rounds<double>(h, v, hsel, h->d_cx, h->d_cy, cg_iters, tol, ep, ed, eg, passed));
+    }
+    HIPCHK(hipMemcpyAsync(h->d_csel, passed.data(), sizeof(int) * ldp, hipMemcpyHostToDevice, h->s));
+    hipLaunchKernelGGL((k_scatter_cols<double>), ew_grid(n, ldp), dim3(256), 0, h->s, (const double*)h->d_xpol,
+                       h->d_xres, n, ldp, h->ld, (const int*)h->d_perm, (const int*)h->d_csel, ROWS_PER_BLOCK_EW);
+    if (m > 0)
+        hipLaunchKernelGGL((k_scatter_cols<double>), ew_grid(m, ldp), dim3(256), 0, h->s, (const double*)h->d_ypol,
+                           h->d_yres, m, ldp, h->ld, (const int*)h->d_perm, (const int*)h->d_csel, ROWS_PER_BLOCK_EW);
+    for (int b = 0; b < h->ld; ++b) h->h_sel[b] = 0;
+    for (int j = 0; j < P; ++j) h->h_sel[idx[j]] = passed[j];
+    HIPCHK(hipMemcpyAsync(h->d_sel, h->h_sel, sizeof(int) * h->ld, hipMemcpyHostToDevice, h->s));
+    HIPCHK(hipGetLastError());
+    return LPE_OK;
 }
 
 int alloc_all(lpe_handle* h) {
@@ -644,10 +856,16 @@ int alloc_all(lpe_handle* h) {
     CHK(dalloc(h, &h->d_rd, m * ld)); CHK(dalloc(h, &h->d_pd, m * ld)); CHK(dalloc(h, &h->d_wd, m * ld));
     CHK(dalloc(h, &h->d_q1, n * ld)); CHK(dalloc(h, &h->d_q2, n * ld));
     CHK(dalloc(h, &h->d_s1, m * ld)); CHK(dalloc(h, &h->d_s2, m * ld));
+    CHK(dalloc(h, &h->d_invdiag, m * ld));
     CHK(dalloc(h, &h->d_rho, 2 * ld)); CHK(dalloc(h, &h->d_alpha, 2 * ld)); CHK(dalloc(h, &h->d_beta, 2 * ld));
     CHK(dalloc(h, &h->d_xpol, n * ld)); CHK(dalloc(h, &h->d_ypol, m * ld));
     CHK(dalloc(h, &h->d_xres, n * ld)); CHK(dalloc(h, &h->d_yres, m * ld));
     CHK(dalloc(h, &h->d_xin, n * ld)); CHK(dalloc(h, &h->d_yin, m * ld));
+    CHK(dalloc(h, &h->d_perm, ld)); CHK(dalloc(h, &h->d_csel, ld));
+    CHK(dalloc(h, &h->d_clo, n * ld)); CHK(dalloc(h, &h->d_cup, n * ld)); CHK(dalloc(h, &h->d_crhs, m * ld));
+    if (!h->shared_c) { CHK(dalloc(h, &h->d_cc, n * ld)); CHK(dalloc(h, &h->d_ccs, n * ld)); }
+    CHK(dalloc(h, &h->d_cvar_o, nv * ld)); CHK(dalloc(h, &h->d_cvar_s, nv * ld));
+    CHK(dalloc(h, &h->d_cx, n * ld)); CHK(dalloc(h, &h->d_cy, m * ld));
     CHK(dalloc(h, &h->d_metrics, (size_t)8 * ld));
     CHK(dalloc(h, &h->d_err, 8));
     HIPCHK(hipHostMalloc((void**)&h->h_st, sizeof(LpState) * ld));
@@ -659,6 +877,9 @@ int alloc_all(lpe_handle* h) {
 
 void free_all(lpe_handle* h) {
     if (h->graph) hipGraphExecDestroy(h->graph);
+    for (auto& kv : h->cg_graphs)
+        if (kv.second) hipGraphExecDestroy(kv.second);
+    h->cg_graphs.clear();
     for (void* p : h->allocations) hipFree(p);
     h->allocations.clear();
     if (h->h_st) hipHostFree(h->h_st);
@@ -1045,8 +1266,15 @@ int lpe_solve(lpe_handle* h, const lpe_solve_opts* o, lpe_solve_info* info, void
     const double eg = o->eps_certify_gap > 0 ? o->eps_certify_gap : 1e-7;
     const int K = o->check_interval;
     const bool do_polish = o->polish != 0;
+    const bool trace = std::getenv("LPE_TRACE") != nullptr;
     const int cg = o->polish_cg_iterations > 0 ? o->polish_cg_iterations : 200;
-    std::vector<double> polish_kkt(h->B, o->polish_kkt > 0 ? o->polish_kkt : 1e-3);
+    // polish schedule: first attempt once the relative KKT error of an LP is
+    // below polish_kkt; after a failed attempt, retry when the LP has run 40%
+    // more iterations (at least one more chunk) or its KKT error halved
+    const double polish_kkt = o->polish_kkt > 0 ? o->polish_kkt : 1e-3;
+    std::vector<long long> last_iter(h->B, -1);
+    std::vector<double> last_rel(h->B, INFINITY);
+    long last_polish = 0;
     lpe_solve_info inf;
     std::memset(&inf, 0, sizeof(inf));
     float ms = 0.0f;
@@ -1071,39 +1299,54 @@ int lpe_solve(lpe_handle* h, const lpe_solve_opts* o, lpe_solve_info* info, void
             h->h_sel[b] = 0;
             if (b >= h->B || h->h_st[b].status != 0) continue;
             running++;
-            if (do_polish && h->h_st[b].rel_kkt <= polish_kkt[b]) { h->h_sel[b] = 1; nsel++; }
+            const LpState& st = h->h_st[b];
+            // at iteration 0 (a warm start) polish only an essentially optimal point
+            bool ready = do_polish && st.rel_kkt <= (k == 0 ? 1e-3 * polish_kkt : polish_kkt);
+            if (ready && last_iter[b] >= 0)
+                ready = st.iters >= std::max<long long>(last_iter[b] + K, (long long)(1.4 * last_iter[b])) ||
+                        st.rel_kkt <= 0.5 * last_rel[b];
+            if (ready) { h->h_sel[b] = 1; nsel++; }
         }
         const bool last = iters >= o->max_iterations;
-        if (!(do_polish && nsel > 0 && (nsel * 4 >= running || last || nsel == running))) return LPE_OK;
+        if (trace) {
+            double lo = INFINITY, hi = 0.0;
+            for (int b = 0; b < h->B; ++b)
+                if (h->h_st[b].status == 0) { lo = std::min(lo, h->h_st[b].rel_kkt); hi = std::max(hi, h->h_st[b].rel_kkt); }
+            std::fprintf(stderr, "[lpe] it %ld running %d ready %d rel_kkt [%.2e, %.2e] omega0 %.2e restarts0 %d\n", iters,
+                         running, nsel, lo, hi, h->h_st[0].omega, h->h_st[0].restarts);
+        }
+        // polish in batches: a quarter of the running LPs ready, all of them,
+        // the last chunk, or stragglers waiting 8 chunks since the last polish
+        if (!(do_polish && nsel > 0 &&
+              (nsel * 4 >= running || last || nsel == running || iters - last_polish >= 8L * K)))
+            return LPE_OK;
+        last_polish = iters;
         HIPCHK(hipEventRecord(e0, h->s));
-        HIPCHK(hipMemcpyAsync(h->d_sel, h->h_sel, sizeof(int) * h->ld, hipMemcpyHostToDevice, h->s));
-        CHK(polish_selected(h, cg, 1e-6));
-        CHK(launch_certificate(h, h->d_xpol, h->d_ypol, ep, ed, eg));
-        HIPCHK(hipMemcpyAsync(h->h_passed, h->d_passed, sizeof(int) * h->ld, hipMemcpyDeviceToHost, h->s));
+        std::vector<int> selected(h->h_sel, h->h_sel + h->ld);
+        CHK(polish_selected(h, cg, 1e-6, ep, ed, eg));
         HIPCHK(hipEventRecord(e1, h->s));
         HIPCHK(hipEventSynchronize(e1));
         HIPCHK(hipEventElapsedTime(&ms, e0, e1));
         inf.polish_ms += ms;
         inf.polish_attempts++;
+        if (trace) {
+            int np = 0;
+            for (int b = 0; b < h->B; ++b) np += h->h_sel[b];
+            std::fprintf(stderr, "[lpe]   polish %d selected, %d certified, %.2f ms\n", nsel, np, ms);
+        }
         bool any = false;
         for (int b = 0; b < h->B; ++b) {
-            if (!h->h_sel[b]) continue;
-            if (h->h_passed[b]) {
+            if (!selected[b]) continue;
+            if (h->h_sel[b]) {
                 any = true;
                 running--;
             } else {
-                h->h_sel[b] = 0;
-                polish_kkt[b] *= 0.1;
+                last_iter[b] = h->h_st[b].iters;
+                last_rel[b] = h->h_st[b].rel_kkt;
             }
         }
         if (any) {
-            // accepted: keep the polish result, freeze the LP
-            HIPCHK(hipMemcpyAsync(h->d_sel, h->h_sel, sizeof(int) * h->ld, hipMemcpyHostToDevice, h->s));
-            hipLaunchKernelGGL(k_masked_copy, ew_grid(h->n, h->ld), dim3(256), 0, h->s, (const double*)h->d_xpol,
-                               h->d_xres, h->n, h->ld, ROWS_PER_BLOCK_EW, (const int*)h->d_sel, 1);
-            if (h->m > 0)
-                hipLaunchKernelGGL(k_masked_copy, ew_grid(h->m, h->ld), dim3(256), 0, h->s, (const double*)h->d_ypol,
-                                   h->d_yres, h->m, h->ld, ROWS_PER_BLOCK_EW, (const int*)h->d_sel, 1);
+            // accepted: polish results are in d_xres / d_yres; freeze the LPs
             hipLaunchKernelGGL(k_accept_polished, dim3(ceil_div(h->ld, 256)), dim3(256), 0, h->s, h->d_st,
                                h->d_flags, h->d_source, (const int*)h->d_sel, h->ld);
             HIPCHK(hipGetLastError());
@@ -1202,18 +1445,7 @@ int lpe_polish(lpe_handle* h, const uint8_t* mask, int32_t cg_iterations, void* 
     HIPCHK(hipStreamSynchronize(h->s));
     for (int b = 0; b < h->ld; ++b)
         h->h_sel[b] = (b < h->B && (mask ? mask[b] != 0 : h->h_st[b].status == 0)) ? 1 : 0;
-    HIPCHK(hipMemcpyAsync(h->d_sel, h->h_sel, sizeof(int) * h->ld, hipMemcpyHostToDevice, h->s));
-    CHK(polish_selected(h, cg_iterations > 0 ? cg_iterations : 200, 1e-6));
-    CHK(launch_certificate(h, h->d_xpol, h->d_ypol, 1e-5, 1e-7, 1e-7));
-    HIPCHK(hipMemcpyAsync(h->h_passed, h->d_passed, sizeof(int) * h->ld, hipMemcpyDeviceToHost, h->s));
-    HIPCHK(hipStreamSynchronize(h->s));
-    for (int b = 0; b < h->ld; ++b) h->h_sel[b] = h->h_sel[b] && h->h_passed[b];
-    HIPCHK(hipMemcpyAsync(h->d_sel, h->h_sel, sizeof(int) * h->ld, hipMemcpyHostToDevice, h->s));
-    hipLaunchKernelGGL(k_masked_copy, ew_grid(h->n, h->ld), dim3(256), 0, h->s, (const double*)h->d_xpol, h->d_xres,
-                       h->n, h->ld, ROWS_PER_BLOCK_EW, (const int*)h->d_sel, 1);
-    if (h->m > 0)
-        hipLaunchKernelGGL(k_masked_copy, ew_grid(h->m, h->ld), dim3(256), 0, h->s, (const double*)h->d_ypol, h->d_yres,
-                           h->m, h->ld, ROWS_PER_BLOCK_EW, (const int*)h->d_sel, 1);
+    CHK(polish_selected(h, cg_iterations > 0 ? cg_iterations : 200, 1e-6, 1e-5, 1e-7, 1e-7));
     hipLaunchKernelGGL(k_accept_polished, dim3(ceil_div(h->ld, 256)), dim3(256), 0, h->s, h->d_st, h->d_flags,
                        h->d_source, (const int*)h->d_sel, h->ld);
     HIPCHK(hipGetLastError());

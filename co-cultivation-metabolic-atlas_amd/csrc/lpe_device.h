@@ -89,20 +89,40 @@ __device__ __forceinline__ void row_dot(const int* __restrict__ ptr, const int* 
 #pragma unroll
     for (int t = 0; t < LPL; ++t) acc[t] = TA(0);
     const int s = ptr[row], e = ptr[row + 1];
-    for (int k = s; k < e; ++k) {
-        const int col = idx[k];
-        const int sl = slot[k];
-        TX xv[LPL];
-        load_vec<TX, LPL>(x + (long)col * ld + b0, xv);
-        if (sl < 0) {
-            const TA a = (TA)val[k];
+    // Chunks of GATHER_DEPTH entries: all (index, slot, value) scalars of the
+    // chunk are loaded first, then all gathers are issued, then the products
+    // are accumulated in storage order - the gathers of one chunk are in
+    // flight together instead of one dependent load at a time.
+    constexpr int D = 8;
+    for (int k0 = s; k0 < e; k0 += D) {
+        const int cnt = e - k0 < D ? e - k0 : D;
+        int col[D], sl[D];
+        TV a[D];
 #pragma unroll
-            for (int t = 0; t < LPL; ++t) acc[t] = acc[t] + a * (TA)xv[t];
-        } else {
-            TV vv[LPL];
-            load_vec<TV, LPL>(var + (long)sl * ld + b0, vv);
+        for (int u = 0; u < D; ++u) {
+            if (u < cnt) { col[u] = idx[k0 + u]; sl[u] = slot[k0 + u]; a[u] = val[k0 + u]; }
+            else { col[u] = 0; sl[u] = -1; a[u] = TV(0); }
+        }
+        TX xv[D][LPL];
 #pragma unroll
-            for (int t = 0; t < LPL; ++t) acc[t] = acc[t] + (TA)vv[t] * (TA)xv[t];
+        for (int u = 0; u < D; ++u)
+            if (u < cnt) load_vec<TX, LPL>(x + (long)col[u] * ld + b0, xv[u]);
+        TV vv[D][LPL];
+#pragma unroll
+        for (int u = 0; u < D; ++u)
+            if (u < cnt && sl[u] >= 0) load_vec<TV, LPL>(var + (long)sl[u] * ld + b0, vv[u]);
+#pragma unroll
+        for (int u = 0; u < D; ++u) {
+            if (u < cnt) {
+                if (sl[u] < 0) {
+                    const TA av = (TA)a[u];
+#pragma unroll
+                    for (int t = 0; t < LPL; ++t) acc[t] = acc[t] + av * (TA)xv[u][t];
+                } else {
+#pragma unroll
+                    for (int t = 0; t < LPL; ++t) acc[t] = acc[t] + (TA)vv[u][t] * (TA)xv[u][t];
+                }
+            }
         }
     }
 }
@@ -319,19 +339,34 @@ __device__ __forceinline__ void block_partials(double (&val)[NV][LPL], const int
     }
 }
 
-// out[v * ld + b] = reduce_rb partial[(rb * NV + v) * ld + b] in rb order.
-__global__ void k_reduce_partials(const double* __restrict__ partial, double* __restrict__ out,
-                                  const int* __restrict__ ops, int nv, int n_rb, int ld) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+// out[v * ld + b] = reduce over rb of partial[(rb * NV + v) * ld + b].  A
+// 256-thread block covers 64 LPs of one value v; its 4 waves reduce the row
+// blocks rb = w, w+4, w+8, ... and wave 0 combines the 4 results in wave
+// order, so the summation order is fixed (deterministic across runs).
+__global__ void __launch_bounds__(256) k_reduce_partials(const double* __restrict__ partial, double* __restrict__ out,
+                                                         const int* __restrict__ ops, int nv, int n_rb, int ld) {
+    __shared__ double part[4][64];
+    const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+    const int b = blockIdx.x * 64 + lane;
     const int v = blockIdx.y;
-    if (b >= ld || v >= nv) return;
     const int op = ops[v];
-    double acc = partial[(long)v * ld + b];
-    for (int r = 1; r < n_rb; ++r) {
-        const double o = partial[((long)r * nv + v) * ld + b];
-        acc = op == RSUM ? acc + o : (o > acc || __builtin_isnan(o) ? o : acc);
+    double acc = op == RSUM ? 0.0 : -INFINITY;
+    if (b < ld) {
+        for (int r = w; r < n_rb; r += 4) {
+            const double o = partial[((long)r * nv + v) * ld + b];
+            acc = op == RSUM ? acc + o : (o > acc || __builtin_isnan(o) ? o : acc);
+        }
     }
-    out[(long)v * ld + b] = acc;
+    part[w][lane] = acc;
+    __syncthreads();
+    if (w == 0 && b < ld) {
+        double t = part[0][lane];
+        for (int q = 1; q < 4; ++q) {
+            const double o = part[q][lane];
+            t = op == RSUM ? t + o : (o > t || __builtin_isnan(o) ? o : t);
+        }
+        out[(long)v * ld + b] = t;
+    }
 }
 
 
@@ -842,11 +877,12 @@ __global__ void k_cert_final(CertFinalArgs a) {
 template <typename T>
 struct PolishColsInitArgs {
     Shape sh;
-    Op<double> AT;           // scaled FP64
-    const T* x;              // scaled iterate
+    Op<double> AT;           // scaled FP64 (orig == 0) or original FP64 (orig == 1)
+    const T* x;              // iterate: scaled (orig == 0) or original units
     const T* y;
-    const double* c;         // scaled c^ ([n] or [n][ld])
+    const double* c;         // c^ scaled (orig == 0) or original c
     int c_shared;
+    int orig;
     const double* dc;
     const double* lower;     // original [n][ld]
     const double* upper;
@@ -876,10 +912,10 @@ __global__ void __launch_bounds__(BLOCK) k_polish_cols_init(PolishColsInitArgs<T
         row_dot<double, T, double, 1>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.y, ld, b0, j, aty);
         const double d = a.dc[j];
         const double cs = a.c_shared ? a.c[j] : a.c[i];
-        const double rc = (cs - aty[0]) / d;
-        const double co = cs / d;
+        const double rc = a.orig ? cs - aty[0] : (cs - aty[0]) / d;
+        const double co = a.orig ? cs : cs / d;
         const double thr = a.tol_r * (1.0 + fabs(co));
-        double xv = (double)a.x[i] * d;
+        double xv = a.orig ? (double)a.x[i] : (double)a.x[i] * d;
         const double lo = a.lower[i], hi = a.upper[i];
         double w = hi - lo;
         w = (isfinite(w) && w > 1.0) ? w : 1.0;
@@ -897,14 +933,18 @@ template <typename T>
 struct PolishRowsInitArgs {
     Shape sh;
     Op<double> A;            // original FP64 values
+    Op<double> As;           // scaled FP64 values (Jacobi diagonal)
     const double* xs;
-    const T* y;              // scaled dual iterate
+    const T* y;              // dual iterate: scaled (orig == 0) or original units
     const double* rhs;       // original [m][ld]
     const double* dr;
+    int orig;
     const int* sel;
+    const uint8_t* fmask;
     uint8_t* rmask;
     double* rp; double* pp; double* wp;
     double* wd;
+    double* invdiag;         // [m][ld] 1 / (R A^ D A^T R)_ii, 0 on inactive rows
     int neq;
     double tol_y, tol_s;
     double* partial;         // [n_rb][1][ld]: sum rp^2
@@ -930,17 +970,25 @@ __global__ void __launch_bounds__(BLOCK) k_polish_rows_init(PolishRowsInitArgs<T
             const long i = row * ld + b0;
             const double bb = a.rhs[i];
             const double res = bb - act[0];
-            const double yv = (double)a.y[i];
+            const double yv = a.orig ? (double)a.y[i] / a.dr[row] : (double)a.y[i];
             uint8_t on = 1;
             if (row >= a.neq) {
-                const double yo = yv * a.dr[row];
+                const double yo = a.orig ? (double)a.y[i] : yv * a.dr[row];
                 on = (yo < -a.tol_y || res <= a.tol_s * fmax(1.0, fabs(bb))) ? 1 : 0;
             }
             a.rmask[i] = on;
+            double dg = 0.0;
+            for (int k = a.As.ptr[row]; k < a.As.ptr[row + 1]; ++k) {
+                const int sl = a.As.slot[k];
+                const double av = sl < 0 ? a.As.val[k] : a.As.var[(long)sl * ld + b0];
+                if (a.fmask[(long)a.As.idx[k] * ld + b0]) dg = dg + av * av;
+            }
+            const double inv = (on && dg > 0.0) ? 1.0 / dg : 0.0;
+            a.invdiag[i] = inv;
             const double rv = on ? a.dr[row] * res : 0.0;
-            a.rp[i] = rv; a.pp[i] = rv; a.wp[i] = 0.0;
+            a.rp[i] = rv; a.pp[i] = rv * inv; a.wp[i] = 0.0;
             a.wd[i] = on ? yv : 0.0;
-            v[0][0] += rv * rv;
+            v[0][0] += rv * rv * inv;
         }
     }
     const int ops[1] = {RSUM};
@@ -998,7 +1046,8 @@ struct PolishRowsArgs {
     const double* q1; const double* q2;
     double* s1; double* s2;
     const double* p1; const double* p2;
-    double* setup_r; double* setup_p;     // dual setup: r = p = s1
+    double* setup_r; double* setup_p;     // dual setup: r = s1, p = s1 * invdiag
+    const double* invdiag;
     const uint8_t* rmask;
     const int* sel;
     double* partial;         // [n_rb][2][ld]
@@ -1024,8 +1073,9 @@ __global__ void __launch_bounds__(BLOCK) k_polish_rows(PolishRowsArgs a) {
             row_dot<double, double, double, 1>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.q1, ld, b0, row, t1);
             const double sv1 = on * t1[0];
             if (a.setup_r) {
-                a.setup_r[i] = sv1; a.setup_p[i] = sv1;
-                v[0][0] += sv1 * sv1;
+                const double z = sv1 * a.invdiag[i];
+                a.setup_r[i] = sv1; a.setup_p[i] = z;
+                v[0][0] += sv1 * z;
             } else {
                 a.s1[i] = sv1;
                 v[0][0] += a.p1[i] * sv1;
@@ -1041,7 +1091,8 @@ __global__ void __launch_bounds__(BLOCK) k_polish_rows(PolishRowsArgs a) {
     block_partials<2, 1>(v, ops, a.partial, rb, ld, b0, lds, valid);
 }
 
-// Per-LP CG scalars.  mode 0: init rho from reduced [2][ld] (rho_p, rho_d);
+// Per-LP CG scalars (Jacobi-preconditioned CG, rho = r . D^-1 r).
+// mode 0: init rho from reduced [2][ld] (rho_p, rho_d);
 // mode 1: alpha = rho / pAp; mode 2: beta = rho_new / rho, rho = rho_new.
 struct CgScalarArgs {
     const double* red;      // [2][ld]
@@ -1077,6 +1128,7 @@ struct CgUpdateArgs {
     double* w1; double* r1; const double* p1; const double* s1;
     double* w2; double* r2; const double* p2; const double* s2;
     const double* alpha;    // [2][ld]
+    const double* invdiag;  // Jacobi preconditioner
     const int* sel;
     double* partial;        // [n_rb][2][ld]
 };
@@ -1097,14 +1149,15 @@ __global__ void __launch_bounds__(BLOCK) k_cg_update(CgUpdateArgs a) {
             const int row = r0 + r;
             if (row >= a.sh.rows) break;
             const long i = row * ld + b0;
+            const double iv = a.invdiag[i];
             a.w1[i] = a.w1[i] + al1 * a.p1[i];
             const double n1 = a.r1[i] - al1 * a.s1[i];
             a.r1[i] = n1;
-            v[0][0] += n1 * n1;
+            v[0][0] += n1 * n1 * iv;
             a.w2[i] = a.w2[i] + al2 * a.p2[i];
             const double n2 = a.r2[i] - al2 * a.s2[i];
             a.r2[i] = n2;
-            v[1][0] += n2 * n2;
+            v[1][0] += n2 * n2 * iv;
         }
     }
     const int ops[2] = {RSUM, RSUM};
@@ -1116,6 +1169,7 @@ struct CgDirArgs {
     int rows, ld, rows_per_block;
     double* p1; const double* r1; double* p2; const double* r2;
     const double* beta;
+    const double* invdiag;
     const int* sel;
 };
 
@@ -1128,8 +1182,9 @@ __global__ void k_cg_direction(CgDirArgs a) {
     const int r1 = min(a.rows, r0 + a.rows_per_block);
     for (int r = r0; r < r1; ++r) {
         const long i = r * ld + b;
-        a.p1[i] = a.r1[i] + be1 * a.p1[i];
-        a.p2[i] = a.r2[i] + be2 * a.p2[i];
+        const double iv = a.invdiag[i];
+        a.p1[i] = a.r1[i] * iv + be1 * a.p1[i];
+        a.p2[i] = a.r2[i] * iv + be2 * a.p2[i];
     }
 }
 
@@ -1422,6 +1477,30 @@ __global__ void k_masked_copy(const double* __restrict__ src, double* __restrict
     const int r0 = blockIdx.y * rows_per_block;
     const int r1 = min(rows, r0 + rows_per_block);
     for (int r = r0; r < r1; ++r) dst[r * l + b] = src[r * l + b];
+}
+
+// dst[r][j] = src[r][perm[j]] (perm[j] >= 0), 0 otherwise.
+template <typename T>
+__global__ void k_gather_cols(const T* __restrict__ src, T* __restrict__ dst, int rows, int ld_src, int ld_dst,
+                              const int* __restrict__ perm, int rows_per_block) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ld_dst) return;
+    const int p = perm[j];
+    const int r0 = blockIdx.y * rows_per_block;
+    const int r1 = min(rows, r0 + rows_per_block);
+    for (int r = r0; r < r1; ++r) dst[(long)r * ld_dst + j] = p >= 0 ? src[(long)r * ld_src + p] : T(0);
+}
+
+// dst[r][perm[j]] = src[r][j] for compacted LPs j with mask[j] != 0.
+template <typename T>
+__global__ void k_scatter_cols(const T* __restrict__ src, T* __restrict__ dst, int rows, int ld_src, int ld_dst,
+                               const int* __restrict__ perm, const int* __restrict__ mask, int rows_per_block) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ld_src || perm[j] < 0 || !mask[j]) return;
+    const int p = perm[j];
+    const int r0 = blockIdx.y * rows_per_block;
+    const int r1 = min(rows, r0 + rows_per_block);
+    for (int r = r0; r < r1; ++r) dst[(long)r * ld_dst + p] = src[(long)r * ld_src + j];
 }
 
 template <typename TI, typename TO>

@@ -45,7 +45,7 @@ class SolveOptions:
     max_iterations: int = 20000
     check_interval: int = 64
     polish: bool = True
-    polish_kkt: float = 1e-3
+    polish_kkt: float = 1e-4
     polish_cg_iterations: int = 200
     use_graph: bool = True
     rows_per_wave: int = 0

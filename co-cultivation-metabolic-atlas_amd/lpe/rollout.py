@@ -36,7 +36,7 @@ class ConsortiumRollout:
     def __init__(self, model: Optional[Consortium] = None, n_env: int = 4096, *, seed: int = 2,
                  precision: str = "fp32", options: Optional[SolveOptions] = None,
                  warm_start="previous", dt: float = 1.0, conc_range=(1e-2, 10.0),
-                 device: Optional[torch.device] = None):
+                 autotune: bool = True, device: Optional[torch.device] = None):
         self.model = model if model is not None else consortium(seed)
         md = self.model
         self.n_env = int(n_env)
@@ -62,6 +62,8 @@ class ConsortiumRollout:
         self.c = md.c.copy()
         self.x = None
         self.y = None
+        self.autotune = autotune
+        self.tuning = None
         self.step_index = 0
         self.log: list[dict] = []
 
@@ -101,6 +103,10 @@ class ConsortiumRollout:
         eng = self.engine
         t0 = time.perf_counter()
         eng.update(self.c, rhs, lower, upper, var_values=coupling)
+        if self.autotune and self.tuning is None:
+            from .autotune import autotune
+            self.tuning = autotune(eng)
+            self.options.check_interval = int(self.tuning["chunk"])
         x0 = y0 = None
         if callable(self.warm_start):
             x0, y0 = self.warm_start(self.conc, lower, self.biomass)

@@ -56,8 +56,8 @@ class SpeciesModel:
         return self.S.shape
 
 
-def species_model(m: int = 1000, n: int = 1500, *, n_exchange: int = 100,
-                  biomass_nnz: int = 50, seed: int = 0,
+def species_model(m: int = 1000, n: int = 1500, *, n_exchange: int | None = None,
+                  biomass_nnz: int | None = None, seed: int = 0,
                   exchange_v0_nonnegative: bool = False) -> SpeciesModel:
     """Generate one species' stoichiometry exactly as the workload specifies.
 
@@ -70,6 +70,10 @@ def species_model(m: int = 1000, n: int = 1500, *, n_exchange: int = 100,
     ``exchange_v0_nonnegative`` draws v0 >= 0 on exchange columns so that any
     tighter uptake bound (lb <= 0) keeps v0 feasible.
     """
+    if n_exchange is None:        # 100 exchange columns per 1500 fluxes
+        n_exchange = max(1, int(round(100 * n / 1500)))
+    if biomass_nnz is None:
+        biomass_nnz = min(50, m)
     rng = np.random.default_rng(seed)
     biomass_col = 0
     exchange_cols = np.arange(n - n_exchange, n)
@@ -193,7 +197,8 @@ class Consortium:
 
 
 def consortium(seed: int = 2, shapes=SPECIES_SHAPES, n_medium: int = N_MEDIUM) -> Consortium:
-    species = [species_model(m, n, seed=seed * 101 + k, exchange_v0_nonnegative=True)
+    species = [species_model(m, n, seed=seed * 101 + k, exchange_v0_nonnegative=True,
+                             n_exchange=max(int(round(100 * n / 1500)), n_medium))
                for k, (m, n) in enumerate(shapes)]
     ms = np.array([s.shape[0] for s in species]); ns = np.array([s.shape[1] for s in species])
     row_off = np.r_[0, np.cumsum(ms)]; col_off = np.r_[0, np.cumsum(ns)]

@@ -35,10 +35,12 @@ from .certificate import certificate, GATES
 
 
 def reference_metric(a, step_safety=0.99, primal_weight=1.0):
-    """sigma_i = w*eta/sum_j|a_ij|, tau_j = eta/(w*sum_i|a_ij|); empty -> 1."""
+    """sigma_i = w*eta/sum_j|a_ij|, tau_j = eta/(w*sum_i|a_ij|); empty -> 1.
+    The absolute sums run in storage order (rows) and row order (columns)."""
     absolute = abs(csr_matrix(a))
-    row_sum = np.asarray(absolute.sum(axis=1)).ravel()
-    col_sum = np.asarray(absolute.sum(axis=0)).ravel()
+    absolute.sort_indices()
+    row_sum = csr_matvec_ordered(absolute, np.ones(absolute.shape[1]))
+    col_sum = csr_rmatvec_ordered(absolute, np.ones(absolute.shape[0]))
     base_sigma = np.where(row_sum > 0.0, step_safety / np.where(row_sum > 0, row_sum, 1.0), 1.0)
     base_tau = np.where(col_sum > 0.0, step_safety / np.where(col_sum > 0, col_sum, 1.0), 1.0)
     return base_sigma * primal_weight, base_tau / primal_weight
