@@ -113,6 +113,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--max-iterations", type=int, default=30000)
+    ap.add_argument("--cg-iterations", type=int, default=200, help="CG steps per polish round")
+    ap.add_argument("--fp32-iterations", type=int, default=6000,
+                    help="FP32 budget per step; uncertified LPs continue on the FP64 fallback engine")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the LPs are independent and would shard "
@@ -134,7 +137,8 @@ def main():
     B = args.envs
 
     target = ConsortiumRollout(model, B, seed=args.seed, precision="fp32",
-                               options=SolveOptions(max_iterations=args.max_iterations, polish=True),
+                               options=SolveOptions(max_iterations=args.fp32_iterations, polish=True,
+                                                    polish_cg_iterations=args.cg_iterations),
                                warm_start="previous", device=dev)
     for _ in range(args.warmup):
         target.step()
@@ -151,8 +155,9 @@ def main():
         certified += int(out["certified"])
         if k == 0:
             first_x = out["x"]
-        step_logs.append({kk: out[kk] for kk in ("certified", "iterations", "restarts", "polish_attempts",
-                                                 "pdhg_ms", "check_ms", "polish_ms", "wall_seconds")})
+        step_logs.append({kk: out.get(kk) for kk in ("certified", "iterations", "restarts", "polish_attempts",
+                                                     "pdhg_ms", "check_ms", "polish_ms", "wall_seconds",
+                                                     "fallback_lps", "fallback_certified", "fallback_seconds")})
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     value = certified / elapsed
@@ -176,7 +181,7 @@ def main():
                               options=SolveOptions(max_iterations=args.max_iterations, polish=False),
                               warm_start="none", device=dev)
     p_cert, p_time, p_logs = 0, 0.0, []
-    for k in range(min(args.plain_steps, len(snapshots))):
+    for k in range(min(args.plain_steps, len(snapshots))) if args.plain_steps > 0 else []:
         plain.restore(snapshots[k])
         torch.cuda.synchronize()
         t = time.perf_counter()

@@ -128,7 +128,11 @@ struct lpe_handle {
     double *d_rd = nullptr, *d_pd = nullptr, *d_wd = nullptr;
     double *d_q1 = nullptr, *d_q2 = nullptr, *d_s1 = nullptr, *d_s2 = nullptr;
     uint8_t *d_fmask = nullptr, *d_rmask = nullptr;
-    double *d_rho = nullptr, *d_alpha = nullptr, *d_beta = nullptr, *d_invdiag = nullptr;
+    double *d_rho = nullptr, *d_alpha = nullptr, *d_beta = nullptr, *d_invdiag = nullptr, *d_rho0 = nullptr;
+    int* d_cgact = nullptr;
+    double* d_cgfloor = nullptr;
+    int last_cg_steps = 0;
+    double cg_ptol2 = 0.0;    // primal CG stop: |Dr (b - A x)|_2^2 <= (1e-8 min(dr))^2
     double *d_xpol = nullptr, *d_ypol = nullptr, *d_xres = nullptr, *d_yres = nullptr;
     double* d_metrics = nullptr;
     double *d_xin = nullptr, *d_yin = nullptr;    // user iterates for lpe_certify / lpe_spmv
@@ -378,7 +382,10 @@ void reduce(lpe_handle* h, const double* partial, double* out, const int* ops, i
 // Rows per wave of the check / certificate passes: 128 rows per block keeps
 // the per-block partial sums (NV doubles per LP and block) small.
 constexpr int CHECK_RPW = 32;
-constexpr int POLISH_RPW = 16;
+// Rows per wave of the polish kernels: few rows per wave for narrow (compacted)
+// batches, where the kernels are latency-bound, more for wide ones, where the
+// per-block partial sums dominate.
+int polish_rpw(int ld) { return ld >= 1024 ? 16 : (ld >= 256 ? 4 : 1); }
 
 // Device check: KKT / certificate values of (current, average), decisions, actions.
 template <typename T>
@@ -538,35 +545,38 @@ int build_results(lpe_handle* h) {
 }
 
 constexpr int CG_BLOCK = 25;
+constexpr double CG_REL = 1e-20;   // rho = r.D^-1 r reduction at which a system stops
 
 void cg_steps(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, int grid_r, int grid_c, int steps) {
     CgScalarArgs cs;
-    cs.red = h->d_red_a; cs.rho = h->d_rho; cs.alpha = h->d_alpha; cs.beta = h->d_beta; cs.sel = v.sel;
-    cs.batch = v.ld; cs.ld = v.ld;
+    cs.rho = h->d_rho; cs.rho0 = h->d_rho0; cs.alpha = h->d_alpha; cs.beta = h->d_beta;
+    cs.act = h->d_cgact; cs.floor2 = h->d_cgfloor; cs.sel = v.sel; cs.rel = CG_REL; cs.batch = v.ld; cs.ld = v.ld;
+    cs.ptol2 = h->cg_ptol2;
     for (int k = 0; k < steps; ++k) {
         PolishColsArgs c1;
         c1.sh = sc; c1.AT = op_scaled_v(h, true, v); c1.p1 = h->d_pp; c1.p2 = h->d_pd; c1.q1 = h->d_q1;
-        c1.q2 = h->d_q2; c1.c = nullptr; c1.c_shared = 0; c1.fmask = h->d_fmask; c1.sel = v.sel;
+        c1.q2 = h->d_q2; c1.c = nullptr; c1.c_shared = 0; c1.fmask = h->d_fmask; c1.sel = v.sel; c1.act = h->d_cgact;
         hipLaunchKernelGGL(k_polish_cols, dim3(grid_c), dim3(BLOCK), 0, h->s, c1);
         PolishRowsArgs c2;
         c2.sh = sr; c2.A = op_scaled_v(h, false, v); c2.q1 = h->d_q1; c2.q2 = h->d_q2; c2.s1 = h->d_s1; c2.s2 = h->d_s2;
         c2.p1 = h->d_pp; c2.p2 = h->d_pd; c2.setup_r = nullptr; c2.setup_p = nullptr; c2.invdiag = h->d_invdiag;
-        c2.rmask = h->d_rmask; c2.sel = v.sel; c2.partial = h->d_partial;
+        c2.rmask = h->d_rmask; c2.sel = v.sel; c2.act = h->d_cgact; c2.partial = h->d_partial;
         hipLaunchKernelGGL(k_polish_rows, dim3(grid_r), dim3(BLOCK), 0, h->s, c2);
         reduce_ld(h, h->d_partial, h->d_red_a, h->d_ops_sum, 2, sr.n_rb, v.ld);
-        cs.mode = 1;
+        cs.mode = 1; cs.d0 = h->d_red_a; cs.d1 = h->d_red_a + v.ld; cs.plain = nullptr;
         hipLaunchKernelGGL(k_cg_scalars, dim3(ceil_div(v.ld, 256)), dim3(256), 0, h->s, cs);
         CgUpdateArgs u;
         u.sh = sr; u.w1 = h->d_wp; u.r1 = h->d_rp_; u.p1 = h->d_pp; u.s1 = h->d_s1;
         u.w2 = h->d_wd; u.r2 = h->d_rd; u.p2 = h->d_pd; u.s2 = h->d_s2; u.alpha = h->d_alpha;
-        u.invdiag = h->d_invdiag; u.sel = v.sel; u.partial = h->d_partial;
+        u.invdiag = h->d_invdiag; u.sel = v.sel; u.act = h->d_cgact; u.partial = h->d_partial;
         hipLaunchKernelGGL(k_cg_update, dim3(grid_r), dim3(BLOCK), 0, h->s, u);
-        reduce_ld(h, h->d_partial, h->d_red_a, h->d_ops_sum, 2, sr.n_rb, v.ld);
-        cs.mode = 2;
+        reduce_ld(h, h->d_partial, h->d_red_a, h->d_ops_sum, 3, sr.n_rb, v.ld);
+        cs.mode = 2; cs.d0 = h->d_red_a; cs.d1 = h->d_red_a + v.ld; cs.plain = h->d_red_a + 2 * (long)v.ld;
         hipLaunchKernelGGL(k_cg_scalars, dim3(ceil_div(v.ld, 256)), dim3(256), 0, h->s, cs);
         CgDirArgs dd;
         dd.rows = h->m; dd.ld = v.ld; dd.rows_per_block = ROWS_PER_BLOCK_EW; dd.p1 = h->d_pp; dd.r1 = h->d_rp_;
         dd.p2 = h->d_pd; dd.r2 = h->d_rd; dd.beta = h->d_beta; dd.invdiag = h->d_invdiag; dd.sel = v.sel;
+        dd.act = h->d_cgact;
         hipLaunchKernelGGL(k_cg_direction, ew_grid(h->m, v.ld), dim3(256), 0, h->s, dd);
     }
 }
@@ -600,8 +610,8 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
              double tol) {
     const long ld = v.ld;
     int grid_r = 0, grid_c = 0;
-    Shape sr = make_shape(std::max(h->m, 1), v.ld, v.ld, POLISH_RPW, 1, &grid_r);
-    Shape sc = make_shape(h->n, v.ld, v.ld, POLISH_RPW, 1, &grid_c);
+    Shape sr = make_shape(std::max(h->m, 1), v.ld, v.ld, polish_rpw(v.ld), 1, &grid_r);
+    Shape sc = make_shape(h->n, v.ld, v.ld, polish_rpw(v.ld), 1, &grid_c);
     {
         PolishColsInitArgs<T> a;
         a.sh = sc; a.AT = orig ? op_orig_v(h, true, v) : op_scaled_v(h, true, v); a.x = xin; a.y = yin;
@@ -617,39 +627,58 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
         a.sel = v.sel; a.rmask = h->d_rmask; a.rp = h->d_rp_; a.pp = h->d_pp; a.wp = h->d_wp; a.wd = h->d_wd;
         a.neq = h->neq; a.tol_y = 1e-9; a.tol_s = 1e-7; a.partial = h->d_partial;
         hipLaunchKernelGGL((k_polish_rows_init<T>), dim3(grid_r), dim3(BLOCK), 0, h->s, a);
-        reduce_ld(h, h->d_partial, h->d_red_a, h->d_ops_sum, 1, sr.n_rb, v.ld);
+        reduce_ld(h, h->d_partial, h->d_red_b, h->d_ops_sum, 2, sr.n_rb, v.ld);   // rho_p, |r_p|^2
         // dual setup: q1 = D (c^ - A^T wd); r_d = R A q1, p_d = r_d / diag
         PolishColsArgs pc;
         pc.sh = sc; pc.AT = op_scaled_v(h, true, v); pc.p1 = h->d_wd; pc.p2 = nullptr; pc.q1 = h->d_q1; pc.q2 = nullptr;
-        pc.c = v.c_s; pc.c_shared = h->shared_c; pc.fmask = h->d_fmask; pc.sel = v.sel;
+        pc.c = v.c_s; pc.c_shared = h->shared_c; pc.fmask = h->d_fmask; pc.sel = v.sel; pc.act = nullptr;
         hipLaunchKernelGGL(k_polish_cols, dim3(grid_c), dim3(BLOCK), 0, h->s, pc);
         PolishRowsArgs pr;
         pr.sh = sr; pr.A = op_scaled_v(h, false, v); pr.q1 = h->d_q1; pr.q2 = nullptr; pr.s1 = nullptr; pr.s2 = nullptr;
         pr.p1 = nullptr; pr.p2 = nullptr; pr.setup_r = h->d_rd; pr.setup_p = h->d_pd; pr.invdiag = h->d_invdiag;
-        pr.rmask = h->d_rmask; pr.sel = v.sel; pr.partial = h->d_partial;
+        pr.rmask = h->d_rmask; pr.sel = v.sel; pr.act = nullptr; pr.partial = h->d_partial;
         hipLaunchKernelGGL(k_polish_rows, dim3(grid_r), dim3(BLOCK), 0, h->s, pr);
-        reduce_ld(h, h->d_partial, h->d_red_a + ld, h->d_ops_sum, 1, sr.n_rb, v.ld);
+        reduce_ld(h, h->d_partial, h->d_red_a, h->d_ops_sum, 1, sr.n_rb, v.ld);   // rho_d
         CgScalarArgs cs;
-        cs.red = h->d_red_a; cs.rho = h->d_rho; cs.alpha = h->d_alpha; cs.beta = h->d_beta; cs.sel = v.sel;
-        cs.batch = v.ld; cs.ld = v.ld; cs.mode = 0;
+        cs.rho = h->d_rho; cs.rho0 = h->d_rho0; cs.alpha = h->d_alpha; cs.beta = h->d_beta;
+        cs.act = h->d_cgact; cs.floor2 = h->d_cgfloor; cs.sel = v.sel; cs.rel = CG_REL; cs.batch = v.ld; cs.ld = v.ld;
+        cs.mode = 0; cs.d0 = h->d_red_b; cs.d1 = h->d_red_a; cs.plain = h->d_red_b + ld; cs.ptol2 = h->cg_ptol2;
+        hipLaunchKernelGGL(k_cg_floor, dim3(ceil_div(v.ld, 256)), dim3(256), 0, h->s, (const double*)h->d_bnorm,
+                           (const double*)h->d_cnorm, v.sel == h->d_sel ? (const int*)nullptr : (const int*)h->d_perm,
+                           h->d_cgfloor, v.ld);
         hipLaunchKernelGGL(k_cg_scalars, dim3(ceil_div(v.ld, 256)), dim3(256), 0, h->s, cs);
-        std::vector<double> rho0(2 * ld), rho(2 * ld);
-        HIPCHK(hipMemcpyAsync(rho0.data(), h->d_rho, sizeof(double) * 2 * ld, hipMemcpyDeviceToHost, h->s));
-        HIPCHK(hipStreamSynchronize(h->s));
+        std::vector<int> act(2 * ld);
         const int blocks = (cg_iters + CG_BLOCK - 1) / CG_BLOCK;
+        int used = 0;
         for (int k = 0; k < blocks; ++k) {
             if (k > 0) {
-                HIPCHK(hipMemcpyAsync(rho.data(), h->d_rho, sizeof(double) * 2 * ld, hipMemcpyDeviceToHost, h->s));
+                HIPCHK(hipMemcpyAsync(act.data(), h->d_cgact, sizeof(int) * 2 * ld, hipMemcpyDeviceToHost, h->s));
                 HIPCHK(hipStreamSynchronize(h->s));
-                bool converged = true;
-                for (long b = 0; b < ld && converged; ++b) {
-                    if (!hsel[b]) continue;
-                    for (int q = 0; q < 2; ++q)
-                        if (!(rho[q * ld + b] <= 1e-20 * rho0[q * ld + b])) converged = false;
-                }
-                if (converged) break;
+                bool any = false;
+                for (long q = 0; q < 2 * ld && !any; ++q) any = act[q] != 0;
+                if (!any) break;
             }
             CHK(cg_block(h, v, sr, sc, grid_r, grid_c));
+            used += CG_BLOCK;
+        }
+        h->last_cg_steps = used;
+        if (std::getenv("LPE_TRACE")) {
+            std::vector<double> rh(2 * ld), r0(2 * ld), fl(2 * ld), pl(ld);
+            HIPCHK(hipMemcpy(act.data(), h->d_cgact, sizeof(int) * 2 * ld, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(rh.data(), h->d_rho, sizeof(double) * 2 * ld, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(r0.data(), h->d_rho0, sizeof(double) * 2 * ld, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(fl.data(), h->d_cgfloor, sizeof(double) * 2 * ld, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(pl.data(), h->d_red_a + 2 * ld, sizeof(double) * ld, hipMemcpyDeviceToHost));
+            int na[2] = {0, 0};
+            for (long b = 0; b < ld; ++b)
+                for (int q = 0; q < 2; ++q)
+                    if (act[q * ld + b]) {
+                        if (na[q]++ < 2)
+                            std::fprintf(stderr, "[lpe]     cg sys %d lp %ld rho %.3e rho0 %.3e floor %.3e plain %.3e\n", q,
+                                         b, rh[q * ld + b], r0[q * ld + b], fl[q * ld + b], pl[b]);
+                    }
+            std::fprintf(stderr, "[lpe]     cg still active: primal %d dual %d (ptol2 %.3e)\n", na[0], na[1],
+                         h->cg_ptol2);
         }
     }
     {
@@ -837,19 +866,29 @@ int alloc_all(lpe_handle* h) {
     Shape sc = make_shape(h->n, h->ld, h->B, CHECK_RPW, 1, &g);
     const size_t pr = (size_t)sr.n_rb * 2 * CR_N, pc = (size_t)sc.n_rb * 2 * CC_N;
     h->partial_elems = std::max(pr, pc) * ld;
+    // polish rows kernels write 2 partials per row block; narrow views use 1 row per wave
+    for (int w = 64; w <= (int)ld; w *= 2) {
+        Shape sp = make_shape(std::max(h->m, 1), w, w, polish_rpw(w), 1, &g);
+        h->partial_elems = std::max(h->partial_elems, (size_t)sp.n_rb * 3 * w);
+    }
+    {
+        Shape sp = make_shape(std::max(h->m, 1), ld, ld, polish_rpw(ld), 1, &g);
+        h->partial_elems = std::max(h->partial_elems, (size_t)sp.n_rb * 3 * ld);
+    }
     CHK(dalloc(h, &h->d_partial, h->partial_elems));
     CHK(dalloc(h, &h->d_red_a, (size_t)2 * CC_N * ld)); CHK(dalloc(h, &h->d_red_b, (size_t)2 * CC_N * ld));
+    // (red_a / red_b also hold the polish's 3 CG dots per LP)
     const int ops_cr[2 * CR_N] = {RSUM, RMAX, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM, RMAX, RSUM, RSUM, RMAX, RSUM, RSUM};
     const int ops_cc[2 * CC_N] = {RSUM, RMAX, RSUM, RSUM, RSUM, RMAX, RSUM, RSUM,
                                   RSUM, RMAX, RSUM, RSUM, RSUM, RMAX, RSUM, RSUM};
     const int ops_xr[XR_N] = {RMAX, RMAX, RSUM, RMAX, RSUM};
     const int ops_xc[XC_N] = {RMAX, RMAX, RMAX, RMAX, RSUM, RSUM, RSUM, RSUM};
-    const int ops_sum[2] = {RSUM, RSUM};
+    const int ops_sum[3] = {RSUM, RSUM, RSUM};
     CHK(dalloc(h, &h->d_ops_cr, 2 * CR_N)); CHK(upload(h->d_ops_cr, ops_cr, 2 * CR_N));
     CHK(dalloc(h, &h->d_ops_cc, 2 * CC_N)); CHK(upload(h->d_ops_cc, ops_cc, 2 * CC_N));
     CHK(dalloc(h, &h->d_ops_xr, XR_N)); CHK(upload(h->d_ops_xr, ops_xr, XR_N));
     CHK(dalloc(h, &h->d_ops_xc, XC_N)); CHK(upload(h->d_ops_xc, ops_xc, XC_N));
-    CHK(dalloc(h, &h->d_ops_sum, 2)); CHK(upload(h->d_ops_sum, ops_sum, 2));
+    CHK(dalloc(h, &h->d_ops_sum, 3)); CHK(upload(h->d_ops_sum, ops_sum, 3));
     // polish and results
     CHK(dalloc(h, &h->d_xs, n * ld)); CHK(dalloc(h, &h->d_fmask, n * ld)); CHK(dalloc(h, &h->d_rmask, m * ld));
     CHK(dalloc(h, &h->d_rp_, m * ld)); CHK(dalloc(h, &h->d_pp, m * ld)); CHK(dalloc(h, &h->d_wp, m * ld));
@@ -857,7 +896,8 @@ int alloc_all(lpe_handle* h) {
     CHK(dalloc(h, &h->d_q1, n * ld)); CHK(dalloc(h, &h->d_q2, n * ld));
     CHK(dalloc(h, &h->d_s1, m * ld)); CHK(dalloc(h, &h->d_s2, m * ld));
     CHK(dalloc(h, &h->d_invdiag, m * ld));
-    CHK(dalloc(h, &h->d_rho, 2 * ld)); CHK(dalloc(h, &h->d_alpha, 2 * ld)); CHK(dalloc(h, &h->d_beta, 2 * ld));
+    CHK(dalloc(h, &h->d_rho, 2 * ld)); CHK(dalloc(h, &h->d_rho0, 2 * ld)); CHK(dalloc(h, &h->d_cgact, 2 * ld));
+    CHK(dalloc(h, &h->d_cgfloor, 2 * ld)); CHK(dalloc(h, &h->d_alpha, 2 * ld)); CHK(dalloc(h, &h->d_beta, 2 * ld));
     CHK(dalloc(h, &h->d_xpol, n * ld)); CHK(dalloc(h, &h->d_ypol, m * ld));
     CHK(dalloc(h, &h->d_xres, n * ld)); CHK(dalloc(h, &h->d_yres, m * ld));
     CHK(dalloc(h, &h->d_xin, n * ld)); CHK(dalloc(h, &h->d_yin, m * ld));
@@ -896,7 +936,7 @@ int init_state(lpe_handle* h) {
     for (int b = 0; b < h->ld; ++b) {
         LpState& s = h->h_st[b];
         std::memset(&s, 0, sizeof(s));
-        s.omega = h->method == LPE_METHOD_RESTARTED ? h->pw : 1.0;
+        s.omega = (h->method == LPE_METHOD_RESTARTED && h->pw > 0.0) ? h->pw : 1.0;
         s.kkt_last = INFINITY;
         s.kkt_prev = INFINITY;
         s.rel_kkt = INFINITY;
@@ -919,6 +959,10 @@ int init_state(lpe_handle* h) {
     HIPCHK(hipMemcpyAsync(h->d_iom, iom.data(), sizeof(double) * h->ld, hipMemcpyHostToDevice, h->s));
     HIPCHK(hipMemcpyAsync(h->d_om_f, omf.data(), sizeof(float) * h->ld, hipMemcpyHostToDevice, h->s));
     HIPCHK(hipMemcpyAsync(h->d_iom_f, iomf.data(), sizeof(float) * h->ld, hipMemcpyHostToDevice, h->s));
+    if (h->method == LPE_METHOD_RESTARTED && h->have_vectors)
+        hipLaunchKernelGGL(k_init_omega, dim3(ceil_div(h->B, 256)), dim3(256), 0, h->s, h->d_st, h->d_om, h->d_om_f,
+                           h->d_iom, h->d_iom_f, (const double*)h->d_bnorm, (const double*)h->d_cnorm,
+                           (const double*)nullptr, h->pw, h->B);
     // host vectors must outlive the async copies
     HIPCHK(hipStreamSynchronize(h->s));
     return LPE_OK;
@@ -950,7 +994,9 @@ int lpe_create(const lpe_desc* d, lpe_handle** out) {
         return fail(LPE_EINVAL, "the fixed-metric method runs in FP64");
     if (!(d->step_safety > 0.0 && d->step_safety < 1.0)) return fail(LPE_EINVAL, "step_safety must be strictly between zero and one");
     if (!(d->theta >= 0.0 && d->theta <= 1.0)) return fail(LPE_EINVAL, "theta must be finite and between zero and one");
-    if (!(d->primal_weight > 0.0) || !std::isfinite(d->primal_weight)) return fail(LPE_EINVAL, "primal_weight must be finite and strictly positive");
+    if (!std::isfinite(d->primal_weight) || d->primal_weight < 0.0 ||
+        (d->primal_weight == 0.0 && d->method != LPE_METHOD_RESTARTED))
+        return fail(LPE_EINVAL, "primal_weight must be finite and strictly positive");
     const long m = d->m, n = d->n, nnz = d->nnz;
     if ((long)d->batch * std::max(m, n) >= (1L << 31) - 1 || nnz >= (1L << 31) - 1)
         return fail(LPE_EINVAL, "batch * dimension exceeds int32 capacity");
@@ -1018,6 +1064,12 @@ int lpe_create(const lpe_desc* d, lpe_handle** out) {
         h->var_absmax[s] = std::fabs(h->val[p] * h->dr[h->var_row[s]] * h->dc[h->var_col[s]]);
     }
     shared_metric(h);
+    {
+        double mdr = INFINITY;
+        for (double v : h->dr) mdr = std::min(mdr, v);
+        if (!(mdr < INFINITY)) mdr = 1.0;
+        h->cg_ptol2 = (1e-8 * mdr) * (1e-8 * mdr);
+    }
     // device structure
     std::vector<double> tval(nnz), sval(nnz), tsval(nnz);
     std::vector<float> fval(nnz), tfval(nnz);
@@ -1332,7 +1384,8 @@ int lpe_solve(lpe_handle* h, const lpe_solve_opts* o, lpe_solve_info* info, void
         if (trace) {
             int np = 0;
             for (int b = 0; b < h->B; ++b) np += h->h_sel[b];
-            std::fprintf(stderr, "[lpe]   polish %d selected, %d certified, %.2f ms\n", nsel, np, ms);
+            std::fprintf(stderr, "[lpe]   polish %d selected, %d certified, %.2f ms, last round %d CG steps\n", nsel,
+                         np, ms, h->last_cg_steps);
         }
         bool any = false;
         for (int b = 0; b < h->B; ++b) {
@@ -1516,6 +1569,22 @@ int lpe_get_scaling(lpe_handle* h, double* row_scale, double* col_scale, double*
     if (sigma) std::copy(h->sig.begin(), h->sig.end(), sigma);
     if (tau) std::copy(h->tau.begin(), h->tau.end(), tau);
     return LPE_OK;
+}
+
+int lpe_set_primal_weight(lpe_handle* h, const double* omega, void* stream) {
+    if (!h || !omega) return fail(LPE_EINVAL, "handle and omega are required");
+    if (h->method != LPE_METHOD_RESTARTED) return fail(LPE_EINVAL, "the fixed-metric method keeps its primal weight");
+    for (int b = 0; b < h->B; ++b)
+        if (!(omega[b] > 0.0) || !std::isfinite(omega[b])) return fail(LPE_EINVAL, "primal weights must be finite and positive");
+    if (!h->ready) return fail(LPE_ESTATE, "reset is required after a problem update");
+    CHK(enter(h, stream));
+    double* tmp = h->d_metrics;   // scratch: [8][ld] doubles >= B
+    HIPCHK(hipMemcpyAsync(tmp, omega, sizeof(double) * h->B, hipMemcpyHostToDevice, h->s));
+    hipLaunchKernelGGL(k_init_omega, dim3(ceil_div(h->B, 256)), dim3(256), 0, h->s, h->d_st, h->d_om, h->d_om_f,
+                       h->d_iom, h->d_iom_f, (const double*)h->d_bnorm, (const double*)h->d_cnorm,
+                       (const double*)tmp, 1.0, h->B);
+    HIPCHK(hipStreamSynchronize(h->s));
+    return leave(h, stream);
 }
 
 int lpe_get_primal_weight(lpe_handle* h, double* omega) {

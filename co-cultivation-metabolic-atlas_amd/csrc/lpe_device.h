@@ -947,19 +947,19 @@ struct PolishRowsInitArgs {
     double* invdiag;         // [m][ld] 1 / (R A^ D A^T R)_ii, 0 on inactive rows
     int neq;
     double tol_y, tol_s;
-    double* partial;         // [n_rb][1][ld]: sum rp^2
+    double* partial;         // [n_rb][2][ld]: r.D^-1 r and |r|^2 of the primal system
 };
 
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_polish_rows_init(PolishRowsInitArgs<T> a) {
-    __shared__ double lds[WAVES_PER_BLOCK * 1 * WAVE];
+    __shared__ double lds[WAVES_PER_BLOCK * 2 * WAVE];
     int tile, rb;
     if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int b0 = tile * WAVE + lane;
     const bool valid = b0 < a.sh.ld;
     const long ld = a.sh.ld;
-    double v[1][1] = {{0.0}};
+    double v[2][1] = {{0.0}, {0.0}};
     if (valid && a.sel[b0]) {
         const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
         for (int r = 0; r < a.sh.rpw; ++r) {
@@ -989,10 +989,11 @@ __global__ void __launch_bounds__(BLOCK) k_polish_rows_init(PolishRowsInitArgs<T
             a.rp[i] = rv; a.pp[i] = rv * inv; a.wp[i] = 0.0;
             a.wd[i] = on ? yv : 0.0;
             v[0][0] += rv * rv * inv;
+            v[1][0] += rv * rv;
         }
     }
-    const int ops[1] = {RSUM};
-    block_partials<1, 1>(v, ops, a.partial, rb, ld, b0, lds, valid);
+    const int ops[2] = {RSUM, RSUM};
+    block_partials<2, 1>(v, ops, a.partial, rb, ld, b0, lds, valid);
 }
 
 // q_j = fmask * (base_j - sum_i a^_ij p_i), with base = c^ (dual setup) or 0 and
@@ -1006,6 +1007,7 @@ struct PolishColsArgs {
     int c_shared;
     const uint8_t* fmask;
     const int* sel;
+    const int* act;          // [2][ld] CG activity (nullptr: both active)
 };
 
 __global__ void __launch_bounds__(BLOCK) k_polish_cols(PolishColsArgs a) {
@@ -1014,6 +1016,8 @@ __global__ void __launch_bounds__(BLOCK) k_polish_cols(PolishColsArgs a) {
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int b0 = tile * WAVE + lane;
     if (b0 >= a.sh.ld || !a.sel[b0]) return;
+    const bool on1 = !a.act || a.act[b0], on2 = !a.act || a.act[a.sh.ld + b0];
+    if (!on1 && !on2) return;
     const long ld = a.sh.ld;
     const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
     for (int r = 0; r < a.sh.rpw; ++r) {
@@ -1027,8 +1031,8 @@ __global__ void __launch_bounds__(BLOCK) k_polish_cols(PolishColsArgs a) {
             const double cc = a.c_shared ? a.c[col] : a.c[i];
             a.q1[i] = f * (cc - s1[0]);
         } else {
-            a.q1[i] = f * s1[0];
-            if (a.p2) {
+            if (on1) a.q1[i] = f * s1[0];
+            if (a.p2 && on2) {
                 double s2[1];
                 row_dot<double, double, double, 1>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.p2, ld, b0, col, s2);
                 a.q2[i] = f * s2[0];
@@ -1050,6 +1054,7 @@ struct PolishRowsArgs {
     const double* invdiag;
     const uint8_t* rmask;
     const int* sel;
+    const int* act;          // [2][ld] CG activity (nullptr: both active)
     double* partial;         // [n_rb][2][ld]
 };
 
@@ -1062,7 +1067,9 @@ __global__ void __launch_bounds__(BLOCK) k_polish_rows(PolishRowsArgs a) {
     const bool valid = b0 < a.sh.ld;
     const long ld = a.sh.ld;
     double v[2][1] = {{0.0}, {0.0}};
-    if (valid && a.sel[b0]) {
+    const bool on1 = valid && (!a.act || a.act[b0]);
+    const bool on2 = valid && (!a.act || a.act[ld + b0]);
+    if (valid && a.sel[b0] && (on1 || on2 || a.setup_r)) {
         const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
         for (int r = 0; r < a.sh.rpw; ++r) {
             const int row = uniform(r0 + r);
@@ -1077,13 +1084,17 @@ __global__ void __launch_bounds__(BLOCK) k_polish_rows(PolishRowsArgs a) {
                 a.setup_r[i] = sv1; a.setup_p[i] = z;
                 v[0][0] += sv1 * z;
             } else {
-                a.s1[i] = sv1;
-                v[0][0] += a.p1[i] * sv1;
-                double t2[1];
-                row_dot<double, double, double, 1>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.q2, ld, b0, row, t2);
-                const double sv2 = on * t2[0];
-                a.s2[i] = sv2;
-                v[1][0] += a.p2[i] * sv2;
+                if (on1) {
+                    a.s1[i] = sv1;
+                    v[0][0] += a.p1[i] * sv1;
+                }
+                if (on2) {
+                    double t2[1];
+                    row_dot<double, double, double, 1>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.q2, ld, b0, row, t2);
+                    const double sv2 = on * t2[0];
+                    a.s2[i] = sv2;
+                    v[1][0] += a.p2[i] * sv2;
+                }
             }
         }
     }
@@ -1091,33 +1102,54 @@ __global__ void __launch_bounds__(BLOCK) k_polish_rows(PolishRowsArgs a) {
     block_partials<2, 1>(v, ops, a.partial, rb, ld, b0, lds, valid);
 }
 
-// Per-LP CG scalars (Jacobi-preconditioned CG, rho = r . D^-1 r).
-// mode 0: init rho from reduced [2][ld] (rho_p, rho_d);
-// mode 1: alpha = rho / pAp; mode 2: beta = rho_new / rho, rho = rho_new.
+// Per-LP CG scalars (Jacobi-preconditioned CG, rho = r . D^-1 r), two systems
+// (0: primal correction, 1: dual least squares).  mode 0: initialise rho and
+// rho0 from the reduced setup dots and mark active systems; mode 1: alpha =
+// rho / (p . M p); mode 2: beta = rho_new / rho, and a system stops (stays
+// inactive) once rho <= rel * rho0.  act[s*ld + b] != 0 while system s of LP b
+// still iterates; the kernels skip LPs whose two systems have stopped.
 struct CgScalarArgs {
-    const double* red;      // [2][ld]
+    const double* d0;       // [ld] primal-system dot (rho or p.Mp)
+    const double* d1;       // [ld] dual-system dot
+    const double* plain;    // [ld] |r_p|^2 (modes 0 and 2)
+    double ptol2;           // primal stop: |r_p|^2 <= ptol2
     double* rho;            // [2][ld]
+    double* rho0;           // [2][ld]
     double* alpha;          // [2][ld]
     double* beta;           // [2][ld]
+    int* act;               // [2][ld]
+    const double* floor2;   // [2][ld] absolute floor of rho (rounding level)
     const int* sel;
+    double rel;
     int batch, ld, mode;
 };
 
 __global__ void k_cg_scalars(CgScalarArgs a) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= a.batch || !a.sel[b]) return;
+    if (b >= a.batch) return;
+    if (!a.sel[b]) { a.act[b] = 0; a.act[a.ld + b] = 0; return; }
     for (int s = 0; s < 2; ++s) {
         const long i = (long)s * a.ld + b;
-        const double v = a.red[i];
+        const double v = s == 0 ? a.d0[b] : a.d1[b];
+        const bool small = s == 0 && a.plain && a.plain[b] <= a.ptol2;
         if (a.mode == 0) {
             a.rho[i] = v;
+            a.rho0[i] = v;
+            a.act[i] = (v > 0.0 && isfinite(v) && !small) ? 1 : 0;
+            a.alpha[i] = 0.0;
+            a.beta[i] = 0.0;
+        } else if (!a.act[i]) {
+            a.alpha[i] = 0.0;
+            a.beta[i] = 0.0;
         } else if (a.mode == 1) {
             const double r = a.rho[i];
             a.alpha[i] = (v > 0.0 && isfinite(v) && r > 0.0) ? r / v : 0.0;
+            if (a.alpha[i] == 0.0) a.act[i] = 0;
         } else {
             const double r = a.rho[i];
             a.beta[i] = (r > 0.0 && isfinite(v)) ? v / r : 0.0;
             a.rho[i] = v;
+            if (!(v > a.rel * a.rho0[i]) || !(v > a.floor2[i]) || !isfinite(v) || small) a.act[i] = 0;
         }
     }
 }
@@ -1130,19 +1162,21 @@ struct CgUpdateArgs {
     const double* alpha;    // [2][ld]
     const double* invdiag;  // Jacobi preconditioner
     const int* sel;
-    double* partial;        // [n_rb][2][ld]
+    const int* act;         // [2][ld]
+    double* partial;        // [n_rb][3][ld]: r_p.D^-1 r_p, r_d.D^-1 r_d, |r_p|^2
 };
 
 __global__ void __launch_bounds__(BLOCK) k_cg_update(CgUpdateArgs a) {
-    __shared__ double lds[WAVES_PER_BLOCK * 2 * WAVE];
+    __shared__ double lds[WAVES_PER_BLOCK * 3 * WAVE];
     int tile, rb;
     if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int b0 = tile * WAVE + lane;
     const bool valid = b0 < a.sh.ld;
     const long ld = a.sh.ld;
-    double v[2][1] = {{0.0}, {0.0}};
-    if (valid && a.sel[b0]) {
+    double v[3][1] = {{0.0}, {0.0}, {0.0}};
+    const bool on1 = valid && a.act[b0], on2 = valid && a.act[ld + b0];
+    if (valid && a.sel[b0] && (on1 || on2)) {
         const double al1 = a.alpha[b0], al2 = a.alpha[ld + b0];
         const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
         for (int r = 0; r < a.sh.rpw; ++r) {
@@ -1150,18 +1184,23 @@ __global__ void __launch_bounds__(BLOCK) k_cg_update(CgUpdateArgs a) {
             if (row >= a.sh.rows) break;
             const long i = row * ld + b0;
             const double iv = a.invdiag[i];
-            a.w1[i] = a.w1[i] + al1 * a.p1[i];
-            const double n1 = a.r1[i] - al1 * a.s1[i];
-            a.r1[i] = n1;
-            v[0][0] += n1 * n1 * iv;
-            a.w2[i] = a.w2[i] + al2 * a.p2[i];
-            const double n2 = a.r2[i] - al2 * a.s2[i];
-            a.r2[i] = n2;
-            v[1][0] += n2 * n2 * iv;
+            if (on1) {
+                a.w1[i] = a.w1[i] + al1 * a.p1[i];
+                const double n1 = a.r1[i] - al1 * a.s1[i];
+                a.r1[i] = n1;
+                v[0][0] += n1 * n1 * iv;
+                v[2][0] += n1 * n1;
+            }
+            if (on2) {
+                a.w2[i] = a.w2[i] + al2 * a.p2[i];
+                const double n2 = a.r2[i] - al2 * a.s2[i];
+                a.r2[i] = n2;
+                v[1][0] += n2 * n2 * iv;
+            }
         }
     }
-    const int ops[2] = {RSUM, RSUM};
-    block_partials<2, 1>(v, ops, a.partial, rb, ld, b0, lds, valid);
+    const int ops[3] = {RSUM, RSUM, RSUM};
+    block_partials<3, 1>(v, ops, a.partial, rb, ld, b0, lds, valid);
 }
 
 // p = r + beta p (two systems).
@@ -1171,20 +1210,23 @@ struct CgDirArgs {
     const double* beta;
     const double* invdiag;
     const int* sel;
+    const int* act;
 };
 
 __global__ void k_cg_direction(CgDirArgs a) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= a.ld || !a.sel[b]) return;
     const long ld = a.ld;
+    const bool on1 = a.act[b], on2 = a.act[ld + b];
+    if (!on1 && !on2) return;
     const double be1 = a.beta[b], be2 = a.beta[ld + b];
     const int r0 = blockIdx.y * a.rows_per_block;
     const int r1 = min(a.rows, r0 + a.rows_per_block);
     for (int r = r0; r < r1; ++r) {
         const long i = r * ld + b;
         const double iv = a.invdiag[i];
-        a.p1[i] = a.r1[i] * iv + be1 * a.p1[i];
-        a.p2[i] = a.r2[i] * iv + be2 * a.p2[i];
+        if (on1) a.p1[i] = a.r1[i] * iv + be1 * a.p1[i];
+        if (on2) a.p2[i] = a.r2[i] * iv + be2 * a.p2[i];
     }
 }
 
@@ -1507,6 +1549,37 @@ template <typename TI, typename TO>
 __global__ void k_convert(const TI* __restrict__ in, TO* __restrict__ out, long count) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) out[i] = (TO)in[i];
+}
+
+// CG stopping floors: rho = r.D^-1 r below (1e-13 (1 + |b^|))^2 (primal,
+// rounding level) or (1e-12 (1 + |c^|))^2 (dual; reduced costs far below
+// what the 1e-7 relative gap gate resolves).
+// perm maps the view's columns to LPs (nullptr: identity).
+__global__ void k_cg_floor(const double* bnorm, const double* cnorm, const int* perm, double* floor2, int ld) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ld) return;
+    const int b = perm ? perm[j] : j;
+    const double bn = b >= 0 ? bnorm[b] : 0.0, cn = b >= 0 ? cnorm[b] : 0.0;
+    const double fp = 1e-13 * (1.0 + bn), fd = 1e-12 * (1.0 + cn);
+    floor2[j] = fp * fp;
+    floor2[ld + j] = fd * fd;
+}
+
+// Per-LP primal weight: the given value, or ||c^|| / ||b^|| (the PDLP default
+// for a cold start) when `given` is not positive and both norms are nonzero.
+__global__ void k_init_omega(LpState* st, double* om, float* om_f, double* iom, float* iom_f,
+                             const double* bnorm, const double* cnorm, const double* given, double fallback,
+                             int batch) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch) return;
+    double w = given ? given[b] : fallback;
+    if (!(w > 0.0) || !isfinite(w)) {
+        const double bn = bnorm[b], cn = cnorm[b];
+        w = (bn > 0.0 && cn > 0.0 && isfinite(bn) && isfinite(cn)) ? cn / bn : 1.0;
+        w = fmin(fmax(w, 1e-8), 1e8);
+    }
+    st[b].omega = w;
+    om[b] = w; om_f[b] = (float)w; iom[b] = 1.0 / w; iom_f[b] = (float)(1.0 / w);
 }
 
 // Mark polished LPs optimal and frozen.

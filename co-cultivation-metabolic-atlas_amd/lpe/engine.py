@@ -92,7 +92,7 @@ class BatchedLpEngine:
 
     def __init__(self, A, neq: int, batch: int, *, var_positions=None, precision: str = "fp64",
                  method: str = "restarted", step_safety: float = 0.99, theta: float = 1.0,
-                 primal_weight: float = 1.0, ruiz_iterations: int = 10, shared_c: bool = False,
+                 primal_weight: Optional[float] = None, ruiz_iterations: int = 10, shared_c: bool = False,
                  device: Optional[torch.device] = None):
         self._lib = _lib.load()
         if not torch.cuda.is_available():
@@ -125,7 +125,8 @@ class BatchedLpEngine:
         d.method = {"fixed": _lib.METHOD_FIXED, "restarted": _lib.METHOD_RESTARTED}[method]
         d.step_safety = float(step_safety)
         d.theta = float(theta)
-        d.primal_weight = float(primal_weight)
+        # restarted method: None -> ||c^||/||b^|| per LP (PDLP default); fixed: 1.0
+        d.primal_weight = float(primal_weight) if primal_weight is not None else (0.0 if method == "restarted" else 1.0)
         d.ruiz_iterations = int(ruiz_iterations)
         d.shared_c = 1 if shared_c else 0
         handle = C.c_void_p()
@@ -180,6 +181,50 @@ class BatchedLpEngine:
         self.last_info = info.as_dict()
         return dict(self.last_info)
 
+    def solve_with_fallback(self, options: Optional[SolveOptions] = None,
+                            fallback: Optional[SolveOptions] = None, capacity: int = 256):
+        """Solve, then hand the LPs that did not certify to an FP64 engine.
+
+        FP32 iterates stall at about single-precision accuracy; an LP whose
+        active set is not identifiable there is continued in FP64 (warm-started
+        from its FP32 primal, dual and primal weight) on a second engine of
+        ``capacity`` LPs.  Returns (info, x [B,n], y [B,m], status [B]) with
+        the fallback's results merged in.
+        """
+        info = self.solve(options)
+        x, y, status = self.state()
+        bad = np.flatnonzero(status != _lib.STATUS_OPTIMAL)
+        info["fallback_lps"] = int(bad.size)
+        info["fallback_certified"] = 0
+        if bad.size == 0 or self._vectors is None:
+            return info, x, y, status
+        fb_opts = fallback or SolveOptions(max_iterations=20000, polish=True)
+        if getattr(self, "_fallback", None) is None:
+            self._fallback = BatchedLpEngine(self.A, self.neq, capacity, var_positions=self.var_positions,
+                                             precision="fp64", shared_c=self.shared_c, device=self.device)
+        fb = self._fallback
+        omega = self.primal_weight()
+        c_t, rhs_t, lo_t, up_t, vv_t = self._vectors
+        for start in range(0, bad.size, capacity):
+            part = bad[start:start + capacity]
+            pad = np.concatenate([part, np.full(capacity - part.size, part[0])])
+            idx = torch.as_tensor(pad, device=self.device)
+            fb.update(c_t if self.shared_c else c_t[idx], rhs_t[idx], lo_t[idx], up_t[idx],
+                      var_values=None if vv_t is None else vv_t[idx])
+            fb.reset(x[idx], y[idx])
+            fb.set_primal_weight(omega[pad]) if self.method == "restarted" else None
+            fb_info = fb.solve(fb_opts)
+            fx, fy, fs = fb.state()
+            k = part.size
+            sel = torch.as_tensor(part, device=self.device)
+            x[sel] = fx[:k]
+            y[sel] = fy[:k]
+            status[part] = fs[:k]
+            info["fallback_certified"] += int((fs[:k] == _lib.STATUS_OPTIMAL).sum())
+            info["fallback_seconds"] = info.get("fallback_seconds", 0.0) + fb_info["seconds"]
+        info["certified"] = int((status == _lib.STATUS_OPTIMAL).sum())
+        return info, x, y, status
+
     def polish(self, mask=None, cg_iterations: int = 200) -> None:
         m = None if mask is None else np.ascontiguousarray(np.asarray(mask, dtype=np.uint8).reshape(self.batch))
         with torch.cuda.device(self.device):
@@ -226,6 +271,11 @@ class BatchedLpEngine:
         p = lambda a: a.ctypes.data_as(C.c_void_p)
         check(self._lib.lpe_get_scaling(self._h, p(dr), p(dc), p(sg), p(tu)))
         return {"row_scale": dr, "col_scale": dc, "sigma": sg, "tau": tu}
+
+    def set_primal_weight(self, omega) -> None:
+        w = np.ascontiguousarray(np.asarray(omega, dtype=np.float64).reshape(self.batch))
+        with torch.cuda.device(self.device):
+            check(self._lib.lpe_set_primal_weight(self._h, w.ctypes.data_as(C.c_void_p), _stream()))
 
     def primal_weight(self) -> np.ndarray:
         w = np.zeros(self.batch)

@@ -36,7 +36,7 @@ class ConsortiumRollout:
     def __init__(self, model: Optional[Consortium] = None, n_env: int = 4096, *, seed: int = 2,
                  precision: str = "fp32", options: Optional[SolveOptions] = None,
                  warm_start="previous", dt: float = 1.0, conc_range=(1e-2, 10.0),
-                 autotune: bool = True, device: Optional[torch.device] = None):
+                 autotune: bool = True, fallback: bool = True, device: Optional[torch.device] = None):
         self.model = model if model is not None else consortium(seed)
         md = self.model
         self.n_env = int(n_env)
@@ -62,7 +62,9 @@ class ConsortiumRollout:
         self.c = md.c.copy()
         self.x = None
         self.y = None
+        self.omega = None
         self.autotune = autotune
+        self.fallback = fallback
         self.tuning = None
         self.step_index = 0
         self.log: list[dict] = []
@@ -113,8 +115,13 @@ class ConsortiumRollout:
         elif self.warm_start == "previous" and self.x is not None:
             x0, y0 = self.x, self.y
         eng.reset(x0, y0)
-        info = eng.solve(self.options)
-        x, y, status = eng.state()
+        if x0 is not None and self.omega is not None and not callable(self.warm_start):
+            eng.set_primal_weight(self.omega)       # warm start of the primal weight
+        if self.fallback and eng.precision == "fp32":
+            info, x, y, status = eng.solve_with_fallback(self.options)
+        else:
+            info = eng.solve(self.options)
+            x, y, status = eng.state()
         certified = status == _lib.STATUS_OPTIMAL
         torch.cuda.synchronize()
         info["wall_seconds"] = time.perf_counter() - t0
@@ -125,6 +132,7 @@ class ConsortiumRollout:
             self.x = torch.where(ok[:, None], x, self.x) if self.x is not None else x
             self.y = torch.where(ok[:, None], y, self.y) if self.y is not None else y
             self.euler(x, ok)
+            self.omega = eng.primal_weight()
             self.step_index += 1
         return dict(info, certified_mask=certified, x=x, y=y)
 

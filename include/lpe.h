@@ -69,7 +69,7 @@ typedef struct lpe_desc {
     int32_t method;           /* LPE_METHOD_*                               */
     double step_safety;       /* eta in the diagonal metric, (0,1)          */
     double theta;             /* extrapolation, [0,1]                       */
-    double primal_weight;     /* initial omega > 0                          */
+    double primal_weight;     /* initial omega > 0; 0 = ||c^||/||b^|| per LP  */
     int32_t ruiz_iterations;  /* 0 disables equilibration                   */
     int32_t shared_c;         /* 1: c is one [n] vector for every LP        */
 } lpe_desc;
@@ -183,6 +183,13 @@ int lpe_get_scaling(lpe_handle *h, double *row_scale, double *col_scale, double 
 
 /* Per-LP primal weights [B] (host). */
 int lpe_get_primal_weight(lpe_handle *h, double *omega);
+
+/* Set per-LP primal weights [B] (host, > 0) after lpe_reset, e.g. the
+ * weights a previous solve of the same environments converged to (warm
+ * start of the restarted method).  A desc.primal_weight of 0 selects the
+ * PDLP default ||c^||/||b^|| per LP at every reset.  No reference
+ * counterpart (src/gpu_pdhg_corrector.py:99-106 takes one fixed weight). */
+int lpe_set_primal_weight(lpe_handle *h, const double *omega, void *stream);
 
 /* Launch-shape controls used by the autotuner. */
 int lpe_set_launch(lpe_handle *h, int32_t rows_per_wave, int32_t lps_per_lane);

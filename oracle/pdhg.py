@@ -140,7 +140,7 @@ def _kkt_values(As, bs, cs, ls, us, ineq, x, y, dr, dc, neq):
 
 
 def restarted_pdhg(a, rhs, lower, upper, c, neq, *, iterations, check_interval=64, precision="fp64",
-                   x0=None, y0=None, ruiz_iterations=10, eta=0.99, theta=1.0, omega0=1.0,
+                   x0=None, y0=None, ruiz_iterations=10, eta=0.99, theta=1.0, omega0=None,
                    certify=True, gates=GATES):
     """Restarted-averaged PDHG for one LP, decisions as in the engine's k_check_decide."""
     a = csr_matrix(a, dtype=np.float64)
@@ -164,6 +164,9 @@ def restarted_pdhg(a, rhs, lower, upper, c, neq, *, iterations, check_interval=6
     y = np.where(ineq, np.minimum(y, T(0)), y).astype(T)
     xbar, xlast, ylast = x.copy(), x.copy(), y.copy()
     xsum, ysum = np.zeros(n, T), np.zeros(m, T)
+    if omega0 is None or omega0 <= 0:   # PDLP default ||c^|| / ||b^||
+        omega0 = cnorm / bnorm if (bnorm > 0 and cnorm > 0) else 1.0
+        omega0 = min(max(omega0, 1e-8), 1e8)
     omega, kkt_last, kkt_prev, cnt, iters, restarts = float(omega0), np.inf, np.inf, 0, 0, 0
     status, accepted_at = "running", -1
     history = []
