@@ -385,7 +385,29 @@ constexpr int CHECK_RPW = 32;
 // Rows per wave of the polish kernels: few rows per wave for narrow (compacted)
 // batches, where the kernels are latency-bound, more for wide ones, where the
 // per-block partial sums dominate.
-int polish_rpw(int ld) { return ld >= 1024 ? 16 : (ld >= 256 ? 4 : 1); }
+int polish_rpw(int ld) { return ld >= 1024 ? 4 : (ld >= 256 ? 2 : 1); }
+// LPs per lane of the polish gather kernels (wide batches: two FP64 values per lane)
+int polish_lpl(int ld) { return ld >= 512 ? 2 : 1; }
+
+void launch_polish_cols(lpe_handle* h, const PolishColsArgs& a, int ldv) {
+    int grid;
+    PolishColsArgs b = a;
+    const int lpl = polish_lpl(ldv);
+    b.sh = make_shape(a.sh.rows, ldv, ldv, a.sh.rpw, lpl, &grid);
+    if (lpl == 2) hipLaunchKernelGGL(k_polish_cols<2>, dim3(grid), dim3(BLOCK), 0, h->s, b);
+    else hipLaunchKernelGGL(k_polish_cols<1>, dim3(grid), dim3(BLOCK), 0, h->s, b);
+}
+
+// Returns the number of row blocks (partials per LP and value) of the launch.
+int launch_polish_rows(lpe_handle* h, const PolishRowsArgs& a, int ldv) {
+    int grid;
+    PolishRowsArgs b = a;
+    const int lpl = polish_lpl(ldv);
+    b.sh = make_shape(a.sh.rows, ldv, ldv, a.sh.rpw, lpl, &grid);
+    if (lpl == 2) hipLaunchKernelGGL(k_polish_rows<2>, dim3(grid), dim3(BLOCK), 0, h->s, b);
+    else hipLaunchKernelGGL(k_polish_rows<1>, dim3(grid), dim3(BLOCK), 0, h->s, b);
+    return b.sh.n_rb;
+}
 
 // Device check: KKT / certificate values of (current, average), decisions, actions.
 template <typename T>
@@ -556,13 +578,13 @@ void cg_steps(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, in
         PolishColsArgs c1;
         c1.sh = sc; c1.AT = op_scaled_v(h, true, v); c1.p1 = h->d_pp; c1.p2 = h->d_pd; c1.q1 = h->d_q1;
         c1.q2 = h->d_q2; c1.c = nullptr; c1.c_shared = 0; c1.fmask = h->d_fmask; c1.sel = v.sel; c1.act = h->d_cgact;
-        hipLaunchKernelGGL(k_polish_cols, dim3(grid_c), dim3(BLOCK), 0, h->s, c1);
+        launch_polish_cols(h, c1, v.ld);
         PolishRowsArgs c2;
         c2.sh = sr; c2.A = op_scaled_v(h, false, v); c2.q1 = h->d_q1; c2.q2 = h->d_q2; c2.s1 = h->d_s1; c2.s2 = h->d_s2;
         c2.p1 = h->d_pp; c2.p2 = h->d_pd; c2.setup_r = nullptr; c2.setup_p = nullptr; c2.invdiag = h->d_invdiag;
         c2.rmask = h->d_rmask; c2.sel = v.sel; c2.act = h->d_cgact; c2.partial = h->d_partial;
-        hipLaunchKernelGGL(k_polish_rows, dim3(grid_r), dim3(BLOCK), 0, h->s, c2);
-        reduce_ld(h, h->d_partial, h->d_red_a, h->d_ops_sum, 2, sr.n_rb, v.ld);
+        const int nrb2 = launch_polish_rows(h, c2, v.ld);
+        reduce_ld(h, h->d_partial, h->d_red_a, h->d_ops_sum, 2, nrb2, v.ld);
         cs.mode = 1; cs.d0 = h->d_red_a; cs.d1 = h->d_red_a + v.ld; cs.plain = nullptr;
         hipLaunchKernelGGL(k_cg_scalars, dim3(ceil_div(v.ld, 256)), dim3(256), 0, h->s, cs);
         CgUpdateArgs u;
@@ -632,13 +654,13 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
         PolishColsArgs pc;
         pc.sh = sc; pc.AT = op_scaled_v(h, true, v); pc.p1 = h->d_wd; pc.p2 = nullptr; pc.q1 = h->d_q1; pc.q2 = nullptr;
         pc.c = v.c_s; pc.c_shared = h->shared_c; pc.fmask = h->d_fmask; pc.sel = v.sel; pc.act = nullptr;
-        hipLaunchKernelGGL(k_polish_cols, dim3(grid_c), dim3(BLOCK), 0, h->s, pc);
+        launch_polish_cols(h, pc, v.ld);
         PolishRowsArgs pr;
         pr.sh = sr; pr.A = op_scaled_v(h, false, v); pr.q1 = h->d_q1; pr.q2 = nullptr; pr.s1 = nullptr; pr.s2 = nullptr;
         pr.p1 = nullptr; pr.p2 = nullptr; pr.setup_r = h->d_rd; pr.setup_p = h->d_pd; pr.invdiag = h->d_invdiag;
         pr.rmask = h->d_rmask; pr.sel = v.sel; pr.act = nullptr; pr.partial = h->d_partial;
-        hipLaunchKernelGGL(k_polish_rows, dim3(grid_r), dim3(BLOCK), 0, h->s, pr);
-        reduce_ld(h, h->d_partial, h->d_red_a, h->d_ops_sum, 1, sr.n_rb, v.ld);   // rho_d
+        const int nrbs = launch_polish_rows(h, pr, v.ld);
+        reduce_ld(h, h->d_partial, h->d_red_a, h->d_ops_sum, 1, nrbs, v.ld);   // rho_d
         CgScalarArgs cs;
         cs.rho = h->d_rho; cs.rho0 = h->d_rho0; cs.alpha = h->d_alpha; cs.beta = h->d_beta;
         cs.act = h->d_cgact; cs.floor2 = h->d_cgfloor; cs.sel = v.sel; cs.rel = CG_REL; cs.batch = v.ld; cs.ld = v.ld;

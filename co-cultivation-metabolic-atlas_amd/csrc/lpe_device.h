@@ -1010,32 +1010,54 @@ struct PolishColsArgs {
     const int* act;          // [2][ld] CG activity (nullptr: both active)
 };
 
+template <int LPL>
 __global__ void __launch_bounds__(BLOCK) k_polish_cols(PolishColsArgs a) {
     int tile, rb;
     if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    const int b0 = tile * WAVE + lane;
-    if (b0 >= a.sh.ld || !a.sel[b0]) return;
-    const bool on1 = !a.act || a.act[b0], on2 = !a.act || a.act[a.sh.ld + b0];
-    if (!on1 && !on2) return;
+    const int b0 = tile * WAVE * LPL + lane * LPL;
+    if (b0 >= a.sh.ld) return;
     const long ld = a.sh.ld;
+    bool sel[LPL], on1[LPL], on2[LPL];
+    bool any1 = false, any2 = false, anys = false;
+#pragma unroll
+    for (int t = 0; t < LPL; ++t) {
+        sel[t] = a.sel[b0 + t] != 0;
+        on1[t] = sel[t] && (!a.act || a.act[b0 + t]);
+        on2[t] = sel[t] && (!a.act || a.act[ld + b0 + t]);
+        any1 = any1 || on1[t]; any2 = any2 || on2[t]; anys = anys || sel[t];
+    }
+    if (a.c ? !anys : (!any1 && !(a.p2 && any2))) return;
     const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
     for (int r = 0; r < a.sh.rpw; ++r) {
         const int col = uniform(r0 + r);
         if (col >= a.sh.rows) break;
         const long i = col * ld + b0;
-        const double f = a.fmask[i] ? 1.0 : 0.0;
-        double s1[1];
-        row_dot<double, double, double, 1>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.p1, ld, b0, col, s1);
+        double f[LPL];
+#pragma unroll
+        for (int t = 0; t < LPL; ++t) f[t] = a.fmask[i + t] ? 1.0 : 0.0;
         if (a.c) {
-            const double cc = a.c_shared ? a.c[col] : a.c[i];
-            a.q1[i] = f * (cc - s1[0]);
+            double s1[LPL];
+            row_dot<double, double, double, LPL>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.p1, ld, b0, col, s1);
+#pragma unroll
+            for (int t = 0; t < LPL; ++t) {
+                const double cc = a.c_shared ? a.c[col] : a.c[i + t];
+                if (sel[t]) a.q1[i + t] = f[t] * (cc - s1[t]);
+            }
         } else {
-            if (on1) a.q1[i] = f * s1[0];
-            if (a.p2 && on2) {
-                double s2[1];
-                row_dot<double, double, double, 1>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.p2, ld, b0, col, s2);
-                a.q2[i] = f * s2[0];
+            if (any1) {
+                double s1[LPL];
+                row_dot<double, double, double, LPL>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.p1, ld, b0, col, s1);
+#pragma unroll
+                for (int t = 0; t < LPL; ++t)
+                    if (on1[t]) a.q1[i + t] = f[t] * s1[t];
+            }
+            if (a.p2 && any2) {
+                double s2[LPL];
+                row_dot<double, double, double, LPL>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.p2, ld, b0, col, s2);
+#pragma unroll
+                for (int t = 0; t < LPL; ++t)
+                    if (on2[t]) a.q2[i + t] = f[t] * s2[t];
             }
         }
     }
@@ -1058,48 +1080,70 @@ struct PolishRowsArgs {
     double* partial;         // [n_rb][2][ld]
 };
 
+template <int LPL>
 __global__ void __launch_bounds__(BLOCK) k_polish_rows(PolishRowsArgs a) {
-    __shared__ double lds[WAVES_PER_BLOCK * 2 * WAVE];
+    __shared__ double lds[WAVES_PER_BLOCK * 2 * WAVE * LPL];
     int tile, rb;
     if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    const int b0 = tile * WAVE + lane;
+    const int b0 = tile * WAVE * LPL + lane * LPL;
     const bool valid = b0 < a.sh.ld;
     const long ld = a.sh.ld;
-    double v[2][1] = {{0.0}, {0.0}};
-    const bool on1 = valid && (!a.act || a.act[b0]);
-    const bool on2 = valid && (!a.act || a.act[ld + b0]);
-    if (valid && a.sel[b0] && (on1 || on2 || a.setup_r)) {
+    double v[2][LPL];
+    bool on1[LPL], on2[LPL], sel[LPL];
+    bool any1 = false, any2 = false, anys = false;
+#pragma unroll
+    for (int t = 0; t < LPL; ++t) {
+        v[0][t] = 0.0; v[1][t] = 0.0;
+        sel[t] = valid && a.sel[b0 + t] != 0;
+        on1[t] = sel[t] && (!a.act || a.act[b0 + t]);
+        on2[t] = sel[t] && (!a.act || a.act[ld + b0 + t]);
+        any1 = any1 || on1[t]; any2 = any2 || on2[t]; anys = anys || sel[t];
+    }
+    const bool setup = a.setup_r != nullptr;
+    if (setup ? anys : (any1 || any2)) {
         const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
         for (int r = 0; r < a.sh.rpw; ++r) {
             const int row = uniform(r0 + r);
             if (row >= a.sh.rows) break;
             const long i = row * ld + b0;
-            const double on = a.rmask[i] ? 1.0 : 0.0;
-            double t1[1];
-            row_dot<double, double, double, 1>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.q1, ld, b0, row, t1);
-            const double sv1 = on * t1[0];
-            if (a.setup_r) {
-                const double z = sv1 * a.invdiag[i];
-                a.setup_r[i] = sv1; a.setup_p[i] = z;
-                v[0][0] += sv1 * z;
-            } else {
-                if (on1) {
-                    a.s1[i] = sv1;
-                    v[0][0] += a.p1[i] * sv1;
+            double on[LPL];
+#pragma unroll
+            for (int t = 0; t < LPL; ++t) on[t] = a.rmask[i + t] ? 1.0 : 0.0;
+            if (setup || any1) {
+                double t1[LPL];
+                row_dot<double, double, double, LPL>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.q1, ld, b0, row, t1);
+#pragma unroll
+                for (int t = 0; t < LPL; ++t) {
+                    const double sv1 = on[t] * t1[t];
+                    if (setup) {
+                        if (sel[t]) {
+                            const double z = sv1 * a.invdiag[i + t];
+                            a.setup_r[i + t] = sv1; a.setup_p[i + t] = z;
+                            v[0][t] += sv1 * z;
+                        }
+                    } else if (on1[t]) {
+                        a.s1[i + t] = sv1;
+                        v[0][t] += a.p1[i + t] * sv1;
+                    }
                 }
-                if (on2) {
-                    double t2[1];
-                    row_dot<double, double, double, 1>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.q2, ld, b0, row, t2);
-                    const double sv2 = on * t2[0];
-                    a.s2[i] = sv2;
-                    v[1][0] += a.p2[i] * sv2;
+            }
+            if (!setup && any2) {
+                double t2[LPL];
+                row_dot<double, double, double, LPL>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.q2, ld, b0, row, t2);
+#pragma unroll
+                for (int t = 0; t < LPL; ++t) {
+                    if (on2[t]) {
+                        const double sv2 = on[t] * t2[t];
+                        a.s2[i + t] = sv2;
+                        v[1][t] += a.p2[i + t] * sv2;
+                    }
                 }
             }
         }
     }
     const int ops[2] = {RSUM, RSUM};
-    block_partials<2, 1>(v, ops, a.partial, rb, ld, b0, lds, valid);
+    block_partials<2, LPL>(v, ops, a.partial, rb, ld, b0, lds, valid);
 }
 
 // Per-LP CG scalars (Jacobi-preconditioned CG, rho = r . D^-1 r), two systems

@@ -104,8 +104,10 @@ def test_consortium_batch_per_lp_coefficients(device):
                   var_positions=md.coupling_nnz.ravel())
     eng.update(md.c, cb.rhs(), cb.lower(), cb.upper(), var_values=cb.coupling_values().reshape(cb.batch, -1))
     eng.reset()
-    info = eng.solve(max_iterations=40000, check_interval=64, polish=True)
-    x, y, status = eng.state()
+    # FP32 stalls on some of these small consortium LPs: the FP64 fallback finishes them
+    from lpe.engine import SolveOptions
+    info, x, y, status = eng.solve_with_fallback(SolveOptions(max_iterations=8000, polish=True),
+                                                 SolveOptions(max_iterations=40000, polish=True), capacity=64)
     assert info["certified"] == cb.batch, info
     for b in range(cb.batch):
         p = cb.problem(b)
