@@ -58,7 +58,7 @@ def _highs_worker(args):
     return r.status, r.fun, r.seconds
 
 
-def cpu_baseline(model, conc, biomass, gpu_x, seconds_budget):
+def cpu_baseline(model, conc, biomass, gpu_x, gpu_ok, seconds_budget):
     """HiGHS (the reference's exact CPU path, via SciPy) on a bounded sample
     of the first timed step's LPs, one LP per worker process."""
     import multiprocessing as mp
@@ -92,7 +92,7 @@ def cpu_baseline(model, conc, biomass, gpu_x, seconds_budget):
     funs = [r[1] for r in res]
     errs = []
     for b in range(n_sample):
-        if res[b][0] != "optimal" or gpu_x is None:
+        if res[b][0] != "optimal" or gpu_x is None or not gpu_ok[b]:
             continue
         obj = float(model.c @ gpu_x[b])
         errs.append(cert.relative_objective_error(obj, funs[b]))
@@ -100,6 +100,60 @@ def cpu_baseline(model, conc, biomass, gpu_x, seconds_budget):
                 sample=f"{done} LPs of the first timed step (cold HiGHS {highs.version()} highs-ds, "
                        f"1 thread per LP, {workers} worker processes), {elapsed:.1f} s",
                 solver=highs.version(), highs_single_lp_seconds=one), errs, n_sample
+
+
+def probe_kernels(args):
+    """Launch only the two step kernels and a copy of known size, for hardware
+    counter runs (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE): the copy's counter
+    readings scale the step kernels' readings to bytes."""
+    import torch
+    from lpe.rollout import ConsortiumRollout
+    from lpe.workloads import consortium
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    model = consortium(seed=args.seed)
+    ro = ConsortiumRollout(model, args.envs, seed=args.seed, precision="fp32", autotune=False, device=dev)
+    rhs, lower, upper, coupling = ro.lp_inputs()
+    eng = ro.engine
+    eng.update(ro.c, rhs, lower, upper, var_values=coupling)
+    eng.set_launch(args.rows_per_wave, args.lps_per_lane)
+    eng.reset()
+    eng.run(64, chunk=64, use_graph=False)
+    ms_p = eng.time_kernel("primal", reps=args.probe_reps)
+    ms_d = eng.time_kernel("dual", reps=args.probe_reps)
+    n_copy = 256 << 20                                   # 256 Mi float32 = 1 GiB read + 1 GiB written
+    src = torch.ones(n_copy, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(args.probe_reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    m, n = model.shape
+    print(json.dumps({"probe": "step kernels", "envs": args.envs, "rows_per_wave": args.rows_per_wave,
+                      "lps_per_lane": args.lps_per_lane, "reps": args.probe_reps,
+                      "primal_ms": ms_p, "dual_ms": ms_d, "copy_bytes_read": 4 * n_copy,
+                      "copy_bytes_written": 4 * n_copy,
+                      "primal_algorithmic_bytes": _primal_bytes(args.envs, m, n, model.A.nnz,
+                                                                model.coupling_nnz.size, 4),
+                      "dual_algorithmic_bytes": _dual_bytes(args.envs, m, n, model.A.nnz,
+                                                            model.coupling_nnz.size, 4)}), flush=True)
+
+
+def measured_traffic(path, envs, lps_per_lane):
+    """HBM bytes per primal-step launch from a committed counter summary:
+    the line 'primal_step_hbm_bytes envs=E lpl=L fetch=F write=W'."""
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if parts[:1] != ["primal_step_hbm_bytes"]:
+                    continue
+                kv = dict(p.split("=", 1) for p in parts[1:] if "=" in p)
+                if int(kv["envs"]) == envs and int(kv["lpl"]) == lps_per_lane:
+                    return int(float(kv["fetch"]) + float(kv["write"]))
+    except (OSError, ValueError, KeyError):
+        return None
+    return None
 
 
 def main():
@@ -116,11 +170,20 @@ def main():
     ap.add_argument("--cg-iterations", type=int, default=200, help="CG steps per polish round")
     ap.add_argument("--fp32-iterations", type=int, default=6000,
                     help="FP32 budget per step; uncertified LPs continue on the FP64 fallback engine")
+    ap.add_argument("--probe-kernels", action="store_true",
+                    help="only launch the step kernels and a copy of known size (hardware counter runs)")
+    ap.add_argument("--probe-reps", type=int, default=5)
+    ap.add_argument("--rows-per-wave", type=int, default=2, help="launch shape of --probe-kernels")
+    ap.add_argument("--lps-per-lane", type=int, default=4, help="launch shape of --probe-kernels")
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "pmc_step_kernels_consortium.md"))
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the LPs are independent and would shard "
                                    "by environment with no per-iteration communication; run --gpus 1"}))
         sys.exit(2)
+    if args.probe_kernels:
+        probe_kernels(args)
+        return
 
     import numpy as np
     import torch
@@ -146,7 +209,7 @@ def main():
               f"{target.log[-1]['iterations']} iterations, {target.log[-1]['wall_seconds']:.2f} s",
               file=sys.stderr, flush=True)
     snapshots, certified, step_logs = [], 0, []
-    first_x = None
+    first_x = first_ok = None
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -154,7 +217,7 @@ def main():
         out = target.step()
         certified += int(out["certified"])
         if k == 0:
-            first_x = out["x"]
+            first_x, first_ok = out["x"], np.asarray(out["certified_mask"])
         step_logs.append({kk: out.get(kk) for kk in ("certified", "iterations", "restarts", "polish_attempts",
                                                      "pdhg_ms", "check_ms", "polish_ms", "wall_seconds",
                                                      "fallback_lps", "fallback_certified", "fallback_seconds")})
@@ -170,7 +233,9 @@ def main():
     eng.reset(target.x, target.y)
     ms_primal = eng.time_kernel("primal", reps=20)
     ms_dual = eng.time_kernel("dual", reps=20)
+    lpl = int((target_tuning or {}).get("lps_per_lane", 1))
     pb = _primal_bytes(B, m, n, nnz, nvar, 4)
+    traffic = measured_traffic(args.traffic_file, B, lpl)
     db = _dual_bytes(B, m, n, nnz, nvar, 4)
     achieved = pb / (ms_primal * 1e-3) / 1e9
 
@@ -192,7 +257,10 @@ def main():
         p_logs.append({kk: out[kk] for kk in ("certified", "iterations", "restarts", "pdhg_ms", "check_ms",
                                               "wall_seconds")})
     plain_value = p_cert / p_time if p_time > 0 else 0.0
-    ms_primal_fp64 = plain.engine.time_kernel("primal", reps=10) if p_logs else None
+    ms_primal_fp64 = None
+    if p_logs:
+        plain.engine.reset()                        # every LP active again (finished LPs skip their lanes)
+        ms_primal_fp64 = plain.engine.time_kernel("primal", reps=10)
     print(f"# plain: {p_cert}/{B * len(p_logs)} certified in {p_time:.2f} s", file=sys.stderr, flush=True)
     del plain
     torch.cuda.empty_cache()
@@ -212,9 +280,9 @@ def main():
         "target_log": step_logs,
         "autotune": {k: v for k, v in (target_tuning or {}).items() if k != "timings"},
         "autotune_timings_ms_per_iteration": (target_tuning or {}).get("timings"),
-        "roofline": {"bound": "hbm", "kernel": "k_primal_step<float,1,false>",
+        "roofline": {"bound": "hbm", "kernel": f"k_primal_step<float,{lpl},false>",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": pb, "avg_launch_ms": round(ms_primal, 5),
                      "dual_step": {"algorithmic_bytes_per_launch": db, "avg_launch_ms": round(ms_dual, 5),
                                    "achieved": round(db / (ms_dual * 1e-3) / 1e9, 1)},
@@ -222,7 +290,7 @@ def main():
     }
     if not args.no_cpu:
         cb, errs, n_sample = cpu_baseline(model, snapshots[0][0].cpu().numpy(), snapshots[0][1].cpu().numpy(),
-                                          first_x_host, args.cpu_seconds)
+                                          first_x_host, first_ok, args.cpu_seconds)
         result["cpu_baseline"] = cb
         result["highs_check"] = {"sample": n_sample, "checked": len(errs),
                                  "max_rel_obj_err": max(errs) if errs else None,

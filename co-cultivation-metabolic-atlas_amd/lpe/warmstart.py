@@ -1,0 +1,321 @@
+"""Learned warm start for the dFBA step: a multilayer perceptron on MI355X.
+
+The network maps an environment's LP parameters (log medium concentrations,
+species biomass, Michaelis-Menten uptake bounds) to an initial primal x0, an
+initial dual y0 and one logit per column for "this variable sits at a bound".
+It trains on solutions the engine itself certified.  Every GEMM, activation,
+loss gradient and Adam update runs in libwarmstart.so (hand-written HIP: BF16
+MFMA GEMMs with FP32 accumulation, FP32 master weights); torch only allocates
+device memory and does the input/output standardisation.
+
+The reference has no MLP warm start; its learned warm starts are GRU/GNN
+research modules (src/temporal_lp_model.py:58-129,
+src/supervised_graph_lp.py:13-91).  Cold start stays the fallback: a predicted
+start is only a starting point, the engine's certificate decides acceptance,
+and ``LearnedWarmStart`` returns no start before the network is trained.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libwarmstart.so")
+EXPORTS = ("ws_gemm_bf16", "ws_bias_act", "ws_act_backward", "ws_loss", "ws_adam", "ws_to_bf16",
+           "ws_version")
+
+_lib = None
+
+
+def load():
+    """Load libwarmstart.so (raises ImportError when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    P, I, F = C.c_void_p, C.c_int, C.c_float
+    lib.ws_gemm_bf16.argtypes = [I, I, I, P, I, I, P, I, I, P, I, I, P]
+    lib.ws_bias_act.argtypes = [P, P, P, I, I, I, P]
+    lib.ws_act_backward.argtypes = [P, P, P, P, I, I, I, P]
+    lib.ws_loss.argtypes = [P, P, P, P, I, I, I, I, F, F, F, P]
+    lib.ws_adam.argtypes = [P, P, P, P, P, C.c_int64, F, F, F, F, I, P]
+    lib.ws_to_bf16.argtypes = [P, P, C.c_int64, P]
+    lib.ws_version.restype = C.c_char_p
+    for name in EXPORTS[:-1]:
+        getattr(lib, name).restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ok(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed with status {rc}")
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def gemm(a: torch.Tensor, b: torch.Tensor, *, trans_a: bool = False, trans_b: bool = False,
+         out: Optional[torch.Tensor] = None, accumulate: bool = False) -> torch.Tensor:
+    """out = op(a) @ op(b) with BF16 operands and FP32 result (ws_gemm_bf16)."""
+    lib = load()
+    if a.dtype != torch.bfloat16 or b.dtype != torch.bfloat16:
+        raise ValueError("gemm operands must be bfloat16")
+    if not (a.is_contiguous() and b.is_contiguous()):
+        raise ValueError("gemm operands must be contiguous")
+    m, k = (a.shape[1], a.shape[0]) if trans_a else (a.shape[0], a.shape[1])
+    kb, n = (b.shape[1], b.shape[0]) if trans_b else (b.shape[0], b.shape[1])
+    if k != kb:
+        raise ValueError(f"inner dimensions differ: {k} vs {kb}")
+    if out is None:
+        out = torch.empty((m, n), dtype=torch.float32, device=a.device)
+    elif out.shape != (m, n) or out.dtype != torch.float32 or not out.is_contiguous():
+        raise ValueError("out must be a contiguous float32 [M, N] tensor")
+    _ok(lib.ws_gemm_bf16(m, n, k, _ptr(a), a.shape[1], int(trans_a), _ptr(b), b.shape[1], int(trans_b),
+                         _ptr(out), n, int(accumulate), _stream()), "ws_gemm_bf16")
+    return out
+
+
+def to_bf16(x: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    lib = load()
+    x = x.contiguous()
+    if x.dtype != torch.float32:
+        raise ValueError("to_bf16 takes float32")
+    out = torch.empty(x.shape, dtype=torch.bfloat16, device=x.device) if out is None else out
+    _ok(lib.ws_to_bf16(_ptr(x), _ptr(out), x.numel(), _stream()), "ws_to_bf16")
+    return out
+
+
+class WarmStartNet:
+    """MLP n_in -> hidden x layers (GELU) -> nx + ny + na, trained with Adam.
+
+    Weights are stored [in][out] (FP32 master, BF16 copy for the GEMMs).
+    Inputs and regression targets are standardised per column with statistics
+    fixed at the first call of ``fit_normalization``.
+    """
+
+    def __init__(self, n_in: int, nx: int, ny: int, na: int, *, hidden: int = 1024, layers: int = 3,
+                 device=None, seed: int = 0, loss_weights=(1.0, 1.0, 1.0)):
+        load()
+        self.device = device or torch.device("cuda", torch.cuda.current_device())
+        self.n_in, self.nx, self.ny, self.na = int(n_in), int(nx), int(ny), int(na)
+        self.n_out = self.nx + self.ny + self.na
+        self.dims = [self.n_in] + [int(hidden)] * int(layers) + [self.n_out]
+        self.loss_weights = tuple(float(w) for w in loss_weights)
+        g = torch.Generator(device="cpu").manual_seed(seed)
+        self.w, self.b = [], []
+        for k, n in zip(self.dims[:-1], self.dims[1:]):
+            bound = 1.0 / math.sqrt(k)
+            self.w.append(((torch.rand((k, n), generator=g, dtype=torch.float32) * 2 - 1) * bound)
+                          .to(self.device))
+            self.b.append(torch.zeros(n, dtype=torch.float32, device=self.device))
+        self.wb = [to_bf16(w) for w in self.w]
+        self.params = self.w + self.b
+        self.m = [torch.zeros_like(p) for p in self.params]
+        self.v = [torch.zeros_like(p) for p in self.params]
+        self.step_count = 0
+        self.in_mean = self.in_std = self.out_mean = self.out_std = None
+        self.trained = False
+
+    @property
+    def n_params(self) -> int:
+        return sum(p.numel() for p in self.params)
+
+    def flops_per_sample(self) -> int:
+        return 2 * sum(k * n for k, n in zip(self.dims[:-1], self.dims[1:]))
+
+    def set_params(self, weights: Sequence, biases: Sequence) -> None:
+        for l, (w, b) in enumerate(zip(weights, biases)):
+            self.w[l].copy_(torch.as_tensor(np.asarray(w), dtype=torch.float32))
+            self.b[l].copy_(torch.as_tensor(np.asarray(b), dtype=torch.float32))
+            to_bf16(self.w[l], self.wb[l])
+
+    # ------------------------------------------------------------ statistics
+    def fit_normalization(self, features: torch.Tensor, targets: torch.Tensor) -> None:
+        f = features.to(torch.float64)
+        self.in_mean = f.mean(0).to(torch.float32)
+        self.in_std = f.std(0).clamp_min(1e-6).to(torch.float32)
+        t = targets[:, :self.nx + self.ny].to(torch.float64)
+        self.out_mean = t.mean(0).to(torch.float32)
+        self.out_std = t.std(0).clamp_min(1e-6).to(torch.float32)
+
+    def _inputs(self, features):
+        x = (features.to(torch.float32) - self.in_mean) / self.in_std
+        return to_bf16(x.contiguous())
+
+    def _targets(self, targets):
+        t = targets.to(torch.float32).clone()
+        k = self.nx + self.ny
+        t[:, :k] = (t[:, :k] - self.out_mean) / self.out_std
+        return t.contiguous()
+
+    # ------------------------------------------------------------- forward
+    def _forward(self, xb: torch.Tensor, keep: bool):
+        lib = load()
+        s = _stream()
+        mb = xb.shape[0]
+        h = xb
+        zs, hs = [], [xb]
+        last = len(self.w) - 1
+        for l in range(len(self.w)):
+            n = self.dims[l + 1]
+            z = gemm(h, self.wb[l])
+            hn = None if l == last else torch.empty((mb, n), dtype=torch.bfloat16, device=self.device)
+            _ok(lib.ws_bias_act(_ptr(z), _ptr(self.b[l]), _ptr(hn), mb, n, 0 if l == last else 1, s),
+                "ws_bias_act")
+            zs.append(z)
+            if hn is not None:
+                hs.append(hn)
+                h = hn
+        return zs[-1], (zs, hs) if keep else None
+
+    def predict_raw(self, features: torch.Tensor) -> torch.Tensor:
+        """Network output [M, nx+ny+na] in standardised units (logits last)."""
+        o, _ = self._forward(self._inputs(features), keep=False)
+        return o
+
+    def predict(self, features: torch.Tensor):
+        """(x [M,nx], y [M,ny], bound logits [M,na]) in original units."""
+        o = self.predict_raw(features)
+        k = self.nx + self.ny
+        reg = o[:, :k] * self.out_std + self.out_mean
+        return reg[:, :self.nx], reg[:, self.nx:k], o[:, k:]
+
+    # ------------------------------------------------------------ training
+    def loss_and_grad(self, xb: torch.Tensor, tb: torch.Tensor):
+        """Forward + backward on a standardised BF16 batch; returns (loss,
+        gradients) with gradients listed like ``params``."""
+        lib = load()
+        s = _stream()
+        mb = xb.shape[0]
+        o, (zs, hs) = self._forward(xb, keep=True)
+        d_out = torch.empty_like(o)
+        row_loss = torch.empty(mb, dtype=torch.float32, device=self.device)
+        wx, wy, wa = self.loss_weights
+        _ok(lib.ws_loss(_ptr(o), _ptr(tb), _ptr(d_out), _ptr(row_loss), mb, self.nx, self.ny, self.na,
+                        wx, wy, wa, s), "ws_loss")
+        gw = [None] * len(self.w)
+        gb = [torch.zeros_like(b) for b in self.b]
+        dh = d_out
+        for l in range(len(self.w) - 1, -1, -1):
+            n = self.dims[l + 1]
+            dz = torch.empty((mb, n), dtype=torch.bfloat16, device=self.device)
+            _ok(lib.ws_act_backward(_ptr(dh), _ptr(zs[l]), _ptr(dz), _ptr(gb[l]), mb, n,
+                                    0 if l == len(self.w) - 1 else 1, s), "ws_act_backward")
+            gw[l] = gemm(hs[l], dz, trans_a=True)
+            if l > 0:
+                dh = gemm(dz, self.wb[l], trans_b=True)
+        return row_loss.sum(), gw + gb
+
+    def adam_step(self, grads, lr: float, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8) -> None:
+        lib = load()
+        s = _stream()
+        self.step_count += 1
+        nw = len(self.w)
+        for i, (p, g, m, v) in enumerate(zip(self.params, grads, self.m, self.v)):
+            wb = self.wb[i] if i < nw else torch.empty(p.shape, dtype=torch.bfloat16, device=self.device)
+            _ok(lib.ws_adam(_ptr(p), _ptr(g), _ptr(m), _ptr(v), _ptr(wb), p.numel(), lr, beta1, beta2, eps,
+                            self.step_count, s), "ws_adam")
+
+    def fit(self, features: torch.Tensor, targets: torch.Tensor, *, epochs: int = 20, batch: int = 1024,
+            lr: float = 1e-3, lr_final: float = 1e-4, seed: int = 0, log=None) -> list:
+        """Mini-batch Adam over device-resident data; cosine learning-rate decay."""
+        if self.in_mean is None:
+            self.fit_normalization(features, targets)
+        x_all = self._inputs(features)
+        t_all = self._targets(targets)
+        n = x_all.shape[0]
+        batch = min(batch, n)
+        steps_per_epoch = max(1, n // batch)
+        total = epochs * steps_per_epoch
+        gen = torch.Generator(device=self.device).manual_seed(seed)
+        history = []
+        k = 0
+        for ep in range(epochs):
+            perm = torch.randperm(n, generator=gen, device=self.device)
+            acc = torch.zeros((), dtype=torch.float32, device=self.device)
+            for i in range(steps_per_epoch):
+                idx = perm[i * batch:(i + 1) * batch]
+                loss, grads = self.loss_and_grad(x_all[idx].contiguous(), t_all[idx].contiguous())
+                rate = lr_final + 0.5 * (lr - lr_final) * (1 + math.cos(math.pi * k / max(1, total - 1)))
+                self.adam_step(grads, rate)
+                acc += loss
+                k += 1
+            history.append(float(acc) / steps_per_epoch)
+            if log:
+                log(f"epoch {ep}: loss {history[-1]:.5f}")
+        self.trained = True
+        return history
+
+
+# --------------------------------------------------------------- dFBA step
+def consortium_features(model, conc: torch.Tensor, lower: torch.Tensor, biomass: torch.Tensor) -> torch.Tensor:
+    """[B, 2K + S + S*K]: log concentrations, concentrations, biomass and the
+    uptake lower bounds of every species' medium exchanges."""
+    mc = torch.as_tensor(model.medium_cols.ravel(), device=lower.device, dtype=torch.long)
+    return torch.cat([torch.log(conc + 1e-6), conc, biomass, lower[:, mc]], dim=1).to(torch.float32)
+
+
+def bound_labels(x: torch.Tensor, lower: torch.Tensor, upper: torch.Tensor, tol: float = 1e-6) -> torch.Tensor:
+    width = torch.clamp(upper - lower, min=1.0)
+    at = ((x - lower) <= tol * width) | ((upper - x) <= tol * width)
+    return at.to(torch.float32)
+
+
+class LearnedWarmStart:
+    """Warm-start callable for ``ConsortiumRollout``: returns the network's
+    (x0, y0), with variables the network places at a bound snapped there.
+    Before training it returns (None, None), i.e. a cold start."""
+
+    def __init__(self, model, *, hidden: int = 1024, layers: int = 3, device=None, seed: int = 0,
+                 snap: bool = True):
+        self.model = model
+        m, n = model.shape
+        self.n_features = 2 * model.medium_cols.shape[1] + model.medium_cols.shape[0] + model.medium_cols.size
+        self.net = WarmStartNet(self.n_features, n, m, n, hidden=hidden, layers=layers, device=device, seed=seed)
+        self.snap = snap
+        self.features, self.targets = [], []
+
+    def add_solutions(self, conc, lower, upper, biomass, x, y, certified) -> int:
+        """Keep the certified LPs of one solved step as training samples."""
+        ok = torch.as_tensor(np.asarray(certified), device=x.device)
+        if not bool(ok.any()):
+            return 0
+        f = consortium_features(self.model, conc, lower, biomass)[ok]
+        t = torch.cat([x[ok].to(torch.float32), y[ok].to(torch.float32),
+                       bound_labels(x[ok], lower[ok], upper[ok])], dim=1)
+        self.features.append(f)
+        self.targets.append(t)
+        return int(ok.sum())
+
+    def train(self, **kwargs) -> list:
+        f = torch.cat(self.features)
+        t = torch.cat(self.targets)
+        return self.net.fit(f, t, **kwargs)
+
+    def __call__(self, conc, lower, biomass):
+        if not self.net.trained:
+            return None, None
+        md = self.model
+        upper = torch.as_tensor(md.upper, device=lower.device).expand_as(lower)
+        x, y, logits = self.net.predict(consortium_features(md, conc, lower, biomass))
+        x = x.to(torch.float64)
+        x = torch.minimum(torch.maximum(x, lower), upper)
+        if self.snap:
+            near_lo = (x - lower) <= (upper - x)
+            at = logits > 0
+            x = torch.where(at & near_lo, lower, torch.where(at, upper, x))
+        return x.contiguous(), y.to(torch.float64).contiguous()

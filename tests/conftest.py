@@ -4,7 +4,8 @@ import sys
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-for p in (ROOT, os.path.join(ROOT, "co-cultivation-metabolic-atlas_amd")):
+PKG = os.path.join(ROOT, "co-cultivation-metabolic-atlas_amd")
+for p in (ROOT, PKG):
     if p not in sys.path:
         sys.path.insert(0, p)
 
