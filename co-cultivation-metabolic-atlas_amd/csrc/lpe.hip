@@ -371,8 +371,8 @@ int run_iterations(lpe_handle* h, long iters, int chunk, int use_graph) {
 }
 
 void reduce_ld(lpe_handle* h, const double* partial, double* out, const int* ops, int nv, int n_rb, int ld) {
-    hipLaunchKernelGGL(k_reduce_partials, dim3(ceil_div(ld, 64), nv), dim3(256), 0, h->s, partial, out, ops,
-                       nv, n_rb, ld);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(ceil_div(ld, 64), nv), dim3(RED_WAVES * 64), 0, h->s, partial, out,
+                       ops, nv, n_rb, ld);
 }
 
 void reduce(lpe_handle* h, const double* partial, double* out, const int* ops, int nv, int n_rb) {
@@ -726,75 +726,92 @@ void gather_cols(lpe_handle* h, const T* src, T* dst, int rows, int ldp) {
                        (const int*)h->d_perm, ROWS_PER_BLOCK_EW);
 }
 
-template <typename T>
-int polish_rounds(lpe_handle* h, const View& v, std::vector<int>& hsel, const T* xin, const T* yin, int cg_iters,
-                  double tol, double ep, double ed, double eg, std::vector<int>& passed) {
+// Certificate of the view's polished point (d_xpol, d_ypol): passed[b] (only
+// for selected b) and the metrics rows [ld][8] on the host.
+int certify_view(lpe_handle* h, const View& v, const std::vector<int>& hsel, double ep, double ed, double eg,
+                 std::vector<int>& passed, std::vector<double>& met) {
     const int ld = v.ld;
-    CHK(polish_t<T>(h, v, hsel.data(), xin, yin, 0, cg_iters, tol));
     CHK(launch_certificate_v(h, v, h->d_xpol, h->d_ypol, ep, ed, eg));
-    std::vector<double> met((size_t)8 * ld);
     passed.assign(ld, 0);
+    met.assign((size_t)8 * ld, 0.0);
     HIPCHK(hipMemcpyAsync(passed.data(), h->d_passed, sizeof(int) * ld, hipMemcpyDeviceToHost, h->s));
     HIPCHK(hipMemcpyAsync(met.data(), h->d_metrics, sizeof(double) * 8 * ld, hipMemcpyDeviceToHost, h->s));
     HIPCHK(hipStreamSynchronize(h->s));
-    // round 2 (iterative refinement from round 1's FP64 result) only where round
-    // 1 found the optimal face (small gap) but the CG residuals were above the gates
-    std::vector<int> sel2(ld, 0);
-    int again = 0;
-    for (int b = 0; b < ld; ++b) {
-        const bool close = met[(size_t)b * 8 + 5] == 1.0 && met[(size_t)b * 8 + 2] <= 1e-4;
-        sel2[b] = (hsel[b] && !passed[b] && close) ? 1 : 0;
-        again += sel2[b];
-    }
-    if (again > 0 && h->m > 0) {
-        int* dsel = const_cast<int*>(v.sel);
-        HIPCHK(hipMemcpyAsync(dsel, sel2.data(), sizeof(int) * ld, hipMemcpyHostToDevice, h->s));
-        CHK(polish_t<double>(h, v, sel2.data(), h->d_xpol, h->d_ypol, 1, cg_iters, tol));
-        CHK(launch_certificate_v(h, v, h->d_xpol, h->d_ypol, ep, ed, eg));
-        std::vector<int> p2(ld, 0);
-        HIPCHK(hipMemcpyAsync(p2.data(), h->d_passed, sizeof(int) * ld, hipMemcpyDeviceToHost, h->s));
-        HIPCHK(hipMemcpyAsync(dsel, hsel.data(), sizeof(int) * ld, hipMemcpyHostToDevice, h->s));
-        HIPCHK(hipStreamSynchronize(h->s));
-        for (int b = 0; b < ld; ++b)
-            if (sel2[b]) passed[b] = p2[b];
-    }
     for (int b = 0; b < ld; ++b) passed[b] = passed[b] && hsel[b];
     return LPE_OK;
 }
 
-// Polish the LPs selected in h->h_sel.  Selections of at most half the batch
-// are gathered into a dense prefix of the compaction buffers first, so the
-// cost follows the number of LPs polished.  On return h->h_sel marks the LPs
-// that passed the certificate, their results are in d_xres / d_yres, and
-// d_sel mirrors h->h_sel.
-int polish_selected(lpe_handle* h, int cg_iters, double tol, double ep, double ed, double eg) {
-    const bool f32 = h->method != LPE_METHOD_FIXED && h->precision == LPE_PREC_FP32;
-    std::vector<int> idx;
-    for (int b = 0; b < h->B; ++b)
-        if (h->h_sel[b]) idx.push_back(b);
-    const int P = (int)idx.size();
-    std::vector<int> passed;
-    if (P == 0) return LPE_OK;
-    if (2 * P > h->ld) {
-        View v = full_view(h);
-        std::vector<int> hsel(h->h_sel, h->h_sel + h->ld);
-        HIPCHK(hipMemcpyAsync(h->d_sel, h->h_sel, sizeof(int) * h->ld, hipMemcpyHostToDevice, h->s));
-        if (f32) CHK(polish_rounds<float>(h, v, hsel, h->it32.x, h->it32.y, cg_iters, tol, ep, ed, eg, passed));
-        else CHK(polish_rounds<double>(h, v, hsel, h->it64.x, h->it64.y, cg_iters, tol, ep, ed, eg, passed));
-        for (int b = 0; b < h->ld; ++b) h->h_sel[b] = passed[b];
-        HIPCHK(hipMemcpyAsync(h->d_sel, h->h_sel, sizeof(int) * h->ld, hipMemcpyHostToDevice, h->s));
-        hipLaunchKernelGGL(k_masked_copy, ew_grid(h->n, h->ld), dim3(256), 0, h->s, (const double*)h->d_xpol,
-                           h->d_xres, h->n, h->ld, ROWS_PER_BLOCK_EW, (const int*)h->d_sel, 1);
-        if (h->m > 0)
-            hipLaunchKernelGGL(k_masked_copy, ew_grid(h->m, h->ld), dim3(256), 0, h->s, (const double*)h->d_ypol,
-                               h->d_yres, h->m, h->ld, ROWS_PER_BLOCK_EW, (const int*)h->d_sel, 1);
-        HIPCHK(hipGetLastError());
-        return LPE_OK;
+void trace_failures(const char* what, const std::vector<int>& hsel, const std::vector<int>& passed,
+                    const std::vector<double>& met, double ep, double ed, double eg) {
+    if (!std::getenv("LPE_TRACE")) return;
+    // failed LPs by gate, with the median primal residual and gap among them
+    int nf = 0, fp = 0, fd = 0, fg = 0;
+    std::vector<double> prim, gap;
+    for (size_t b = 0; b < hsel.size(); ++b) {
+        if (!hsel[b] || passed[b]) continue;
+        const double* r = &met[b * 8];
+        nf++;
+        fp += !(r[0] <= ep);
+        fd += !(r[1] <= ed);
+        fg += !(r[2] <= eg);
+        prim.push_back(r[0]);
+        gap.push_back(r[2]);
     }
-    // compacted view: power-of-two width buckets keep the captured CG graphs few
-    int ldp = 64;
+    if (!nf) return;
+    std::sort(prim.begin(), prim.end());
+    std::sort(gap.begin(), gap.end());
+    std::fprintf(stderr, "[lpe]     %s failed %d: primal %d dual %d gap %d; median primal %.1e gap %.1e, "
+                 "10%% gap %.1e, 90%% gap %.1e\n", what, nf, fp, fd, fg, prim[nf / 2], gap[nf / 2], gap[nf / 10],
+                 gap[(9 * nf) / 10]);
+}
+
+// Round 2 (iterative refinement from round 1's FP64 result) only where round
+// 1 found the optimal face (finite, gap <= 1e-4) but missed a gate.
+std::vector<int> refine_set(const std::vector<int>& hsel, const std::vector<int>& passed,
+                            const std::vector<double>& met) {
+    std::vector<int> sel2(hsel.size(), 0);
+    for (size_t b = 0; b < hsel.size(); ++b) {
+        const bool close = met[b * 8 + 5] == 1.0 && met[b * 8 + 2] <= 1e-4;
+        sel2[b] = (hsel[b] && !passed[b] && close) ? 1 : 0;
+    }
+    return sel2;
+}
+
+// Both rounds on one view (used for compacted selections).
+template <typename T>
+int polish_rounds(lpe_handle* h, const View& v, std::vector<int>& hsel, const T* xin, const T* yin, int cg_iters,
+                  double tol, double ep, double ed, double eg, std::vector<int>& passed) {
+    const int ld = v.ld;
+    std::vector<double> met;
+    CHK(polish_t<T>(h, v, hsel.data(), xin, yin, 0, cg_iters, tol));
+    CHK(certify_view(h, v, hsel, ep, ed, eg, passed, met));
+    trace_failures("round 1", hsel, passed, met, ep, ed, eg);
+    std::vector<int> sel2 = refine_set(hsel, passed, met);
+    int again = 0;
+    for (int b = 0; b < ld; ++b) again += sel2[b];
+    if (again > 0 && h->m > 0) {
+        int* dsel = const_cast<int*>(v.sel);
+        HIPCHK(hipMemcpyAsync(dsel, sel2.data(), sizeof(int) * ld, hipMemcpyHostToDevice, h->s));
+        CHK(polish_t<double>(h, v, sel2.data(), h->d_xpol, h->d_ypol, 1, cg_iters, tol));
+        std::vector<int> p2;
+        CHK(certify_view(h, v, sel2, ep, ed, eg, p2, met));
+        trace_failures("round 2", sel2, p2, met, ep, ed, eg);
+        HIPCHK(hipMemcpyAsync(dsel, hsel.data(), sizeof(int) * ld, hipMemcpyHostToDevice, h->s));
+        for (int b = 0; b < ld; ++b)
+            if (sel2[b]) passed[b] = p2[b];
+    }
+    return LPE_OK;
+}
+
+// Gathers the problem data of the LPs idx[0..P) into a dense prefix of the
+// compaction buffers; the width ldp is a power of two >= 64, so the captured
+// CG graphs stay few.
+int compact_view(lpe_handle* h, const std::vector<int>& idx, int& ldp, std::vector<int>& hsel, View& v) {
+    const int P = (int)idx.size();
+    ldp = 64;
     while (ldp < P) ldp *= 2;
-    std::vector<int> perm(ldp, -1), hsel(ldp, 0);
+    std::vector<int> perm(ldp, -1);
+    hsel.assign(ldp, 0);
     for (int j = 0; j < P; ++j) { perm[j] = idx[j]; hsel[j] = 1; }
     HIPCHK(hipMemcpyAsync(h->d_perm, perm.data(), sizeof(int) * ldp, hipMemcpyHostToDevice, h->s));
     HIPCHK(hipMemcpyAsync(h->d_csel, hsel.data(), sizeof(int) * ldp, hipMemcpyHostToDevice, h->s));
@@ -810,11 +827,101 @@ int polish_selected(lpe_handle* h, int cg_iters, double tol, double ep, double e
         gather_cols<double>(h, h->d_var_o, h->d_cvar_o, h->nvar, ldp);
         gather_cols<double>(h, h->d_var_s, h->d_cvar_s, h->nvar, ldp);
     }
-    View v;
     v.ld = ldp; v.c = h->shared_c ? h->d_c_o : h->d_cc; v.c_s = h->shared_c ? h->d_c_s : h->d_ccs;
     v.rhs = h->d_crhs; v.lo = h->d_clo; v.up = h->d_cup;
     v.var_o = h->nvar > 0 ? h->d_cvar_o : h->d_var_o; v.var_s = h->nvar > 0 ? h->d_cvar_s : h->d_var_s;
     v.sel = h->d_csel;
+    HIPCHK(hipGetLastError());
+    return LPE_OK;
+}
+
+// Copies the view's polished results of the LPs with mask[j] into d_xres /
+// d_yres (full layout).  Full view: mask is uploaded to d_sel; compacted view:
+// to d_csel, scattered through d_perm.
+int keep_results(lpe_handle* h, const View& v, const std::vector<int>& mask) {
+    const bool full = v.sel == h->d_sel;
+    int* dmask = full ? h->d_sel : h->d_csel;
+    HIPCHK(hipMemcpyAsync(dmask, mask.data(), sizeof(int) * v.ld, hipMemcpyHostToDevice, h->s));
+    if (full) {
+        hipLaunchKernelGGL(k_masked_copy, ew_grid(h->n, h->ld), dim3(256), 0, h->s, (const double*)h->d_xpol,
+                           h->d_xres, h->n, h->ld, ROWS_PER_BLOCK_EW, (const int*)dmask, 1);
+        if (h->m > 0)
+            hipLaunchKernelGGL(k_masked_copy, ew_grid(h->m, h->ld), dim3(256), 0, h->s, (const double*)h->d_ypol,
+                               h->d_yres, h->m, h->ld, ROWS_PER_BLOCK_EW, (const int*)dmask, 1);
+    } else {
+        hipLaunchKernelGGL((k_scatter_cols<double>), ew_grid(h->n, v.ld), dim3(256), 0, h->s,
+                           (const double*)h->d_xpol, h->d_xres, h->n, v.ld, h->ld, (const int*)h->d_perm,
+                           (const int*)dmask, ROWS_PER_BLOCK_EW);
+        if (h->m > 0)
+            hipLaunchKernelGGL((k_scatter_cols<double>), ew_grid(h->m, v.ld), dim3(256), 0, h->s,
+                               (const double*)h->d_ypol, h->d_yres, h->m, v.ld, h->ld, (const int*)h->d_perm,
+                               (const int*)dmask, ROWS_PER_BLOCK_EW);
+    }
+    HIPCHK(hipGetLastError());
+    return LPE_OK;
+}
+
+// Polish the LPs selected in h->h_sel.  Selections of at most half the batch
+// are gathered into a dense prefix of the compaction buffers first, so the
+// cost follows the number of LPs polished; so is the refinement round of a
+// full-batch polish when at most half of the batch needs it.  On return
+// h->h_sel marks the LPs that passed the certificate, their results are in
+// d_xres / d_yres, and d_sel mirrors h->h_sel.
+int polish_selected(lpe_handle* h, int cg_iters, double tol, double ep, double ed, double eg) {
+    const bool f32 = h->method != LPE_METHOD_FIXED && h->precision == LPE_PREC_FP32;
+    std::vector<int> idx;
+    for (int b = 0; b < h->B; ++b)
+        if (h->h_sel[b]) idx.push_back(b);
+    const int P = (int)idx.size();
+    if (P == 0) return LPE_OK;
+    const int m = h->m, n = h->n;
+    std::vector<int> passed;
+    if (2 * P > h->ld) {
+        View v = full_view(h);
+        std::vector<int> hsel(h->h_sel, h->h_sel + h->ld);
+        std::vector<double> met;
+        HIPCHK(hipMemcpyAsync(h->d_sel, h->h_sel, sizeof(int) * h->ld, hipMemcpyHostToDevice, h->s));
+        if (f32) CHK(polish_t<float>(h, v, hsel.data(), h->it32.x, h->it32.y, 0, cg_iters, tol));
+        else CHK(polish_t<double>(h, v, hsel.data(), h->it64.x, h->it64.y, 0, cg_iters, tol));
+        CHK(certify_view(h, v, hsel, ep, ed, eg, passed, met));
+        trace_failures("round 1", hsel, passed, met, ep, ed, eg);
+        CHK(keep_results(h, v, passed));
+        std::vector<int> sel2 = refine_set(hsel, passed, met);
+        std::vector<int> idx2;
+        for (int b = 0; b < h->ld; ++b)
+            if (sel2[b]) idx2.push_back(b);
+        if (!idx2.empty() && m > 0) {
+            std::vector<int> p2;
+            if (2 * idx2.size() > (size_t)h->ld) {
+                HIPCHK(hipMemcpyAsync(h->d_sel, sel2.data(), sizeof(int) * h->ld, hipMemcpyHostToDevice, h->s));
+                CHK(polish_t<double>(h, v, sel2.data(), h->d_xpol, h->d_ypol, 1, cg_iters, tol));
+                CHK(certify_view(h, v, sel2, ep, ed, eg, p2, met));
+                trace_failures("round 2", sel2, p2, met, ep, ed, eg);
+                CHK(keep_results(h, v, p2));
+                for (int b = 0; b < h->ld; ++b) passed[b] = passed[b] || p2[b];
+            } else {
+                int ldp = 0;
+                std::vector<int> hsel2;
+                View vc;
+                CHK(compact_view(h, idx2, ldp, hsel2, vc));
+                gather_cols<double>(h, h->d_xpol, h->d_cx, n, ldp);
+                gather_cols<double>(h, h->d_ypol, h->d_cy, m, ldp);
+                CHK(polish_t<double>(h, vc, hsel2.data(), h->d_cx, h->d_cy, 1, cg_iters, tol));
+                CHK(certify_view(h, vc, hsel2, ep, ed, eg, p2, met));
+                trace_failures("round 2 (compacted)", hsel2, p2, met, ep, ed, eg);
+                CHK(keep_results(h, vc, p2));
+                for (size_t j = 0; j < idx2.size(); ++j) passed[idx2[j]] = passed[idx2[j]] || p2[j];
+            }
+        }
+        for (int b = 0; b < h->ld; ++b) h->h_sel[b] = b < h->B ? passed[b] : 0;
+        HIPCHK(hipMemcpyAsync(h->d_sel, h->h_sel, sizeof(int) * h->ld, hipMemcpyHostToDevice, h->s));
+        HIPCHK(hipGetLastError());
+        return LPE_OK;
+    }
+    int ldp = 0;
+    std::vector<int> hsel;
+    View v;
+    CHK(compact_view(h, idx, ldp, hsel, v));
     if (f32) {
         float* cx = reinterpret_cast<float*>(h->d_cx);
         float* cy = reinterpret_cast<float*>(h->d_cy);
@@ -826,12 +933,7 @@ int polish_selected(lpe_handle* h, int cg_iters, double tol, double ep, double e
         gather_cols<double>(h, h->it64.y, h->d_cy, m, ldp);
         CHK(polish_rounds<double>(h, v, hsel, h->d_cx, h->d_cy, cg_iters, tol, ep, ed, eg, passed));
     }
-    HIPCHK(hipMemcpyAsync(h->d_csel, passed.data(), sizeof(int) * ldp, hipMemcpyHostToDevice, h->s));
-    hipLaunchKernelGGL((k_scatter_cols<double>), ew_grid(n, ldp), dim3(256), 0, h->s, (const double*)h->d_xpol,
-                       h->d_xres, n, ldp, h->ld, (const int*)h->d_perm, (const int*)h->d_csel, ROWS_PER_BLOCK_EW);
-    if (m > 0)
-        hipLaunchKernelGGL((k_scatter_cols<double>), ew_grid(m, ldp), dim3(256), 0, h->s, (const double*)h->d_ypol,
-                           h->d_yres, m, ldp, h->ld, (const int*)h->d_perm, (const int*)h->d_csel, ROWS_PER_BLOCK_EW);
+    CHK(keep_results(h, v, passed));
     for (int b = 0; b < h->ld; ++b) h->h_sel[b] = 0;
     for (int j = 0; j < P; ++j) h->h_sel[idx[j]] = passed[j];
     HIPCHK(hipMemcpyAsync(h->d_sel, h->h_sel, sizeof(int) * h->ld, hipMemcpyHostToDevice, h->s));
@@ -900,9 +1002,10 @@ int alloc_all(lpe_handle* h) {
     CHK(dalloc(h, &h->d_partial, h->partial_elems));
     CHK(dalloc(h, &h->d_red_a, (size_t)2 * CC_N * ld)); CHK(dalloc(h, &h->d_red_b, (size_t)2 * CC_N * ld));
     // (red_a / red_b also hold the polish's 3 CG dots per LP)
-    const int ops_cr[2 * CR_N] = {RSUM, RMAX, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM, RMAX, RSUM, RSUM, RMAX, RSUM, RSUM};
-    const int ops_cc[2 * CC_N] = {RSUM, RMAX, RSUM, RSUM, RSUM, RMAX, RSUM, RSUM,
-                                  RSUM, RMAX, RSUM, RSUM, RSUM, RMAX, RSUM, RSUM};
+    const int ops_cr[2 * CR_N] = {RSUM, RMAX, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM,
+                                  RSUM, RMAX, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM};
+    const int ops_cc[2 * CC_N] = {RSUM, RMAX, RSUM, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM,
+                                  RSUM, RMAX, RSUM, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM};
     const int ops_xr[XR_N] = {RMAX, RMAX, RSUM, RMAX, RSUM};
     const int ops_xc[XC_N] = {RMAX, RMAX, RMAX, RMAX, RSUM, RSUM, RSUM, RSUM};
     const int ops_sum[3] = {RSUM, RSUM, RSUM};
@@ -965,6 +1068,7 @@ int init_state(lpe_handle* h) {
         s.avg_div = 1.0;
         s.accepted_iter = -1;
         s.status = b < h->B ? 0 : 1;
+        s.asig = -1.0;
     }
     HIPCHK(hipMemcpyAsync(h->d_st, h->h_st, sizeof(LpState) * h->ld, hipMemcpyHostToDevice, h->s));
     std::vector<int> flags(h->ld), zeros(h->ld, 0);
@@ -1346,9 +1450,12 @@ int lpe_solve(lpe_handle* h, const lpe_solve_opts* o, lpe_solve_info* info, void
     // below polish_kkt; after a failed attempt, retry when the LP has run 40%
     // more iterations (at least one more chunk) or its KKT error halved
     const double polish_kkt = o->polish_kkt > 0 ? o->polish_kkt : 1e-3;
-    std::vector<long long> last_iter(h->B, -1);
+    const double polish_kkt_start = o->polish_kkt_start > 0 ? o->polish_kkt_start : 1e-3 * polish_kkt;
+    std::vector<long long> last_iter(h->B, -1), first_ready(h->B, -1);
     std::vector<double> last_rel(h->B, INFINITY);
+    std::vector<int> failed(h->B, 0);
     long last_polish = 0;
+    bool stop = false;
     lpe_solve_info inf;
     std::memset(&inf, 0, sizeof(inf));
     float ms = 0.0f;
@@ -1375,7 +1482,12 @@ int lpe_solve(lpe_handle* h, const lpe_solve_opts* o, lpe_solve_info* info, void
             running++;
             const LpState& st = h->h_st[b];
             // at iteration 0 (a warm start) polish only an essentially optimal point
-            bool ready = do_polish && st.rel_kkt <= (k == 0 ? 1e-3 * polish_kkt : polish_kkt);
+            bool ready = do_polish && st.rel_kkt <= (k == 0 ? polish_kkt_start : polish_kkt);
+            if (ready && k > 0 && first_ready[b] < 0) first_ready[b] = st.iters;
+            // optionally the active set must have held over the last checks (or
+            // the LP has been below the KKT threshold for 4 chunks)
+            if (ready && k > 0 && o->polish_stable_checks > 0)
+                ready = st.asig_same >= o->polish_stable_checks || st.iters - first_ready[b] >= 4L * K;
             if (ready && last_iter[b] >= 0)
                 ready = st.iters >= std::max<long long>(last_iter[b] + K, (long long)(1.4 * last_iter[b])) ||
                         st.rel_kkt <= 0.5 * last_rel[b];
@@ -1418,7 +1530,13 @@ int lpe_solve(lpe_handle* h, const lpe_solve_opts* o, lpe_solve_info* info, void
             } else {
                 last_iter[b] = h->h_st[b].iters;
                 last_rel[b] = h->h_st[b].rel_kkt;
+                failed[b]++;
             }
+        }
+        if (o->stop_after_failed_polishes > 0 && running > 0 && 16L * running <= h->B) {
+            stop = true;
+            for (int b = 0; b < h->B && stop; ++b)
+                if (h->h_st[b].status == 0 && !h->h_sel[b] && failed[b] < o->stop_after_failed_polishes) stop = false;
         }
         if (any) {
             // accepted: polish results are in d_xres / d_yres; freeze the LPs
@@ -1430,7 +1548,7 @@ int lpe_solve(lpe_handle* h, const lpe_solve_opts* o, lpe_solve_info* info, void
     };
     // iteration 0: a warm start may already be optimal (src/gpu_pdhg_corrector.py:318-328)
     if (running > 0) CHK(after_chunk(0));
-    while (running > 0 && iters < o->max_iterations) {
+    while (running > 0 && iters < o->max_iterations && !stop) {
         const int k = (int)std::min<long>(K, o->max_iterations - iters);
         HIPCHK(hipEventRecord(e0, h->s));
         CHK(run_iterations(h, k, k, o->use_graph));

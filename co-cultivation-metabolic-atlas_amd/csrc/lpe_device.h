@@ -340,31 +340,50 @@ __device__ __forceinline__ void block_partials(double (&val)[NV][LPL], const int
 }
 
 // out[v * ld + b] = reduce over rb of partial[(rb * NV + v) * ld + b].  A
-// 256-thread block covers 64 LPs of one value v; its 4 waves reduce the row
-// blocks rb = w, w+4, w+8, ... and wave 0 combines the 4 results in wave
-// order, so the summation order is fixed (deterministic across runs).
-__global__ void __launch_bounds__(256) k_reduce_partials(const double* __restrict__ partial, double* __restrict__ out,
-                                                         const int* __restrict__ ops, int nv, int n_rb, int ld) {
-    __shared__ double part[4][64];
+// 1024-thread block covers 64 LPs of one value v.  Wave w walks the row blocks
+// rb = w, w + 16, ... with four independent accumulators (four loads in flight
+// per lane), combines them as ((a0, a1), (a2, a3)), and wave 0 combines the 16
+// wave results in wave order: the summation order is fixed (deterministic).
+constexpr int RED_WAVES = 16;
+
+__device__ __forceinline__ double red_combine(int op, double acc, double o) {
+    return op == RSUM ? acc + o : (o > acc || __builtin_isnan(o) ? o : acc);
+}
+
+__global__ void __launch_bounds__(RED_WAVES * 64) k_reduce_partials(const double* __restrict__ partial,
+                                                                     double* __restrict__ out,
+                                                                     const int* __restrict__ ops, int nv,
+                                                                     int n_rb, int ld) {
+    __shared__ double part[RED_WAVES][64];
     const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
     const int b = blockIdx.x * 64 + lane;
     const int v = blockIdx.y;
     const int op = ops[v];
-    double acc = op == RSUM ? 0.0 : -INFINITY;
+    const double init = op == RSUM ? 0.0 : -INFINITY;
+    double acc = init;
     if (b < ld) {
-        for (int r = w; r < n_rb; r += 4) {
-            const double o = partial[((long)r * nv + v) * ld + b];
-            acc = op == RSUM ? acc + o : (o > acc || __builtin_isnan(o) ? o : acc);
+        const long stride = (long)nv * ld;
+        const double* p = partial + (long)v * ld + b;
+        double a0 = init, a1 = init, a2 = init, a3 = init;
+        int r = w;
+        for (; r + 3 * RED_WAVES < n_rb; r += 4 * RED_WAVES) {
+            const double o0 = p[r * stride];
+            const double o1 = p[(r + RED_WAVES) * stride];
+            const double o2 = p[(r + 2 * RED_WAVES) * stride];
+            const double o3 = p[(r + 3 * RED_WAVES) * stride];
+            a0 = red_combine(op, a0, o0);
+            a1 = red_combine(op, a1, o1);
+            a2 = red_combine(op, a2, o2);
+            a3 = red_combine(op, a3, o3);
         }
+        for (; r < n_rb; r += RED_WAVES) a0 = red_combine(op, a0, p[r * stride]);
+        acc = red_combine(op, red_combine(op, a0, a1), red_combine(op, a2, a3));
     }
     part[w][lane] = acc;
     __syncthreads();
     if (w == 0 && b < ld) {
         double t = part[0][lane];
-        for (int q = 1; q < 4; ++q) {
-            const double o = part[q][lane];
-            t = op == RSUM ? t + o : (o > t || __builtin_isnan(o) ? o : t);
-        }
+        for (int q = 1; q < RED_WAVES; ++q) t = red_combine(op, t, part[q][lane]);
         out[(long)v * ld + b] = t;
     }
 }
@@ -375,6 +394,13 @@ __global__ void __launch_bounds__(256) k_reduce_partials(const double* __restric
 // Current iterate (x, y) and average (xsum/cnt, ysum/cnt) are evaluated in
 // one pass; each pass writes NV block partials per LP.
 // ---------------------------------------------------------------------------
+// Integer weight in [1, 2^20] of index i for the active-set signatures: sums of
+// at most 2 * 2^20 * rows are exact in FP64, so the signature is independent of
+// the summation order.
+__device__ __forceinline__ double asig_weight(unsigned i) {
+    return (double)(((i * 2654435761u) >> 12) & 0xFFFFFu) + 1.0;
+}
+
 enum CheckRowVal {
     CR_RP2 = 0,    // sum of squared scaled primal residual
     CR_PERR,       // max original-unit row error (eq |Ax-b|, ineq (Ax-b)+)
@@ -383,6 +409,7 @@ enum CheckRowVal {
     CR_YPOS,       // max original y over inequality rows
     CR_DY2,        // sum (y - y_last)^2
     CR_NONFIN,     // count of non-finite activity / y
+    CR_ASIG,       // active-set signature: hashed sum over inequality rows with y < 0
     CR_N
 };
 enum CheckColVal {
@@ -394,6 +421,7 @@ enum CheckColVal {
     CC_BVIOL,      // max original bound violation
     CC_DX2,        // sum (x - x_last)^2
     CC_NONFIN,
+    CC_ASIG,       // active-set signature: hashed sum over columns at a bound (1 lower, 2 upper)
     CC_N
 };
 
@@ -464,11 +492,12 @@ __global__ void __launch_bounds__(BLOCK) k_check_rows(CheckRowsArgs<T> a) {
                 const double dy = yv - (double)a.ylast[row * ld + b0];
                 w[CR_DY2][0] += dy * dy;
                 if (!__builtin_isfinite(ax[0]) || !__builtin_isfinite(yv)) w[CR_NONFIN][0] += 1.0;
+                if (which == 0 && ineq && yv < 0.0) w[CR_ASIG][0] += asig_weight((unsigned)row + 7919u);
             }
         }
     }
-    const int ops[2 * CR_N] = {RSUM, RMAX, RSUM, RSUM, RMAX, RSUM, RSUM,
-                               RSUM, RMAX, RSUM, RSUM, RMAX, RSUM, RSUM};
+    const int ops[2 * CR_N] = {RSUM, RMAX, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM,
+                               RSUM, RMAX, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM};
     block_partials<2 * CR_N, 1>(v, ops, a.partial, rb, ld, b0, lds, valid);
 }
 
@@ -546,11 +575,17 @@ __global__ void __launch_bounds__(BLOCK) k_check_cols(CheckColsArgs<T> a) {
                 const double dx = xv - (double)a.xlast[col * ld + b0];
                 w[CC_DX2][0] += dx * dx;
                 if (!__builtin_isfinite(rc) || !__builtin_isfinite(xv)) w[CC_NONFIN][0] += 1.0;
+                if (which == 0) {
+                    const double width = (lfin && ufin && hi - lo > 1.0) ? hi - lo : 1.0;
+                    const bool at_lo = lfin && xv - lo <= 1e-6 * width;
+                    const bool at_hi = !at_lo && ufin && hi - xv <= 1e-6 * width;
+                    if (at_lo || at_hi) w[CC_ASIG][0] += (at_lo ? 1.0 : 2.0) * asig_weight((unsigned)col);
+                }
             }
         }
     }
-    const int ops[2 * CC_N] = {RSUM, RMAX, RSUM, RSUM, RSUM, RMAX, RSUM, RSUM,
-                               RSUM, RMAX, RSUM, RSUM, RSUM, RMAX, RSUM, RSUM};
+    const int ops[2 * CC_N] = {RSUM, RMAX, RSUM, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM,
+                               RSUM, RMAX, RSUM, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM};
     block_partials<2 * CC_N, 1>(v, ops, a.partial, rb, ld, b0, lds, valid);
 }
 
@@ -579,6 +614,9 @@ struct LpState {
     int action;          // Action for k_check_apply
     int restarts;
     int source;          // 0: result from iterate, 1: from polish
+    double asig;         // active-set signature of the current iterate at the last check
+    int asig_same;       // consecutive checks with an unchanged signature
+    int pad_;
 };
 
 enum Action { ACT_NONE = 0, ACT_RESTART_CUR = 1, ACT_RESTART_AVG = 2, ACT_ACCEPT_CUR = 3, ACT_ACCEPT_AVG = 4 };
@@ -639,6 +677,9 @@ __global__ void k_check_decide(DecideArgs a) {
     }
     const int best = (nw == 2 && kkt[1] < kkt[0]) ? 1 : 0;
     s.rel_kkt = rel[best];
+    const double sig = a.cols[CC_ASIG * ld + b] + a.rows[CR_ASIG * ld + b];
+    s.asig_same = (a.K > 0 && sig == s.asig) ? s.asig_same + 1 : 0;
+    s.asig = sig;
     s.avg_div = (double)(a.cnt[b] + a.K);
     if (a.certify && (pass[0] || pass[1])) {
         s.status = 1;

@@ -1,13 +1,15 @@
 // Warm-start network kernels for gfx950 (see include/warmstart.h).
 //
 // GEMM: 128x128 output tile per 256-thread block, four waves in a 2x2
-// arrangement, each wave 64x64 = 2x2 tiles of the 32x32x8 BF16 MFMA with FP32
-// accumulation.  K advances by 32: the block stages op(A) as [m][k] and op(B)
-// as [n][k] in LDS (the transposes of the backward products are folded into
-// the staging), so each MFMA operand is one 8-byte LDS read per lane.
+// arrangement, each wave 64x64 = 2x2 tiles of the 32x32x16 BF16 MFMA with
+// FP32 accumulation.  K advances by 32: the block stages op(A) as [m][k] and
+// op(B) as [n][k] in LDS (the transposes of the backward products are folded
+// into the staging), so each MFMA operand is one 16-byte LDS read per lane.
+// With 16-byte aligned operands the next K block is loaded into registers
+// while the MFMAs of the current one run.
 //
-// MFMA 32x32x8 bf16 lane layout (64 lanes): A operand lane l holds
-// A[l % 32][4*(l / 32) + 0..3]; B operand lane l holds B[4*(l / 32) + 0..3][l % 32];
+// MFMA 32x32x16 bf16 lane layout (64 lanes): A operand lane l holds
+// A[l % 32][8*(l / 32) + 0..7]; B operand lane l holds B[8*(l / 32) + 0..7][l % 32];
 // accumulator element r of lane l is D[8*(r / 4) + 4*(l / 32) + (r % 4)][l % 32].
 #include "warmstart.h"
 
@@ -17,7 +19,7 @@
 
 namespace {
 
-typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float float16_t __attribute__((ext_vector_type(16)));
 
 constexpr int BM = 128, BN = 128, BK = 32, PAD = 8;
@@ -29,14 +31,84 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
     return (uint16_t)(u >> 16);
 }
 
-__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+// Operand tiles are 128 (tile rows) x 32 (k).  KMAJOR: the operand is X[row][k]
+// (k contiguous); otherwise X[k][row].  Entries past K are zeroed (they meet
+// the other operand's k); entries past R only reach output rows/columns that
+// are never stored.
+
+// 16-byte path (ldx % 8 == 0, X 16-byte aligned): thread tid owns chunks
+// tid and tid + 256 of the tile, 8 elements each.
+template <bool KMAJOR>
+__device__ __forceinline__ void load_vec(uint4 (&v)[2], const uint16_t* __restrict__ X, int ldx, int r0, int R,
+                                         int k0, int K, int tid) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int c = tid + q * 256;
+        v[q] = make_uint4(0, 0, 0, 0);
+        if (KMAJOR) {
+            const int gr = r0 + c / 4, gk = k0 + (c % 4) * 8;
+            if (gr < R && gk < K) {
+                v[q] = *reinterpret_cast<const uint4*>(X + (long)gr * ldx + gk);
+                if (gk + 8 > K) {
+                    uint16_t* e = reinterpret_cast<uint16_t*>(&v[q]);
+#pragma unroll
+                    for (int t = 0; t < 8; ++t)
+                        if (gk + t >= K) e[t] = 0;
+                }
+            }
+        } else {
+            const int gk = k0 + c / 16, gr = r0 + (c % 16) * 8;
+            if (gk < K && gr < R) v[q] = *reinterpret_cast<const uint4*>(X + (long)gk * ldx + gr);
+        }
+    }
+}
+
+template <bool KMAJOR>
+__device__ __forceinline__ void store_vec(uint16_t (*S)[BK + PAD], const uint4 (&v)[2], int tid) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int c = tid + q * 256;
+        if (KMAJOR) {
+            *reinterpret_cast<uint4*>(&S[c / 4][(c % 4) * 8]) = v[q];
+        } else {
+            const int kr = c / 16, rc = (c % 16) * 8;
+            const uint16_t* e = reinterpret_cast<const uint16_t*>(&v[q]);
+#pragma unroll
+            for (int t = 0; t < 8; ++t) S[rc + t][kr] = e[t];
+        }
+    }
+}
+
+// Element path for unaligned operands.
+template <bool KMAJOR>
+__device__ __forceinline__ void stage_scalar(uint16_t (*S)[BK + PAD], const uint16_t* __restrict__ X, int ldx,
+                                             int r0, int R, int k0, int K, int tid) {
+    if (KMAJOR) {
+        const int row = tid / 2, kk = (tid % 2) * 16;
+        const int gr = r0 + row;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const int gk = k0 + kk + e;
+            S[row][kk + e] = (gr < R && gk < K) ? X[(long)gr * ldx + gk] : (uint16_t)0;
+        }
+    } else {
+        const int kr = tid / 8, rr = (tid % 8) * 16;
+        const int gk = k0 + kr;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const int gr = r0 + rr + e;
+            S[rr + e][kr] = (gr < R && gk < K) ? X[(long)gk * ldx + gr] : (uint16_t)0;
+        }
+    }
+}
 
 template <int TA, int TB>
 __global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, const uint16_t* __restrict__ A, int lda,
                                               const uint16_t* __restrict__ B, int ldb, float* __restrict__ C,
-                                              int ldc, int accumulate) {
+                                              int ldc, int accumulate, int va, int vb) {
     __shared__ __align__(16) uint16_t As[BM][BK + PAD];
     __shared__ __align__(16) uint16_t Bs[BN][BK + PAD];
+    constexpr bool AK = !TA, BKM = TB != 0;   // k-contiguous operands
     const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
     const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
     const int wm = (wave / 2) * 64, wn = (wave % 2) * 64;
@@ -48,58 +120,33 @@ __global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, const uint16_
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
+    uint4 ra[2], rb[2];
+    if (va) load_vec<AK>(ra, A, lda, m0, M, 0, K, tid);
+    if (vb) load_vec<BKM>(rb, B, ldb, n0, N, 0, K, tid);
     for (int k0 = 0; k0 < K; k0 += BK) {
-        // stage op(A) -> As[m][k]
-        if (!TA) {
-            const int row = tid / 2, kk = (tid % 2) * 16;
-            const int gm = m0 + row;
-#pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const int gk = k0 + kk + e;
-                As[row][kk + e] = (gm < M && gk < K) ? A[(long)gm * lda + gk] : (uint16_t)0;
-            }
-        } else {
-            const int kr = tid / 8, mm = (tid % 8) * 16;
-            const int gk = k0 + kr;
-#pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const int gm = m0 + mm + e;
-                As[mm + e][kr] = (gm < M && gk < K) ? A[(long)gk * lda + gm] : (uint16_t)0;
-            }
-        }
-        // stage op(B) -> Bs[n][k]
-        if (!TB) {
-            const int kr = tid / 8, nn = (tid % 8) * 16;
-            const int gk = k0 + kr;
-#pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const int gn = n0 + nn + e;
-                Bs[nn + e][kr] = (gn < N && gk < K) ? B[(long)gk * ldb + gn] : (uint16_t)0;
-            }
-        } else {
-            const int col = tid / 2, kk = (tid % 2) * 16;
-            const int gn = n0 + col;
-#pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const int gk = k0 + kk + e;
-                Bs[col][kk + e] = (gn < N && gk < K) ? B[(long)gn * ldb + gk] : (uint16_t)0;
-            }
-        }
+        if (va) store_vec<AK>(As, ra, tid);
+        else stage_scalar<AK>(As, A, lda, m0, M, k0, K, tid);
+        if (vb) store_vec<BKM>(Bs, rb, tid);
+        else stage_scalar<BKM>(Bs, B, ldb, n0, N, k0, K, tid);
         __syncthreads();
+        if (k0 + BK < K) {                      // next block in flight during the MFMAs
+            if (va) load_vec<AK>(ra, A, lda, m0, M, k0 + BK, K, tid);
+            if (vb) load_vec<BKM>(rb, B, ldb, n0, N, k0 + BK, K, tid);
+        }
 #pragma unroll
-        for (int ks = 0; ks < BK / 8; ++ks) {
-            short4_t a[2], b[2];
+        for (int ks = 0; ks < BK / 16; ++ks) {
+            bf16x8_t a[2], b[2];
 #pragma unroll
             for (int i = 0; i < 2; ++i)
-                a[i] = *reinterpret_cast<const short4_t*>(&As[wm + i * 32 + lane % 32][ks * 8 + (lane / 32) * 4]);
+                a[i] = *reinterpret_cast<const bf16x8_t*>(&As[wm + i * 32 + lane % 32][ks * 16 + (lane / 32) * 8]);
 #pragma unroll
             for (int j = 0; j < 2; ++j)
-                b[j] = *reinterpret_cast<const short4_t*>(&Bs[wn + j * 32 + lane % 32][ks * 8 + (lane / 32) * 4]);
+                b[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[wn + j * 32 + lane % 32][ks * 16 + (lane / 32) * 8]);
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         }
         __syncthreads();
     }
@@ -215,13 +262,30 @@ __global__ void k_to_bf16(const float* x, uint16_t* y, long n) {
     if (i < n) y[i] = f2bf(x[i]);
 }
 
+// Y[C][R] = X[R][C]^T through a 64 x 65 LDS tile (coalesced reads and writes).
+__global__ void __launch_bounds__(256) k_transpose(const uint16_t* __restrict__ X, uint16_t* __restrict__ Y, int R,
+                                                   int C) {
+    __shared__ uint16_t tile[64][65];
+    const int c0 = blockIdx.x * 64, r0 = blockIdx.y * 64;
+    const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
+    for (int i = ty; i < 64; i += 4) {
+        const int r = r0 + i, c = c0 + tx;
+        if (r < R && c < C) tile[i][tx] = X[(long)r * C + c];
+    }
+    __syncthreads();
+    for (int i = ty; i < 64; i += 4) {
+        const int c = c0 + i, r = r0 + tx;
+        if (c < C && r < R) Y[(long)c * R + r] = tile[tx][i];
+    }
+}
+
 inline int blocks(long n, int t) { return (int)((n + t - 1) / t); }
 
 }  // namespace
 
 extern "C" {
 
-const char* ws_version(void) { return "warmstart 0.1 gfx950 mfma_f32_32x32x8bf16_1k"; }
+const char* ws_version(void) { return "warmstart 0.2 gfx950 mfma_f32_32x32x16_bf16"; }
 
 int ws_gemm_bf16(int M, int N, int K, const uint16_t* A, int lda, int trans_a, const uint16_t* B, int ldb,
                  int trans_b, float* C, int ldc, int accumulate, void* stream) {
@@ -229,10 +293,16 @@ int ws_gemm_bf16(int M, int N, int K, const uint16_t* A, int lda, int trans_a, c
     if (M == 0 || N == 0) return 0;
     dim3 grid(blocks(N, BN), blocks(M, BM));
     hipStream_t s = (hipStream_t)stream;
-    if (!trans_a && !trans_b) hipLaunchKernelGGL((k_gemm<0, 0>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate);
-    else if (!trans_a && trans_b) hipLaunchKernelGGL((k_gemm<0, 1>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate);
-    else if (trans_a && !trans_b) hipLaunchKernelGGL((k_gemm<1, 0>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate);
-    else hipLaunchKernelGGL((k_gemm<1, 1>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate);
+    const int va = (lda % 8 == 0 && ((uintptr_t)A & 15) == 0) ? 1 : 0;
+    const int vb = (ldb % 8 == 0 && ((uintptr_t)B & 15) == 0) ? 1 : 0;
+    if (!trans_a && !trans_b)
+        hipLaunchKernelGGL((k_gemm<0, 0>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate, va, vb);
+    else if (!trans_a && trans_b)
+        hipLaunchKernelGGL((k_gemm<0, 1>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate, va, vb);
+    else if (trans_a && !trans_b)
+        hipLaunchKernelGGL((k_gemm<1, 0>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate, va, vb);
+    else
+        hipLaunchKernelGGL((k_gemm<1, 1>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate, va, vb);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
@@ -270,6 +340,13 @@ int ws_adam(float* w, const float* g, float* m, float* v, uint16_t* wb, int64_t 
     const float c1 = 1.0f - powf(b1, (float)t), c2 = 1.0f - powf(b2, (float)t);
     hipLaunchKernelGGL(k_adam, dim3(blocks(n, 256)), dim3(256), 0, (hipStream_t)stream, w, g, m, v, wb, (long)n, lr,
                        b1, b2, eps, c1, c2);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int ws_transpose_bf16(const uint16_t* X, uint16_t* Y, int R, int C, void* stream) {
+    if (!X || !Y || R < 0 || C < 0) return -1;
+    if (R == 0 || C == 0) return 0;
+    hipLaunchKernelGGL(k_transpose, dim3(blocks(C, 64), blocks(R, 64)), dim3(256), 0, (hipStream_t)stream, X, Y, R, C);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

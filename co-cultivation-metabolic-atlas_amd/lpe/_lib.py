@@ -52,7 +52,8 @@ class LpeSolveOpts(C.Structure):
                 ("polish_cg_iterations", C.c_int32), ("use_graph", C.c_int32),
                 ("rows_per_wave", C.c_int32), ("lps_per_lane", C.c_int32),
                 ("eps_certify_primal", C.c_double), ("eps_certify_dual", C.c_double),
-                ("eps_certify_gap", C.c_double)]
+                ("eps_certify_gap", C.c_double), ("stop_after_failed_polishes", C.c_int32),
+                ("polish_kkt_start", C.c_double), ("polish_stable_checks", C.c_int32)]
 
 
 class LpeSolveInfo(C.Structure):

@@ -75,8 +75,12 @@ class GpuLpBackend:
             if all(p is not None for p in prev):
                 x0 = np.stack([p[0] for p in prev]); y0 = np.stack([p[1] for p in prev])
         eng.reset(x0, y0)
-        info = eng.solve(self.options)
-        x, y, status = eng.state()
+        if self.precision == "fp32":
+            # FP32 stragglers continue on an FP64 engine from their FP32 iterate
+            info, x, y, status = eng.solve_with_fallback(self.options)
+        else:
+            info = eng.solve(self.options)
+            x, y, status = eng.state()
         metrics = eng.certify(x, y).cpu().numpy()
         x, y = x.cpu().numpy(), y.cpu().numpy()
         out = []

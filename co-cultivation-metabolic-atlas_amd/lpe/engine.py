@@ -9,6 +9,7 @@ layout the reference's services accept (src/gpu_pdhg_workspace.py:169-186).
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 from dataclasses import dataclass
 from typing import Optional
 
@@ -53,6 +54,9 @@ class SolveOptions:
     eps_primal: float = 1e-5
     eps_dual: float = 1e-7
     eps_gap: float = 1e-7
+    stop_after_failed_polishes: int = 0
+    polish_kkt_start: float = 0.0
+    polish_stable_checks: int = 0
 
     def to_c(self) -> _lib.LpeSolveOpts:
         o = _lib.LpeSolveOpts()
@@ -67,6 +71,9 @@ class SolveOptions:
         o.eps_certify_primal = float(self.eps_primal)
         o.eps_certify_dual = float(self.eps_dual)
         o.eps_certify_gap = float(self.eps_gap)
+        o.stop_after_failed_polishes = int(self.stop_after_failed_polishes)
+        o.polish_kkt_start = float(self.polish_kkt_start)
+        o.polish_stable_checks = int(self.polish_stable_checks)
         return o
 
 
@@ -191,6 +198,11 @@ class BatchedLpEngine:
         ``capacity`` LPs.  Returns (info, x [B,n], y [B,m], status [B]) with
         the fallback's results merged in.
         """
+        capacity = max(1, min(int(capacity), self.batch))
+        options = options or SolveOptions()
+        if options.polish and not options.stop_after_failed_polishes:
+            # LPs whose FP32 polish failed twice go to FP64 instead of iterating on
+            options = dataclasses.replace(options, stop_after_failed_polishes=2)
         info = self.solve(options)
         x, y, status = self.state()
         bad = np.flatnonzero(status != _lib.STATUS_OPTIMAL)
@@ -199,6 +211,9 @@ class BatchedLpEngine:
         if bad.size == 0 or self._vectors is None:
             return info, x, y, status
         fb_opts = fallback or SolveOptions(max_iterations=20000, polish=True)
+        if getattr(self, "_fallback", None) is not None and self._fallback.batch != capacity:
+            self._fallback.close()
+            self._fallback = None
         if getattr(self, "_fallback", None) is None:
             self._fallback = BatchedLpEngine(self.A, self.neq, capacity, var_positions=self.var_positions,
                                              precision="fp64", shared_c=self.shared_c, device=self.device)

@@ -14,7 +14,8 @@ Each step mirrors the reference's dFBA step for the cooperative consortium
 
 All per-environment state is a torch tensor on the GPU; the LPs of one step
 are one batched solve.  Warm starts: "previous" (the last step's primal and
-dual), "none" (cold) or a callable returning (x0, y0) (the learned model).
+dual), "none" (cold) or a callable (conc, lower, biomass, x_prev, y_prev) ->
+(x0, y0) such as the learned model of ``lpe.warmstart``.
 """
 
 from __future__ import annotations
@@ -111,11 +112,11 @@ class ConsortiumRollout:
             self.options.check_interval = int(self.tuning["chunk"])
         x0 = y0 = None
         if callable(self.warm_start):
-            x0, y0 = self.warm_start(self.conc, lower, self.biomass)
+            x0, y0 = self.warm_start(self.conc, lower, self.biomass, self.x, self.y)
         elif self.warm_start == "previous" and self.x is not None:
             x0, y0 = self.x, self.y
         eng.reset(x0, y0)
-        if x0 is not None and self.omega is not None and not callable(self.warm_start):
+        if x0 is not None and self.omega is not None:
             eng.set_primal_weight(self.omega)       # warm start of the primal weight
         if self.fallback and eng.precision == "fp32":
             info, x, y, status = eng.solve_with_fallback(self.options)

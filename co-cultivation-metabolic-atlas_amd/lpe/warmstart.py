@@ -27,8 +27,8 @@ import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libwarmstart.so")
-EXPORTS = ("ws_gemm_bf16", "ws_bias_act", "ws_act_backward", "ws_loss", "ws_adam", "ws_to_bf16",
-           "ws_version")
+EXPORTS = ("ws_gemm_bf16", "ws_bias_act", "ws_act_backward", "ws_loss", "ws_adam", "ws_transpose_bf16",
+           "ws_to_bf16", "ws_version")
 
 _lib = None
 
@@ -47,6 +47,7 @@ def load():
     lib.ws_act_backward.argtypes = [P, P, P, P, I, I, I, P]
     lib.ws_loss.argtypes = [P, P, P, P, I, I, I, I, F, F, F, P]
     lib.ws_adam.argtypes = [P, P, P, P, P, C.c_int64, F, F, F, F, I, P]
+    lib.ws_transpose_bf16.argtypes = [P, P, I, I, P]
     lib.ws_to_bf16.argtypes = [P, P, C.c_int64, P]
     lib.ws_version.restype = C.c_char_p
     for name in EXPORTS[:-1]:
@@ -89,6 +90,17 @@ def gemm(a: torch.Tensor, b: torch.Tensor, *, trans_a: bool = False, trans_b: bo
     return out
 
 
+def transpose(x: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """BF16 [R, C] -> [C, R] (ws_transpose_bf16)."""
+    lib = load()
+    if x.dtype != torch.bfloat16 or x.dim() != 2 or not x.is_contiguous():
+        raise ValueError("transpose takes a contiguous 2-D bfloat16 tensor")
+    r, c = x.shape
+    out = torch.empty((c, r), dtype=torch.bfloat16, device=x.device) if out is None else out
+    _ok(lib.ws_transpose_bf16(_ptr(x), _ptr(out), r, c, _stream()), "ws_transpose_bf16")
+    return out
+
+
 def to_bf16(x: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tensor:
     lib = load()
     x = x.contiguous()
@@ -112,8 +124,12 @@ class WarmStartNet:
         load()
         self.device = device or torch.device("cuda", torch.cuda.current_device())
         self.n_in, self.nx, self.ny, self.na = int(n_in), int(nx), int(ny), int(na)
-        self.n_out = self.nx + self.ny + self.na
-        self.dims = [self.n_in] + [int(hidden)] * int(layers) + [self.n_out]
+        # widths padded to multiples of 8 so every GEMM operand row is 16-byte aligned:
+        # zero input columns, and extra bound logits trained towards label 0
+        self.n_in_p = -(-self.n_in // 8) * 8
+        self.na_p = self.na + (-(self.nx + self.ny + self.na)) % 8
+        self.n_out = self.nx + self.ny + self.na_p
+        self.dims = [self.n_in_p] + [int(hidden)] * int(layers) + [self.n_out]
         self.loss_weights = tuple(float(w) for w in loss_weights)
         g = torch.Generator(device="cpu").manual_seed(seed)
         self.w, self.b = [], []
@@ -122,7 +138,9 @@ class WarmStartNet:
             self.w.append(((torch.rand((k, n), generator=g, dtype=torch.float32) * 2 - 1) * bound)
                           .to(self.device))
             self.b.append(torch.zeros(n, dtype=torch.float32, device=self.device))
-        self.wb = [to_bf16(w) for w in self.w]
+        # BF16 copies in both layouts: every GEMM then reads two k-contiguous operands
+        self.wb = [to_bf16(w) for w in self.w]              # [in][out]
+        self.wbt = [transpose(w) for w in self.wb]          # [out][in]
         self.params = self.w + self.b
         self.m = [torch.zeros_like(p) for p in self.params]
         self.v = [torch.zeros_like(p) for p in self.params]
@@ -138,10 +156,16 @@ class WarmStartNet:
         return 2 * sum(k * n for k, n in zip(self.dims[:-1], self.dims[1:]))
 
     def set_params(self, weights: Sequence, biases: Sequence) -> None:
+        """Load weights [in][out] and biases given at the unpadded widths."""
         for l, (w, b) in enumerate(zip(weights, biases)):
-            self.w[l].copy_(torch.as_tensor(np.asarray(w), dtype=torch.float32))
-            self.b[l].copy_(torch.as_tensor(np.asarray(b), dtype=torch.float32))
+            w = torch.as_tensor(np.asarray(w), dtype=torch.float32)
+            b = torch.as_tensor(np.asarray(b), dtype=torch.float32)
+            self.w[l].zero_()
+            self.b[l].zero_()
+            self.w[l][:w.shape[0], :w.shape[1]].copy_(w)
+            self.b[l][:b.shape[0]].copy_(b)
             to_bf16(self.w[l], self.wb[l])
+            transpose(self.wb[l], self.wbt[l])
 
     # ------------------------------------------------------------ statistics
     def fit_normalization(self, features: torch.Tensor, targets: torch.Tensor) -> None:
@@ -153,14 +177,16 @@ class WarmStartNet:
         self.out_std = t.std(0).clamp_min(1e-6).to(torch.float32)
 
     def _inputs(self, features):
-        x = (features.to(torch.float32) - self.in_mean) / self.in_std
-        return to_bf16(x.contiguous())
+        x = torch.zeros((features.shape[0], self.n_in_p), dtype=torch.float32, device=self.device)
+        x[:, :self.n_in] = (features.to(torch.float32) - self.in_mean) / self.in_std
+        return to_bf16(x)
 
     def _targets(self, targets):
-        t = targets.to(torch.float32).clone()
+        t = torch.zeros((targets.shape[0], self.n_out), dtype=torch.float32, device=self.device)
         k = self.nx + self.ny
-        t[:, :k] = (t[:, :k] - self.out_mean) / self.out_std
-        return t.contiguous()
+        t[:, :k] = (targets[:, :k].to(torch.float32) - self.out_mean) / self.out_std
+        t[:, k:k + self.na] = targets[:, k:k + self.na].to(torch.float32)
+        return t
 
     # ------------------------------------------------------------- forward
     def _forward(self, xb: torch.Tensor, keep: bool):
@@ -172,7 +198,7 @@ class WarmStartNet:
         last = len(self.w) - 1
         for l in range(len(self.w)):
             n = self.dims[l + 1]
-            z = gemm(h, self.wb[l])
+            z = gemm(h, self.wbt[l], trans_b=True)
             hn = None if l == last else torch.empty((mb, n), dtype=torch.bfloat16, device=self.device)
             _ok(lib.ws_bias_act(_ptr(z), _ptr(self.b[l]), _ptr(hn), mb, n, 0 if l == last else 1, s),
                 "ws_bias_act")
@@ -192,7 +218,7 @@ class WarmStartNet:
         o = self.predict_raw(features)
         k = self.nx + self.ny
         reg = o[:, :k] * self.out_std + self.out_mean
-        return reg[:, :self.nx], reg[:, self.nx:k], o[:, k:]
+        return reg[:, :self.nx], reg[:, self.nx:k], o[:, k:k + self.na]
 
     # ------------------------------------------------------------ training
     def loss_and_grad(self, xb: torch.Tensor, tb: torch.Tensor):
@@ -205,7 +231,7 @@ class WarmStartNet:
         d_out = torch.empty_like(o)
         row_loss = torch.empty(mb, dtype=torch.float32, device=self.device)
         wx, wy, wa = self.loss_weights
-        _ok(lib.ws_loss(_ptr(o), _ptr(tb), _ptr(d_out), _ptr(row_loss), mb, self.nx, self.ny, self.na,
+        _ok(lib.ws_loss(_ptr(o), _ptr(tb), _ptr(d_out), _ptr(row_loss), mb, self.nx, self.ny, self.na_p,
                         wx, wy, wa, s), "ws_loss")
         gw = [None] * len(self.w)
         gb = [torch.zeros_like(b) for b in self.b]
@@ -215,7 +241,8 @@ class WarmStartNet:
             dz = torch.empty((mb, n), dtype=torch.bfloat16, device=self.device)
             _ok(lib.ws_act_backward(_ptr(dh), _ptr(zs[l]), _ptr(dz), _ptr(gb[l]), mb, n,
                                     0 if l == len(self.w) - 1 else 1, s), "ws_act_backward")
-            gw[l] = gemm(hs[l], dz, trans_a=True)
+            # dW = H^T dZ from the two transposed activations (k = batch contiguous)
+            gw[l] = gemm(transpose(hs[l]), transpose(dz), trans_b=True)
             if l > 0:
                 dh = gemm(dz, self.wb[l], trans_b=True)
         return row_loss.sum(), gw + gb
@@ -229,6 +256,8 @@ class WarmStartNet:
             wb = self.wb[i] if i < nw else torch.empty(p.shape, dtype=torch.bfloat16, device=self.device)
             _ok(lib.ws_adam(_ptr(p), _ptr(g), _ptr(m), _ptr(v), _ptr(wb), p.numel(), lr, beta1, beta2, eps,
                             self.step_count, s), "ws_adam")
+            if i < nw:
+                transpose(wb, self.wbt[i])
 
     def fit(self, features: torch.Tensor, targets: torch.Tensor, *, epochs: int = 20, batch: int = 1024,
             lr: float = 1e-3, lr_final: float = 1e-4, seed: int = 0, log=None) -> list:
@@ -276,18 +305,28 @@ def bound_labels(x: torch.Tensor, lower: torch.Tensor, upper: torch.Tensor, tol:
 
 
 class LearnedWarmStart:
-    """Warm-start callable for ``ConsortiumRollout``: returns the network's
-    (x0, y0), with variables the network places at a bound snapped there.
-    Before training it returns (None, None), i.e. a cold start."""
+    """Warm-start callable for ``ConsortiumRollout``.
+
+    mode "predict": x0, y0 = the network's prediction for this step's LPs,
+    variables the network places at a bound snapped there.
+    mode "correct": the previous step's solution moved by the network's
+    predicted change, x0 = x_prev + f(p_t) - f(p_{t-1}) (the systematic error
+    of f cancels); the first step falls back to "predict".
+    Before training it passes the previous solution through (None on the
+    first step: a cold start)."""
 
     def __init__(self, model, *, hidden: int = 1024, layers: int = 3, device=None, seed: int = 0,
-                 snap: bool = True):
+                 snap: bool = True, mode: str = "predict"):
+        if mode not in ("predict", "correct"):
+            raise ValueError("mode must be 'predict' or 'correct'")
         self.model = model
         m, n = model.shape
         self.n_features = 2 * model.medium_cols.shape[1] + model.medium_cols.shape[0] + model.medium_cols.size
         self.net = WarmStartNet(self.n_features, n, m, n, hidden=hidden, layers=layers, device=device, seed=seed)
         self.snap = snap
+        self.mode = mode
         self.features, self.targets = [], []
+        self._last = None
 
     def add_solutions(self, conc, lower, upper, biomass, x, y, certified) -> int:
         """Keep the certified LPs of one solved step as training samples."""
@@ -301,21 +340,34 @@ class LearnedWarmStart:
         self.targets.append(t)
         return int(ok.sum())
 
+    @property
+    def n_samples(self) -> int:
+        return sum(int(f.shape[0]) for f in self.features)
+
     def train(self, **kwargs) -> list:
         f = torch.cat(self.features)
         t = torch.cat(self.targets)
         return self.net.fit(f, t, **kwargs)
 
-    def __call__(self, conc, lower, biomass):
-        if not self.net.trained:
-            return None, None
+    def predict(self, conc, lower, biomass):
         md = self.model
         upper = torch.as_tensor(md.upper, device=lower.device).expand_as(lower)
         x, y, logits = self.net.predict(consortium_features(md, conc, lower, biomass))
-        x = x.to(torch.float64)
-        x = torch.minimum(torch.maximum(x, lower), upper)
+        x = torch.minimum(torch.maximum(x.to(torch.float64), lower), upper)
         if self.snap:
             near_lo = (x - lower) <= (upper - x)
             at = logits > 0
             x = torch.where(at & near_lo, lower, torch.where(at, upper, x))
         return x.contiguous(), y.to(torch.float64).contiguous()
+
+    def __call__(self, conc, lower, biomass, x_prev=None, y_prev=None):
+        if not self.net.trained:
+            return x_prev, y_prev
+        x, y = self.predict(conc, lower, biomass)
+        last, self._last = self._last, (x, y)
+        if self.mode == "correct" and last is not None and x_prev is not None:
+            md = self.model
+            upper = torch.as_tensor(md.upper, device=lower.device).expand_as(lower)
+            xc = torch.minimum(torch.maximum(x_prev + (x - last[0]), lower), upper)
+            return xc.contiguous(), (y_prev + (y - last[1])).contiguous()
+        return x, y

@@ -96,6 +96,16 @@ typedef struct lpe_solve_opts {
     double eps_certify_primal;/* reference gates (0 = 1e-5, 1e-7, 1e-7)      */
     double eps_certify_dual;
     double eps_certify_gap;
+    int32_t stop_after_failed_polishes; /* > 0: end the solve once at most
+                                 * 1/16 of the batch is still running and each
+                                 * of those LPs has failed this many polish
+                                 * attempts (their status is then
+                                 * LPE_STATUS_LIMIT; 0 = run to the budget)   */
+    double polish_kkt_start;  /* relative KKT for a polish of the starting
+                                 point itself (0 = 1e-3 * polish_kkt)         */
+    int32_t polish_stable_checks; /* > 0: first polish only after the active
+                                 * set held for this many checks (or 4 chunks
+                                 * below polish_kkt)                          */
 } lpe_solve_opts;
 
 /* Summary of the last lpe_solve. */

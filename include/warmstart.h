@@ -48,6 +48,9 @@ int ws_loss(const float *O, const float *T, float *dO, float *row_loss, int M, i
 int ws_adam(float *w, const float *g, float *m, float *v, uint16_t *w_bf16, int64_t n, float lr, float beta1,
             float beta2, float eps, int t, void *stream);
 
+/* Y[C][R] = transpose of X[R][C] (BF16). */
+int ws_transpose_bf16(const uint16_t *X, uint16_t *Y, int R, int C, void *stream);
+
 /* Round FP32 to BF16 (round to nearest even). */
 int ws_to_bf16(const float *x, uint16_t *y, int64_t n, void *stream);
 
