@@ -7,8 +7,9 @@ shared-medium coupling coefficients per environment), then updates medium
 and biomass by explicit Euler on the device.  A bench "step" is one dFBA step.
 
 Target form: restarted-averaged PDHG with FP32 bulk arithmetic, FP64 checks,
-FP64 active-set polish and FP64 certificate, warm-started from the previous
-step's primal/dual.  Plain form: FP64 restarted-averaged PDHG, cold start, no
+FP64 active-set polish and FP64 certificate, warm-started by the learned MLP
+(trained before the timed region on certified solves of rollouts with other
+seeds) or, with --warm-start previous, from the previous step's primal/dual.  Plain form: FP64 restarted-averaged PDHG, cold start, no
 polish, on the same LPs.  An LP counts as certified when it passes the
 reference's certificate (primal <= 1e-5, dual <= 1e-7, relative KKT gap <=
 1e-7); a HiGHS sample checks <= 1e-6 relative objective error.
@@ -156,6 +157,65 @@ def measured_traffic(path, envs, lps_per_lane):
     return None
 
 
+def train_warm_start(model, B, dev, args, options):
+    """Learned warm start: certified solves of separate rollouts (other seeds)
+    train the MLP before the timed region.  Returns (callable, report)."""
+    import torch
+    from lpe.rollout import ConsortiumRollout
+    from lpe.warmstart import LearnedWarmStart
+
+    lws = LearnedWarmStart(model, device=dev, mode="predict")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for seed in [int(s) for s in args.train_seeds.split(",") if s]:
+        ro = ConsortiumRollout(model, B, seed=seed, precision="fp32", options=options(), device=dev)
+        for _ in range(args.train_steps):
+            conc, bio = ro.conc.clone(), ro.biomass.clone()
+            _, lower, upper, _ = ro.lp_inputs()
+            out = ro.step()
+            lws.add_solutions(conc, lower, upper, bio, out["x"], out["y"], out["certified_mask"])
+        del ro
+        torch.cuda.empty_cache()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    hist = lws.train(epochs=args.train_epochs, batch=1024, lr=1e-3)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    net = lws.net
+    # MFMA throughput of one training step (forward + backward, batch 1024) and of inference at B
+    xb = torch.randn(1024, net.n_in_p, device=dev).to(torch.bfloat16)
+    tb = torch.zeros(1024, net.n_out, device=dev)
+    net.loss_and_grad(xb, tb)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        net.loss_and_grad(xb, tb)
+    e1.record()
+    torch.cuda.synchronize()
+    train_ms = e0.elapsed_time(e1) / 10
+    xi = torch.randn(B, net.n_in_p, device=dev).to(torch.bfloat16)
+    net._forward(xi, keep=False)
+    e0.record()
+    for _ in range(10):
+        net._forward(xi, keep=False)
+    e1.record()
+    torch.cuda.synchronize()
+    infer_ms = e0.elapsed_time(e1) / 10
+    fps = net.flops_per_sample()
+    report = {"kind": "mlp", "layers": net.dims, "params": net.n_params, "train_samples": lws.n_samples,
+              "train_rollouts": args.train_seeds, "train_steps_per_rollout": args.train_steps,
+              "solve_seconds": round(t1 - t0, 2), "train_seconds": round(t2 - t1, 2), "epochs": args.train_epochs,
+              "loss_first_last": [round(hist[0], 5), round(hist[-1], 5)],
+              "mfma_roofline": {"bound": "mfma", "peak": 2500.0, "unit": "TFLOP/s",
+                                "train_step_ms": round(train_ms, 4),
+                                "train_achieved": round(3 * fps * 1024 / (train_ms * 1e-3) / 1e12, 1),
+                                "inference_ms": round(infer_ms, 4),
+                                "inference_achieved": round(fps * B / (infer_ms * 1e-3) / 1e12, 1)}}
+    print(f"# warm-start MLP: {lws.n_samples} samples, solves {t1 - t0:.1f} s, training {t2 - t1:.1f} s, "
+          f"loss {hist[0]:.4f} -> {hist[-1]:.4f}", file=sys.stderr, flush=True)
+    return lws, report
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,6 +236,11 @@ def main():
     ap.add_argument("--rows-per-wave", type=int, default=2, help="launch shape of --probe-kernels")
     ap.add_argument("--lps-per-lane", type=int, default=4, help="launch shape of --probe-kernels")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "pmc_step_kernels_consortium.md"))
+    ap.add_argument("--warm-start", choices=("mlp", "previous"), default="mlp",
+                    help="mlp: the learned warm start (trained before the timed region); previous: last step's solution")
+    ap.add_argument("--train-seeds", default="11,12", help="rollout seeds whose certified solves train the MLP")
+    ap.add_argument("--train-steps", type=int, default=6, help="dFBA steps per training rollout")
+    ap.add_argument("--train-epochs", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the LPs are independent and would shard "
@@ -199,10 +264,13 @@ def main():
     nvar = model.coupling_nnz.size
     B = args.envs
 
-    target = ConsortiumRollout(model, B, seed=args.seed, precision="fp32",
-                               options=SolveOptions(max_iterations=args.fp32_iterations, polish=True,
-                                                    polish_cg_iterations=args.cg_iterations),
-                               warm_start="previous", device=dev)
+    target_options = lambda: SolveOptions(max_iterations=args.fp32_iterations, polish=True,  # noqa: E731
+                                          polish_cg_iterations=args.cg_iterations)
+    warm, mlp_report = "previous", None
+    if args.warm_start == "mlp":
+        warm, mlp_report = train_warm_start(model, B, dev, args, target_options)
+    target = ConsortiumRollout(model, B, seed=args.seed, precision="fp32", options=target_options(),
+                               warm_start=warm, device=dev)
     for _ in range(args.warmup):
         target.step()
         print(f"# warmup step done: {target.log[-1]['certified']}/{B} certified, "
@@ -271,13 +339,16 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64",
         "data": "synthetic (seeded generator of the 3-species consortium LP; no reference traces)",
         "config": {"workload": f"3-species consortium dFBA step, {B} environments, LP {m}x{n} "
-                               f"({nnz} nnz, {nvar} per-environment coefficients), warm start from the "
-                               "previous step, FP32 PDHG + FP64 polish/certificate",
+                               f"({nnz} nnz, {nvar} per-environment coefficients), "
+                               + ("learned MLP warm start" if args.warm_start == "mlp" else
+                                  "warm start from the previous step")
+                               + ", FP32 PDHG + FP64 polish/certificate",
                    "envs": B, "rows": m, "cols": n, "nnz": nnz, "rollout_steps_timed": args.steps},
         "plain_fp64": {"value": round(plain_value, 3), "unit": "certified LPs/s", "steps": len(p_logs),
                        "certified": p_cert, "seconds": round(p_time, 3), "log": p_logs},
         "speedup_vs_plain": round(value / plain_value, 3) if plain_value > 0 else None,
         "target_log": step_logs,
+        "warm_start": mlp_report or {"kind": "previous step's primal/dual"},
         "autotune": {k: v for k, v in (target_tuning or {}).items() if k != "timings"},
         "autotune_timings_ms_per_iteration": (target_tuning or {}).get("timings"),
         "roofline": {"bound": "hbm", "kernel": f"k_primal_step<float,{lpl},false>",

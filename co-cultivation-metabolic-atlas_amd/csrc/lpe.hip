@@ -20,6 +20,15 @@
 #include <string>
 #include <vector>
 
+// Reduced-cost rule of the refinement round, whose dual input is the first
+// round's FP64 least-squares dual (see k_polish_cols_init).
+#ifndef LPE_R2_TOL_FAR
+#define LPE_R2_TOL_FAR 1e-2
+#endif
+#ifndef LPE_R2_TOL_R
+#define LPE_R2_TOL_R 1e-5
+#endif
+
 using namespace lpe;
 
 namespace {
@@ -639,7 +648,9 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
         a.sh = sc; a.AT = orig ? op_orig_v(h, true, v) : op_scaled_v(h, true, v); a.x = xin; a.y = yin;
         a.c = orig ? v.c : v.c_s; a.c_shared = h->shared_c; a.orig = orig;
         a.dc = h->d_dc; a.lower = v.lo; a.upper = v.up; a.sel = v.sel; a.xs = h->d_xs;
-        a.fmask = h->d_fmask; a.tol = tol; a.tol_far = 1e-2; a.tol_r = 1e-5;
+        a.fmask = h->d_fmask; a.tol = tol;
+        a.tol_far = orig ? LPE_R2_TOL_FAR : 1e-2;
+        a.tol_r = orig ? LPE_R2_TOL_R : 1e-5;
         hipLaunchKernelGGL((k_polish_cols_init<T>), dim3(grid_c), dim3(BLOCK), 0, h->s, a);
     }
     if (h->m > 0) {
@@ -682,9 +693,25 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
             }
             CHK(cg_block(h, v, sr, sc, grid_r, grid_c));
             used += CG_BLOCK;
+            if (std::getenv("LPE_TRACE_CG")) {
+                // residual history of the first selected LP (both systems)
+                HIPCHK(hipStreamSynchronize(h->s));
+                int b0 = 0;
+                while (b0 < ld && !hsel[b0]) ++b0;
+                if (b0 < ld) {
+                    double rh[2], r0[2];
+                    HIPCHK(hipMemcpy(&rh[0], h->d_rho + b0, sizeof(double), hipMemcpyDeviceToHost));
+                    HIPCHK(hipMemcpy(&rh[1], h->d_rho + ld + b0, sizeof(double), hipMemcpyDeviceToHost));
+                    HIPCHK(hipMemcpy(&r0[0], h->d_rho0 + b0, sizeof(double), hipMemcpyDeviceToHost));
+                    HIPCHK(hipMemcpy(&r0[1], h->d_rho0 + ld + b0, sizeof(double), hipMemcpyDeviceToHost));
+                    std::fprintf(stderr, "[lpe]       cg lp %d step %d: primal %.2e (x%.1e) dual %.2e (x%.1e)\n", b0,
+                                 used, rh[0], rh[0] / r0[0], rh[1], rh[1] / r0[1]);
+                }
+            }
         }
         h->last_cg_steps = used;
         if (std::getenv("LPE_TRACE")) {
+            HIPCHK(hipStreamSynchronize(h->s));
             std::vector<double> rh(2 * ld), r0(2 * ld), fl(2 * ld), pl(ld);
             HIPCHK(hipMemcpy(act.data(), h->d_cgact, sizeof(int) * 2 * ld, hipMemcpyDeviceToHost));
             HIPCHK(hipMemcpy(rh.data(), h->d_rho, sizeof(double) * 2 * ld, hipMemcpyDeviceToHost));
@@ -746,7 +773,7 @@ void trace_failures(const char* what, const std::vector<int>& hsel, const std::v
     if (!std::getenv("LPE_TRACE")) return;
     // failed LPs by gate, with the median primal residual and gap among them
     int nf = 0, fp = 0, fd = 0, fg = 0;
-    std::vector<double> prim, gap;
+    std::vector<double> prim, gap, bgap, rgap;
     for (size_t b = 0; b < hsel.size(); ++b) {
         if (!hsel[b] || passed[b]) continue;
         const double* r = &met[b * 8];
@@ -756,13 +783,17 @@ void trace_failures(const char* what, const std::vector<int>& hsel, const std::v
         fg += !(r[2] <= eg);
         prim.push_back(r[0]);
         gap.push_back(r[2]);
+        bgap.push_back(r[6] / std::max(1.0, std::fabs(r[3])));
+        rgap.push_back(r[7] / std::max(1.0, std::fabs(r[3])));
     }
     if (!nf) return;
     std::sort(prim.begin(), prim.end());
     std::sort(gap.begin(), gap.end());
-    std::fprintf(stderr, "[lpe]     %s failed %d: primal %d dual %d gap %d; median primal %.1e gap %.1e, "
-                 "10%% gap %.1e, 90%% gap %.1e\n", what, nf, fp, fd, fg, prim[nf / 2], gap[nf / 2], gap[nf / 10],
-                 gap[(9 * nf) / 10]);
+    std::sort(bgap.begin(), bgap.end());
+    std::sort(rgap.begin(), rgap.end());
+    std::fprintf(stderr, "[lpe]     %s failed %d: primal %d dual %d gap %d; median primal %.1e gap %.1e "
+                 "(bound %.1e row %.1e), 10%% gap %.1e, 90%% gap %.1e\n", what, nf, fp, fd, fg, prim[nf / 2],
+                 gap[nf / 2], bgap[nf / 2], rgap[nf / 2], gap[nf / 10], gap[(9 * nf) / 10]);
 }
 
 // Round 2 (iterative refinement from round 1's FP64 result) only where round
@@ -1697,6 +1728,7 @@ int lpe_get_scaling(lpe_handle* h, double* row_scale, double* col_scale, double*
     if (h->method == LPE_METHOD_FIXED) {
         // per-LP metric: return LP 0's values
         if (sigma && h->m > 0) {
+            HIPCHK(hipStreamSynchronize(h->s));
             for (int i = 0; i < h->m; ++i)
                 HIPCHK(hipMemcpy(sigma + i, h->d_sigref + (long)i * h->ld, sizeof(double), hipMemcpyDeviceToHost));
         }
@@ -1729,7 +1761,8 @@ int lpe_set_primal_weight(lpe_handle* h, const double* omega, void* stream) {
 
 int lpe_get_primal_weight(lpe_handle* h, double* omega) {
     if (!h || !omega) return fail(LPE_EINVAL, "handle and omega are required");
-    HIPCHK(hipMemcpy(h->h_st, h->d_st, sizeof(LpState) * h->ld, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(h->h_st, h->d_st, sizeof(LpState) * h->ld, hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
     for (int b = 0; b < h->B; ++b) omega[b] = h->h_st[b].omega;
     return LPE_OK;
 }
