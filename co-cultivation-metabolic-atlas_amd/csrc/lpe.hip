@@ -671,7 +671,9 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
         pr.p1 = nullptr; pr.p2 = nullptr; pr.setup_r = h->d_rd; pr.setup_p = h->d_pd; pr.invdiag = h->d_invdiag;
         pr.rmask = h->d_rmask; pr.sel = v.sel; pr.act = nullptr; pr.partial = h->d_partial;
         const int nrbs = launch_polish_rows(h, pr, v.ld);
-        reduce_ld(h, h->d_partial, h->d_red_a, h->d_ops_sum, 1, nrbs, v.ld);   // rho_d
+        // the rows kernel writes two partials per row block (the second is 0 in
+        // setup mode): reduce both so the stride matches, rho_d is the first
+        reduce_ld(h, h->d_partial, h->d_red_a, h->d_ops_sum, 2, nrbs, v.ld);
         CgScalarArgs cs;
         cs.rho = h->d_rho; cs.rho0 = h->d_rho0; cs.alpha = h->d_alpha; cs.beta = h->d_beta;
         cs.act = h->d_cgact; cs.floor2 = h->d_cgfloor; cs.sel = v.sel; cs.rel = CG_REL; cs.batch = v.ld; cs.ld = v.ld;

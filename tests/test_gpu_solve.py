@@ -150,3 +150,40 @@ def test_nonfinite_and_infeasible_fail_closed(device):
     assert info["certified"] == 0 and status[0] != 1
     m = eng.certify(np.full((1, 1), np.nan), np.full((1, 2), -1.0)).cpu().numpy()
     assert m[0, 4] == 0.0
+
+
+def test_polish_matches_oracle_active_set_least_squares(device):
+    """One polish call from the same FP64 PDHG iterate: the engine's active-set
+    polish (Jacobi-PCG on both systems) and oracle/polish.py (plain CG on the
+    same systems) converge to the same least-squares answers, so the LPs the
+    oracle's polish certifies are certified by the engine with the same
+    objective (1e-9 relative).  Catches a wrong operator, mask, preconditioner
+    or CG scalar in either system."""
+    from lpe import _lib
+    from oracle import polish as opolish
+    model = workloads.consortium(seed=4, shapes=((120, 180), (150, 220), (90, 130)), n_medium=20)
+    cb = workloads.consortium_batch(batch=4, seed=4, model=model)
+    md = cb.model
+    eng = _engine(md.A, md.neq, cb.batch, precision="fp64", shared_c=True, var_positions=md.coupling_nnz.ravel())
+    eng.update(md.c, cb.rhs(), cb.lower(), cb.upper(), var_values=cb.coupling_values().reshape(cb.batch, -1))
+    eng.reset()
+    eng.solve(max_iterations=3000, check_interval=64, polish=False)
+    x, y, _ = eng.state()
+    x, y = x.cpu().numpy(), y.cpu().numpy()
+    sc = eng.scaling()
+    oracle_ok = []
+    for b in range(cb.batch):
+        p = cb.problem(b)
+        xo, yo = opolish.polish(*p, x[b], y[b], sc["row_scale"], sc["col_scale"], cg_iterations=400)[:2]
+        c = cert.certificate(*p, xo, yo)
+        oracle_ok.append((c["certificate_passed"], c["objective"]))
+    assert sum(ok for ok, _ in oracle_ok) >= 2, oracle_ok
+    eng.polish(np.ones(cb.batch, np.uint8), cg_iterations=200)
+    xg, yg, status = eng.state()
+    xg = xg.cpu().numpy()
+    for b, (ok, obj) in enumerate(oracle_ok):
+        if not ok:
+            continue
+        assert status[b] == _lib.STATUS_OPTIMAL, (b, status)
+        got = float(md.c @ xg[b])
+        assert abs(got - obj) <= 1e-9 * max(1.0, abs(obj)), (b, got, obj)
