@@ -576,7 +576,10 @@ int build_results(lpe_handle* h) {
 }
 
 constexpr int CG_BLOCK = 25;
-constexpr double CG_REL = 1e-20;   // rho = r.D^-1 r reduction at which a system stops
+#ifndef LPE_CG_REL
+#define LPE_CG_REL 1e-20
+#endif
+constexpr double CG_REL = LPE_CG_REL;   // rho = r.D^-1 r reduction at which a system stops
 
 void cg_steps(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, int grid_r, int grid_c, int steps) {
     CgScalarArgs cs;
