@@ -103,17 +103,29 @@ def cpu_baseline(model, conc, biomass, gpu_x, gpu_ok, seconds_budget):
                 solver=highs.version(), highs_single_lp_seconds=one), errs, n_sample
 
 
+GEM_FIXTURE = os.path.join(ROOT, "tests", "golden", "gem_consortium.npz")
+
+
+def build_model(args):
+    """The joint LP structure: the synthetic 3-species consortium, or the
+    reference's three member GEMs (parsed SBML, committed as a fixture)."""
+    if args.workload == "gem":
+        from lpe import sbml
+        return sbml.consortium_from_gems(sbml.load_gems(GEM_FIXTURE), seed=args.seed)
+    from lpe.workloads import consortium
+    return consortium(seed=args.seed)
+
+
 def probe_kernels(args):
     """Launch only the two step kernels and a copy of known size, for hardware
     counter runs (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE): the copy's counter
     readings scale the step kernels' readings to bytes."""
     import torch
     from lpe.rollout import ConsortiumRollout
-    from lpe.workloads import consortium
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
-    model = consortium(seed=args.seed)
+    model = build_model(args)
     ro = ConsortiumRollout(model, args.envs, seed=args.seed, precision="fp32", autotune=False, device=dev)
     rhs, lower, upper, coupling = ro.lp_inputs()
     eng = ro.engine
@@ -135,9 +147,9 @@ def probe_kernels(args):
                       "primal_ms": ms_p, "dual_ms": ms_d, "copy_bytes_read": 4 * n_copy,
                       "copy_bytes_written": 4 * n_copy,
                       "primal_algorithmic_bytes": _primal_bytes(args.envs, m, n, model.A.nnz,
-                                                                model.coupling_nnz.size, 4),
+                                                                model.var_positions.size, 4),
                       "dual_algorithmic_bytes": _dual_bytes(args.envs, m, n, model.A.nnz,
-                                                            model.coupling_nnz.size, 4)}), flush=True)
+                                                            model.var_positions.size, 4)}), flush=True)
 
 
 def measured_traffic(path, envs, lps_per_lane):
@@ -241,6 +253,8 @@ def main():
     ap.add_argument("--train-seeds", default="11,12", help="rollout seeds whose certified solves train the MLP")
     ap.add_argument("--train-steps", type=int, default=6, help="dFBA steps per training rollout")
     ap.add_argument("--train-epochs", type=int, default=20)
+    ap.add_argument("--workload", choices=("synthetic", "gem"), default="synthetic",
+                    help="synthetic: seeded 3-species consortium; gem: the reference's member GEMs (SBML)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the LPs are independent and would shard "
@@ -254,14 +268,13 @@ def main():
     import torch
     from lpe.engine import SolveOptions
     from lpe.rollout import ConsortiumRollout
-    from lpe.workloads import consortium
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
-    model = consortium(seed=args.seed)
+    model = build_model(args)
     m, n = model.shape
     nnz = model.A.nnz
-    nvar = model.coupling_nnz.size
+    nvar = model.var_positions.size
     B = args.envs
 
     target_options = lambda: SolveOptions(max_iterations=args.fp32_iterations, polish=True,  # noqa: E731
@@ -338,7 +351,9 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64",
         "data": "synthetic (seeded generator of the 3-species consortium LP; no reference traces)",
-        "config": {"workload": f"3-species consortium dFBA step, {B} environments, LP {m}x{n} "
+        "config": {"workload": ("3-species consortium dFBA step" if args.workload == "synthetic" else
+                                "3-member real-GEM consortium dFBA step (OR16, NS21, P. freudenreichii SBML)")
+                               + f", {B} environments, LP {m}x{n} "
                                f"({nnz} nnz, {nvar} per-environment coefficients), "
                                + ("learned MLP warm start" if args.warm_start == "mlp" else
                                   "warm start from the previous step")

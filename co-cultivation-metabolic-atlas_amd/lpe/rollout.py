@@ -28,7 +28,7 @@ import torch
 
 from . import _lib
 from .engine import BatchedLpEngine, SolveOptions
-from .workloads import BOUND, Consortium, consortium
+from .workloads import BOUND, UNLIMITED_SUPPLY, Consortium, consortium
 
 GROWTH_SCALE = 1e-4
 
@@ -45,18 +45,24 @@ class ConsortiumRollout:
         self.device = device or torch.device("cuda", torch.cuda.current_device())
         self.options = options or SolveOptions()
         self.warm_start = warm_start
-        self.engine = BatchedLpEngine(md.A, md.neq, self.n_env, var_positions=md.coupling_nnz.ravel(),
+        self.engine = BatchedLpEngine(md.A, md.neq, self.n_env, var_positions=md.var_positions,
                                       precision=precision, shared_c=True, device=self.device)
         rng = np.random.default_rng(seed + 7)
         lo, hi = conc_range
         dev = self.device
-        self.conc = torch.as_tensor(np.exp(rng.uniform(np.log(lo), np.log(hi), size=(self.n_env, md.medium_cols.shape[1]))),
+        self.conc = torch.as_tensor(np.exp(rng.uniform(np.log(lo), np.log(hi), size=(self.n_env, md.n_medium))),
                                     device=dev)
-        self.biomass = torch.as_tensor(rng.uniform(0.05, 0.5, size=(self.n_env, md.medium_cols.shape[0])), device=dev)
+        self.biomass = torch.as_tensor(rng.uniform(0.05, 0.5, size=(self.n_env, md.n_species)), device=dev)
         self._lower0 = torch.as_tensor(md.lower, device=dev)
         self._upper0 = torch.as_tensor(md.upper, device=dev)
         self._rhs_species = torch.as_tensor(md.rhs_species, device=dev)
-        self._medium_cols = torch.as_tensor(md.medium_cols, device=dev, dtype=torch.long)
+        ts, ti = md.terms()
+        self._term_species = torch.as_tensor(ts, device=dev, dtype=torch.long)
+        self._term_medium = torch.as_tensor(ti, device=dev, dtype=torch.long)
+        self._term_cols = torch.as_tensor(md.term_cols, device=dev, dtype=torch.long)
+        unl = md.unlimited()
+        self._mm_terms = torch.as_tensor(np.flatnonzero(~unl[ti]), device=dev, dtype=torch.long)
+        self._unlimited = torch.as_tensor(unl, device=dev)
         self._biomass_cols = torch.as_tensor(md.biomass_cols, device=dev, dtype=torch.long)
         self._vmax = torch.as_tensor(md.vmax, device=dev)
         self._km = torch.as_tensor(md.km, device=dev)
@@ -77,22 +83,25 @@ class ConsortiumRollout:
         biomass = self.biomass if biomass is None else biomass
         B = conc.shape[0]
         lower = self._lower0.expand(B, -1).clone()
-        for k in range(md.medium_cols.shape[0]):
-            mm = -self._vmax[k] * conc / (self._km[k] + conc)
-            lower[:, self._medium_cols[k]] = torch.clamp(mm, min=-BOUND)
+        t = self._mm_terms
+        s, i = self._term_species[t], self._term_medium[t]
+        ci = conc[:, i]
+        mm = -self._vmax[s, i] * ci / (self._km[s, i] + ci)
+        lower[:, self._term_cols[t]] = torch.clamp(mm, min=-BOUND)
         upper = self._upper0.expand(B, -1).contiguous()
         rhs = torch.zeros((B, md.shape[0]), dtype=torch.float64, device=self.device)
         rhs[:, :md.neq] = self._rhs_species
-        rhs[:, md.neq:md.neq + conc.shape[1]] = conc / self.dt
-        coupling = -torch.clamp(biomass, min=1e-12)[:, :, None].expand(-1, -1, conc.shape[1]).reshape(B, -1)
+        rhs[:, md.neq:md.neq + conc.shape[1]] = torch.where(self._unlimited, UNLIMITED_SUPPLY, conc / self.dt)
+        coupling = -torch.clamp(biomass, min=1e-12)[:, self._term_species]
         return rhs, lower, upper, coupling.contiguous()
 
     def euler(self, x: torch.Tensor, mask: Optional[torch.Tensor] = None):
         """Medium and biomass update from the solved fluxes (skipped for LPs
         that did not certify, whose environment keeps its state)."""
         md = self.model
-        v_ex = x[:, self._medium_cols]                       # [B, 3, K]
-        delta = (v_ex * self.biomass[:, :, None]).sum(dim=1) * self.dt
+        terms = torch.zeros((x.shape[0], md.n_species, md.n_medium), dtype=x.dtype, device=x.device)
+        terms[:, self._term_species, self._term_medium] = x[:, self._term_cols] * self.biomass[:, self._term_species]
+        delta = terms.sum(dim=1) * self.dt                   # species summed in order (deterministic)
         conc = torch.clamp(self.conc + delta, min=0.0)
         mu = x[:, self._biomass_cols] * GROWTH_SCALE
         biomass = self.biomass * (1.0 + mu * self.dt)

@@ -294,7 +294,7 @@ class WarmStartNet:
 def consortium_features(model, conc: torch.Tensor, lower: torch.Tensor, biomass: torch.Tensor) -> torch.Tensor:
     """[B, 2K + S + S*K]: log concentrations, concentrations, biomass and the
     uptake lower bounds of every species' medium exchanges."""
-    mc = torch.as_tensor(model.medium_cols.ravel(), device=lower.device, dtype=torch.long)
+    mc = torch.as_tensor(model.term_cols, device=lower.device, dtype=torch.long)
     return torch.cat([torch.log(conc + 1e-6), conc, biomass, lower[:, mc]], dim=1).to(torch.float32)
 
 
@@ -321,7 +321,7 @@ class LearnedWarmStart:
             raise ValueError("mode must be 'predict' or 'correct'")
         self.model = model
         m, n = model.shape
-        self.n_features = 2 * model.medium_cols.shape[1] + model.medium_cols.shape[0] + model.medium_cols.size
+        self.n_features = 2 * model.n_medium + model.n_species + len(model.term_cols)
         self.net = WarmStartNet(self.n_features, n, m, n, hidden=hidden, layers=layers, device=device, seed=seed)
         self.snap = snap
         self.mode = mode

@@ -160,6 +160,7 @@ def uptake_batch(batch: int = 256, seed: int = 1, m: int = 1000, n: int = 1500) 
 
 
 SPECIES_SHAPES = ((1200, 1800), (1500, 2200), (900, 1300))
+UNLIMITED_SUPPLY = 1e6      # medium-row supply of unlimited metabolites (src/community_solver.py:1146-1148)
 N_MEDIUM = 60
 
 
@@ -173,6 +174,13 @@ class Consortium:
     ``-biomass_s`` (consumption is a negative exchange flux); these
     ``3*N_MEDIUM`` entries are the per-environment matrix values, located at
     ``coupling_nnz`` positions of the CSR value array.
+
+    A member need not exchange every medium metabolite (models read from SBML,
+    ``lpe.sbml``): ``medium_mask[s, i]`` marks the present (species, metabolite)
+    terms, and absent ones hold -1 in ``medium_cols`` / ``coupling_nnz``.  The
+    present terms, species-major, are the per-environment coefficients
+    (``var_positions``); ``medium_unlimited[i]`` marks metabolites whose row
+    supply is effectively unbounded (water, protons: src/community_solver.py:1146-1148).
     """
 
     A: csr_matrix                 # template with biomass = 1
@@ -190,10 +198,40 @@ class Consortium:
     vmax: np.ndarray              # [3, N_MEDIUM]
     km: np.ndarray                # [3, N_MEDIUM]
     t_col: int
+    medium_mask: np.ndarray | None = None        # [S, K] bool (None: all present)
+    medium_unlimited: np.ndarray | None = None   # [K] bool (None: none)
 
     @property
     def shape(self):
         return self.A.shape
+
+    @property
+    def n_medium(self) -> int:
+        return self.medium_cols.shape[1]
+
+    @property
+    def n_species(self) -> int:
+        return self.medium_cols.shape[0]
+
+    def terms(self):
+        """(species, medium index) of the present medium terms, species-major."""
+        mask = self.medium_mask if self.medium_mask is not None else np.ones(self.medium_cols.shape, bool)
+        return np.nonzero(mask)
+
+    @property
+    def term_cols(self) -> np.ndarray:
+        s, i = self.terms()
+        return self.medium_cols[s, i]
+
+    @property
+    def var_positions(self) -> np.ndarray:
+        """CSR value positions of the per-environment coefficients (-biomass_s)."""
+        s, i = self.terms()
+        return self.coupling_nnz[s, i]
+
+    def unlimited(self) -> np.ndarray:
+        return (self.medium_unlimited if self.medium_unlimited is not None
+                else np.zeros(self.n_medium, dtype=bool))
 
 
 def consortium(seed: int = 2, shapes=SPECIES_SHAPES, n_medium: int = N_MEDIUM) -> Consortium:
@@ -264,9 +302,11 @@ class ConsortiumBatch:
         """[B, n] lower bounds with Michaelis-Menten uptake on medium exchanges."""
         md = self.model
         lo = np.tile(md.lower, (self.batch, 1))
-        for k in range(md.medium_cols.shape[0]):
-            lo[:, md.medium_cols[k]] = np.maximum(
-                michaelis_menten_lower(md.vmax[k], md.km[k], self.conc), -BOUND)
+        unl = md.unlimited()
+        for s, i in zip(*md.terms()):
+            if not unl[i]:
+                lo[:, md.medium_cols[s, i]] = np.maximum(
+                    michaelis_menten_lower(md.vmax[s, i], md.km[s, i], self.conc[:, i]), -BOUND)
         return lo
 
     def upper(self) -> np.ndarray:
@@ -278,17 +318,18 @@ class ConsortiumBatch:
         m = md.shape[0]
         out = np.zeros((self.batch, m))
         out[:, :md.neq] = md.rhs_species
-        out[:, md.medium_rows] = self.conc / self.dt
+        out[:, md.medium_rows] = np.where(md.unlimited(), UNLIMITED_SUPPLY, self.conc / self.dt)
         return out
 
     def coupling_values(self) -> np.ndarray:
-        """[B, 3, N_MEDIUM] values of the medium-row coefficients (-biomass_s)."""
-        bm = np.maximum(self.biomass, 1e-12)
-        return -np.repeat(bm[:, :, None], self.model.medium_cols.shape[1], axis=2)
+        """[B, T] values of the present medium terms' coefficients (-biomass_s),
+        in ``var_positions`` order."""
+        s, _ = self.model.terms()
+        return -np.maximum(self.biomass, 1e-12)[:, s]
 
     def matrix(self, b: int) -> csr_matrix:
         A = self.model.A.copy()
-        A.data[self.model.coupling_nnz.ravel()] = self.coupling_values()[b].ravel()
+        A.data[self.model.var_positions] = self.coupling_values()[b]
         return A
 
     def problem(self, b: int):
