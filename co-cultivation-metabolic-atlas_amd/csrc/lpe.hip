@@ -579,6 +579,10 @@ constexpr int CG_BLOCK = 25;
 #ifndef LPE_CG_REL
 #define LPE_CG_REL 1e-20
 #endif
+// primal CG stop: |Dr (b - A x)|_2 <= LPE_CG_PRIMAL_TOL * min(Dr)
+#ifndef LPE_CG_PRIMAL_TOL
+#define LPE_CG_PRIMAL_TOL 1e-8
+#endif
 constexpr double CG_REL = LPE_CG_REL;   // rho = r.D^-1 r reduction at which a system stops
 
 void cg_steps(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, int grid_r, int grid_c, int steps) {
@@ -1230,7 +1234,7 @@ int lpe_create(const lpe_desc* d, lpe_handle** out) {
         double mdr = INFINITY;
         for (double v : h->dr) mdr = std::min(mdr, v);
         if (!(mdr < INFINITY)) mdr = 1.0;
-        h->cg_ptol2 = (1e-8 * mdr) * (1e-8 * mdr);
+        h->cg_ptol2 = (LPE_CG_PRIMAL_TOL * mdr) * (LPE_CG_PRIMAL_TOL * mdr);
     }
     // device structure
     std::vector<double> tval(nnz), sval(nnz), tsval(nnz);
