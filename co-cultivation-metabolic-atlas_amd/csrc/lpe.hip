@@ -577,11 +577,11 @@ int build_results(lpe_handle* h) {
 
 constexpr int CG_BLOCK = 25;
 #ifndef LPE_CG_REL
-#define LPE_CG_REL 1e-20
+#define LPE_CG_REL 1e-14
 #endif
 // primal CG stop: |Dr (b - A x)|_2 <= LPE_CG_PRIMAL_TOL * min(Dr)
 #ifndef LPE_CG_PRIMAL_TOL
-#define LPE_CG_PRIMAL_TOL 1e-8
+#define LPE_CG_PRIMAL_TOL 1e-6
 #endif
 constexpr double CG_REL = LPE_CG_REL;   // rho = r.D^-1 r reduction at which a system stops
 
@@ -1483,6 +1483,10 @@ int lpe_solve(lpe_handle* h, const lpe_solve_opts* o, lpe_solve_info* info, void
     const double ed = o->eps_certify_dual > 0 ? o->eps_certify_dual : 1e-7;
     const double eg = o->eps_certify_gap > 0 ? o->eps_certify_gap : 1e-7;
     const int K = o->check_interval;
+    // first chunk (a warm start may be polished after fewer iterations) and the
+    // graph chunk that tiles both chunk lengths
+    const int K0 = (o->first_check_interval > 0 && o->first_check_interval < K) ? o->first_check_interval : K;
+    const int G = (K % K0 == 0) ? K0 : K;
     const bool do_polish = o->polish != 0;
     const bool trace = std::getenv("LPE_TRACE") != nullptr;
     const int cg = o->polish_cg_iterations > 0 ? o->polish_cg_iterations : 200;
@@ -1589,9 +1593,9 @@ int lpe_solve(lpe_handle* h, const lpe_solve_opts* o, lpe_solve_info* info, void
     // iteration 0: a warm start may already be optimal (src/gpu_pdhg_corrector.py:318-328)
     if (running > 0) CHK(after_chunk(0));
     while (running > 0 && iters < o->max_iterations && !stop) {
-        const int k = (int)std::min<long>(K, o->max_iterations - iters);
+        const int k = (int)std::min<long>(inf.chunks == 0 ? K0 : K, o->max_iterations - iters);
         HIPCHK(hipEventRecord(e0, h->s));
-        CHK(run_iterations(h, k, k, o->use_graph));
+        CHK(run_iterations(h, k, k % G == 0 ? G : k, o->use_graph));
         HIPCHK(hipEventRecord(e1, h->s));
         HIPCHK(hipEventSynchronize(e1));
         HIPCHK(hipEventElapsedTime(&ms, e0, e1));

@@ -53,7 +53,8 @@ class LpeSolveOpts(C.Structure):
                 ("rows_per_wave", C.c_int32), ("lps_per_lane", C.c_int32),
                 ("eps_certify_primal", C.c_double), ("eps_certify_dual", C.c_double),
                 ("eps_certify_gap", C.c_double), ("stop_after_failed_polishes", C.c_int32),
-                ("polish_kkt_start", C.c_double), ("polish_stable_checks", C.c_int32)]
+                ("polish_kkt_start", C.c_double), ("polish_stable_checks", C.c_int32),
+                ("first_check_interval", C.c_int32)]
 
 
 class LpeSolveInfo(C.Structure):

@@ -57,6 +57,7 @@ class SolveOptions:
     stop_after_failed_polishes: int = 0
     polish_kkt_start: float = 0.0
     polish_stable_checks: int = 0
+    first_check_interval: int = 0
 
     def to_c(self) -> _lib.LpeSolveOpts:
         o = _lib.LpeSolveOpts()
@@ -74,6 +75,7 @@ class SolveOptions:
         o.stop_after_failed_polishes = int(self.stop_after_failed_polishes)
         o.polish_kkt_start = float(self.polish_kkt_start)
         o.polish_stable_checks = int(self.polish_stable_checks)
+        o.first_check_interval = int(self.first_check_interval)
         return o
 
 

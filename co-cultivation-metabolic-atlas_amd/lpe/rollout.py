@@ -20,6 +20,7 @@ dual), "none" (cold) or a callable (conc, lower, biomass, x_prev, y_prev) ->
 
 from __future__ import annotations
 
+import dataclasses
 import time
 from typing import Callable, Optional
 
@@ -127,10 +128,14 @@ class ConsortiumRollout:
         eng.reset(x0, y0)
         if x0 is not None and self.omega is not None:
             eng.set_primal_weight(self.omega)       # warm start of the primal weight
+        opts = self.options
+        if x0 is not None and not opts.first_check_interval:
+            # a warm start is checked (and possibly polished) after half a chunk
+            opts = dataclasses.replace(opts, first_check_interval=max(1, opts.check_interval // 2))
         if self.fallback and eng.precision == "fp32":
-            info, x, y, status = eng.solve_with_fallback(self.options)
+            info, x, y, status = eng.solve_with_fallback(opts)
         else:
-            info = eng.solve(self.options)
+            info = eng.solve(opts)
             x, y, status = eng.state()
         certified = status == _lib.STATUS_OPTIMAL
         torch.cuda.synchronize()

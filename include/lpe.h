@@ -106,6 +106,10 @@ typedef struct lpe_solve_opts {
     int32_t polish_stable_checks; /* > 0: first polish only after the active
                                  * set held for this many checks (or 4 chunks
                                  * below polish_kkt)                          */
+    int32_t first_check_interval; /* > 0: iterations before the first check
+                                 * (a warm start may be polished sooner); the
+                                 * graph chunk is then this length when it
+                                 * divides check_interval (0 = check_interval) */
 } lpe_solve_opts;
 
 /* Summary of the last lpe_solve. */

@@ -57,7 +57,8 @@ def test_tiny_closed_form(device, precision):
     t_star = conc[:, 0] / (biomass[:, 0] + 2 * biomass[:, 1])
     np.testing.assert_allclose(out["t_star"].cpu().numpy(), t_star, rtol=1e-7)
     np.testing.assert_allclose(out["t"].cpu().numpy(), 0.99 * t_star, rtol=1e-7)
-    np.testing.assert_allclose(out["transfer"].sum(dim=1).cpu().numpy(), 0.99 * conc[:, 0], rtol=1e-7)
+    # the stage-3 objective to the path's bar (1e-6 relative; the certificate's gap gate is 1e-7)
+    np.testing.assert_allclose(out["transfer"].sum(dim=1).cpu().numpy(), 0.99 * conc[:, 0], rtol=1e-6)
 
 
 @pytest.mark.parametrize("live", [False, True])

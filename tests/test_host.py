@@ -127,3 +127,31 @@ def test_validated_problem_copies_and_rejects():
         _validated_problem((a, np.ones(3), np.zeros(2), np.ones(2), np.ones(2), 2))
     with pytest.raises(ValueError):
         _validated_problem((a, np.ones(2), np.ones(2), np.zeros(2), np.ones(2), 2))
+
+
+def test_ctypes_structs_match_the_c_layout(tmp_path):
+    """Size and every field offset of the ctypes mirrors equal the C compiler's
+    layout of include/lpe.h (gcc), so the Python layer and the library agree."""
+    import shutil
+    import subprocess
+    from lpe import _lib
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    structs = {"lpe_desc": _lib.LpeDesc, "lpe_vectors": _lib.LpeVectors,
+               "lpe_solve_opts": _lib.LpeSolveOpts, "lpe_solve_info": _lib.LpeSolveInfo}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "lpe.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
