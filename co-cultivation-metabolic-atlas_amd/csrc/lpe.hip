@@ -1136,6 +1136,19 @@ int init_state(lpe_handle* h) {
 
 }  // namespace
 
+// Shared with the other translation units of the library (reduction.hip).
+namespace lpe {
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+
+int handle_dims(const lpe_handle* h, int* m, int* n, int* batch) {
+    if (!h) return fail(LPE_EINVAL, "handle is required");
+    *m = h->m;
+    *n = h->n;
+    *batch = h->B;
+    return LPE_OK;
+}
+}  // namespace lpe
+
 // ===========================================================================
 // C ABI
 // ===========================================================================

@@ -1,4 +1,5 @@
-"""ctypes binding of liblpe.so (declarations mirror include/lpe.h).
+"""ctypes binding of liblpe.so (declarations mirror include/lpe.h and
+include/lpe_reduction.h).
 
 The HIP library is the only compute path: importing this module raises if the
 library has not been built, and nothing falls back to CPU arithmetic.
@@ -27,7 +28,12 @@ METRIC_NAMES = ("primal_residual", "dual_violation", "relative_kkt_gap", "object
 EXPORTS = ("lpe_create", "lpe_update", "lpe_reset", "lpe_run", "lpe_set_accepted", "lpe_solve",
            "lpe_certify", "lpe_state", "lpe_polish", "lpe_spmv", "lpe_get_scaling",
            "lpe_get_primal_weight", "lpe_set_primal_weight", "lpe_set_launch", "lpe_time_iterations", "lpe_time_kernel", "lpe_destroy",
-           "lpe_last_error", "lpe_version")
+           "lpe_last_error", "lpe_version",
+           "lpe_reduction_create", "lpe_reduction_get_dims", "lpe_reduction_get_plan",
+           "lpe_reduction_get_pattern", "lpe_reduction_get_lift_map", "lpe_reduction_reduce",
+           "lpe_reduction_expand", "lpe_reduction_compress", "lpe_reduction_lift", "lpe_reduction_destroy")
+
+RED_INFEASIBLE, RED_OVERFLOW, RED_INHOMOGENEOUS = 1, 2, 4
 
 
 class LpeDesc(C.Structure):
@@ -67,6 +73,24 @@ class LpeSolveInfo(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class LpeReductionDesc(C.Structure):
+    _fields_ = [("m", C.c_int32), ("n", C.c_int32), ("neq", C.c_int32), ("nnz", C.c_int64),
+                ("row_ptr", C.POINTER(C.c_int32)), ("col_idx", C.POINTER(C.c_int32)),
+                ("values", C.POINTER(C.c_double)), ("rhs", C.POINTER(C.c_double)),
+                ("n_var", C.c_int32), ("var_pos", C.POINTER(C.c_int64)), ("max_scale_ratio", C.c_double)]
+
+
+class LpeReductionDims(C.Structure):
+    _fields_ = [("m", C.c_int32), ("n", C.c_int32), ("neq", C.c_int32), ("nnz", C.c_int64),
+                ("n_var", C.c_int32), ("n_eliminated", C.c_int32), ("lift_nnz", C.c_int64)]
+
+
+class LpeReduced(C.Structure):
+    _fields_ = [("c", C.c_void_p), ("rhs", C.c_void_p), ("lower", C.c_void_p), ("upper", C.c_void_p),
+                ("var_values", C.c_void_p), ("lower_witness", C.c_void_p), ("upper_witness", C.c_void_p),
+                ("status", C.c_void_p)]
+
+
 class LpeError(RuntimeError):
     pass
 
@@ -104,6 +128,16 @@ def load() -> C.CDLL:
         "lpe_destroy": ([P], C.c_int),
         "lpe_last_error": ([C.c_char_p, I64], C.c_int),
         "lpe_version": ([], C.c_char_p),
+        "lpe_reduction_create": ([C.POINTER(LpeReductionDesc), C.POINTER(P)], C.c_int),
+        "lpe_reduction_get_dims": ([P, C.POINTER(LpeReductionDims)], C.c_int),
+        "lpe_reduction_get_plan": ([P, P, P, P, P, P], C.c_int),
+        "lpe_reduction_get_pattern": ([P, P, P, P, P], C.c_int),
+        "lpe_reduction_get_lift_map": ([P, P, P, P], C.c_int),
+        "lpe_reduction_reduce": ([P, I32, I32, C.POINTER(LpeVectors), C.POINTER(LpeReduced), I32, P], C.c_int),
+        "lpe_reduction_expand": ([P, I32, P, P, I32, P], C.c_int),
+        "lpe_reduction_compress": ([P, I32, P, P, I32, P], C.c_int),
+        "lpe_reduction_lift": ([P, P, I32, I32, P, P, P, P, P, P, P, P, I32, P], C.c_int),
+        "lpe_reduction_destroy": ([P], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -16,6 +16,7 @@ from lpe import requests
 from lpe.corrector import _validated_problem, shared_pattern
 
 LIBS = {"lpe.h": os.path.join(PKG, "lpe", "liblpe.so"),
+        "lpe_reduction.h": os.path.join(PKG, "lpe", "liblpe.so"),
         "warmstart.h": os.path.join(PKG, "lpe", "libwarmstart.so")}
 
 
@@ -39,7 +40,7 @@ def test_library_exports_every_declared_symbol(header):
 
 def test_python_bindings_cover_the_headers():
     from lpe import _lib, warmstart
-    assert set(declared("lpe.h")) == set(_lib.EXPORTS)
+    assert set(declared("lpe.h")) | set(declared("lpe_reduction.h")) == set(_lib.EXPORTS)
     assert set(declared("warmstart.h")) == set(warmstart.EXPORTS)
 
 
@@ -131,15 +132,18 @@ def test_validated_problem_copies_and_rejects():
 
 def test_ctypes_structs_match_the_c_layout(tmp_path):
     """Size and every field offset of the ctypes mirrors equal the C compiler's
-    layout of include/lpe.h (gcc), so the Python layer and the library agree."""
+    layout of include/lpe.h and include/lpe_reduction.h (gcc), so the Python
+    layer and the library agree."""
     import shutil
     import subprocess
     from lpe import _lib
     if shutil.which("gcc") is None:
         pytest.skip("gcc not available")
     structs = {"lpe_desc": _lib.LpeDesc, "lpe_vectors": _lib.LpeVectors,
-               "lpe_solve_opts": _lib.LpeSolveOpts, "lpe_solve_info": _lib.LpeSolveInfo}
-    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "lpe.h"', "int main(void) {"]
+               "lpe_solve_opts": _lib.LpeSolveOpts, "lpe_solve_info": _lib.LpeSolveInfo,
+               "lpe_reduction_desc": _lib.LpeReductionDesc, "lpe_reduction_dims": _lib.LpeReductionDims,
+               "lpe_reduced": _lib.LpeReduced}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "lpe_reduction.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
         for fname, _ in py._fields_:
