@@ -56,9 +56,10 @@ def _validated_problem(problem: Any):
     return a, rhs, lower, upper, c, neq
 
 
-def shared_pattern(matrices):
+def shared_pattern(matrices, all_entries: bool = False):
     """Union pattern of same-shape CSR matrices, the template values (LP 0)
-    and the positions / per-LP values of every entry that differs."""
+    and the positions / per-LP values of every entry that differs (of every
+    entry when ``all_entries``, so any later values fit the pattern)."""
     shape = matrices[0].shape
     union = abs(matrices[0])
     for a in matrices[1:]:
@@ -70,9 +71,17 @@ def shared_pattern(matrices):
     vals = [np.asarray(csr_matrix(a)[rows, union.indices]).ravel() if union.nnz else np.zeros(0)
             for a in matrices]
     vals = np.stack(vals)
-    varying = np.flatnonzero(np.any(vals != vals[0], axis=0))
+    varying = np.arange(vals.shape[1]) if all_entries else np.flatnonzero(np.any(vals != vals[0], axis=0))
     template = csr_matrix((vals[0].copy(), union.indices.copy(), union.indptr.copy()), shape=shape)
     return template, varying, vals[:, varying]
+
+
+def pattern_values(template, matrices):
+    """Values of each matrix at the template's entries ([len(matrices), nnz])."""
+    rows = np.repeat(np.arange(template.shape[0]), np.diff(template.indptr))
+    if not template.nnz:
+        return np.zeros((len(matrices), 0))
+    return np.stack([np.asarray(csr_matrix(a)[rows, template.indices]).ravel() for a in matrices])
 
 
 class GpuPdhgCorrector:
@@ -82,6 +91,9 @@ class GpuPdhgCorrector:
     """
 
     name = "mi355x_fp64_batched_pdhg_original_lp_corrector"
+    # per-LP coefficient positions of the engine: "varying" (entries that
+    # differ in the constructor's batch) or "all" (values may change later)
+    per_lp_entries = "varying"
 
     def __init__(self, problems: Iterable[Any], *, step_safety: float = 0.99, theta: float = 1.0,
                  primal_weight: float = 1.0) -> None:
@@ -105,7 +117,8 @@ class GpuPdhgCorrector:
         self.m, self.n = shape
         self.neq = neq
         self.step_safety, self.theta, self.primal_weight = float(step_safety), float(theta), float(primal_weight)
-        template, varying, values = shared_pattern([p[0] for p in normalized])
+        template, varying, values = shared_pattern([p[0] for p in normalized], self.per_lp_entries == "all")
+        self._template = template
         host_assembly_seconds = time.perf_counter() - started
         t = time.perf_counter()
         self.engine = BatchedLpEngine(template, neq, self.batch, var_positions=varying, method="fixed",

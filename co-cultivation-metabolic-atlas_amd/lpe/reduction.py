@@ -30,7 +30,7 @@ from scipy.sparse import coo_matrix, csr_matrix
 
 from . import _lib
 from ._lib import check
-from .corrector import GpuPdhgCorrector, _validated_problem, shared_pattern
+from .corrector import GpuPdhgCorrector, _validated_problem, pattern_values, shared_pattern
 from .engine import BatchedLpEngine, SolveOptions, _device_array, _stream
 
 
@@ -495,7 +495,8 @@ class GpuReducedPdhgCorrector(GpuPdhgCorrector):
         self.original_m, self.original_n = shape
         self.original_neq = neq
         before = time.perf_counter()
-        template, varying, values = shared_pattern([p[0] for p in original])
+        template, varying, values = shared_pattern([p[0] for p in original], self.per_lp_entries == "all")
+        self._original_template = template
         self._original = BatchedLpEngine(template, neq, self.batch, var_positions=varying, method="fixed",
                                          device=self.engine.device)
         self._original.update(np.stack([p[4] for p in original]), np.stack([p[1] for p in original]),
@@ -578,4 +579,158 @@ class GpuReducedPdhgCorrector(GpuPdhgCorrector):
                            "restoration, strict ORIGINAL-LP certificate. No CPU LP fallback.")
         self.history[-1]["timing"] = dict(result["timing"])
         self.history[-1]["equality_reduction"] = True
+        return result
+
+
+def _same_pattern(left, right) -> bool:
+    return (left.shape == right.shape and np.array_equal(left.indptr, right.indptr)
+            and np.array_equal(left.indices, right.indices))
+
+
+class GpuResidentReducedPdhg(GpuReducedPdhgCorrector):
+    """Persistent fixed-pattern reduced service (src/gpu_resident_reduced_pdhg.py:23-177).
+
+    ``try_update`` refreshes every value of a same-pattern batch in the
+    resident engines and returns False (without mutation) for batch / order /
+    CSR-pattern changes; invalid numeric input raises before any mutation.
+    ``solve`` certifies the lifted original LP at every checkpoint and lets
+    the final certificate decide acceptance; results are owned copies.
+    """
+
+    name = "mi355x_resident_reduced_pdhg_original_certificate"
+    per_lp_entries = "all"
+
+    def __init__(self, problems, *, environment_ids=None, chunk_size: int = 64, use_graph: bool = True, plan=None,
+                 **kwargs):
+        started = time.perf_counter()
+        super().__init__(problems, plan=plan, **kwargs)
+        self.environment_ids = tuple(range(self.batch)) if environment_ids is None else tuple(environment_ids)
+        if len(self.environment_ids) != self.batch:
+            raise ValueError("environment_ids must name every LP of the batch")
+        self.chunk_size = int(chunk_size)
+        self.use_graph = bool(use_graph)
+        self.setup_timing.update(setup_total_seconds=time.perf_counter() - started)
+        self.last_update_timing = dict(reused=False, reason="initial_setup", total_seconds=0.0)
+        self._valid = True
+
+    def try_update(self, problems, *, environment_ids=None) -> bool:
+        started = time.perf_counter()
+        original = tuple(_validated_problem(p) for p in problems)
+        ids = tuple(range(len(original))) if environment_ids is None else tuple(environment_ids)
+
+        def rebuild(reason):
+            self.last_update_timing = dict(reused=False, reason=reason, total_seconds=time.perf_counter() - started)
+            return False
+
+        if len(original) != self.batch or ids != self.environment_ids:
+            return rebuild("batch_or_environment_order_changed")
+        # fingerprint, homogeneity, dimensions and bound intersections are
+        # validated before any resident buffer changes
+        reductions = tuple(self.plan.reduce(p) for p in original)
+        reduced = tuple(r.problem for r in reductions)
+        if not all(_same_pattern(p[0], q[0]) for p, q in zip(original, self.original_problems)):
+            return rebuild("original_csr_pattern_changed")
+        if not all(_same_pattern(p[0], q[0]) for p, q in zip(reduced, self.reduced_problems)):
+            return rebuild("reduced_csr_pattern_changed")
+        preparation = time.perf_counter() - started
+        solve_problems = reduced
+        if self._pad:
+            empty = csr_matrix((1, self.plan.reduced_variables))
+            solve_problems = tuple((empty, np.zeros(1), p[2], p[3], p[4], 0) for p in reduced)
+        torch.cuda.synchronize()
+        self._valid = False
+        before = time.perf_counter()
+        self.engine.update(np.stack([p[4] for p in solve_problems]), np.stack([p[1] for p in solve_problems]),
+                           np.stack([p[2] for p in solve_problems]), np.stack([p[3] for p in solve_problems]),
+                           var_values=pattern_values(self._template, [p[0] for p in solve_problems]))
+        self._original.update(np.stack([p[4] for p in original]), np.stack([p[1] for p in original]),
+                              np.stack([p[2] for p in original]), np.stack([p[3] for p in original]),
+                              var_values=pattern_values(self._original_template, [p[0] for p in original]))
+        torch.cuda.synchronize()
+        self.original_problems = original
+        self.reductions, self.reduced_problems = reductions, reduced
+        self.problems = solve_problems
+        self._lw = np.stack([r.lower_witness for r in reductions])
+        self._uw = np.stack([r.upper_witness for r in reductions])
+        self._c = np.stack([p[4] for p in original])
+        self._last_lift = None
+        self._valid = True
+        self.last_update_timing = dict(reused=True, reason="same_pattern_value_update",
+                                       host_preparation_seconds=preparation,
+                                       device_update_seconds=time.perf_counter() - before,
+                                       total_seconds=time.perf_counter() - started)
+        return True
+
+    @property
+    def lower_witness(self) -> np.ndarray:
+        return self._lw.copy()
+
+    @property
+    def upper_witness(self) -> np.ndarray:
+        return self._uw.copy()
+
+    def solve(self, initial_x=None, initial_y=None, *, iterations: int = 1000, check_interval: int = 64):
+        if not self._valid:
+            raise RuntimeError("An interrupted update invalidated this solver; rebuild it")
+        iterations = self._valid_budget(iterations, "iterations", allow_zero=True)
+        check_interval = self._valid_budget(check_interval, "check_interval", allow_zero=False)
+        started = time.perf_counter()
+        dev = self.engine.device
+        before = time.perf_counter()
+        full_x = self._warm(initial_x, (self.batch, self.original_n), "initial_x")
+        full_y = self._warm(initial_y, (self.batch, self.original_m), "initial_y")
+        z = None if full_x is None else torch.as_tensor(self.plan.compress_primal(full_x.cpu().numpy()), device=dev)
+        yr = None
+        if full_y is not None:
+            yr = torch.zeros((self.batch, self.m), dtype=torch.float64, device=dev)
+            if self.m - self._pad:
+                yr[:, :self.m - self._pad] = full_y.to(dev)[:, self._kept]
+        self.engine.reset(z, yr)
+        torch.cuda.synchronize()
+        warm_seconds = time.perf_counter() - before
+        correction_seconds = certificate_seconds = control_seconds = 0.0
+        accepted = np.zeros(self.batch, dtype=bool)
+        accepted_iteration = np.full(self.batch, -1, dtype=np.int64)
+        checkpoints = []
+        completed = 0
+        while True:
+            zs, ys, _ = self.engine.state()
+            before = time.perf_counter()
+            metrics = self._certificate(zs, ys)
+            elapsed = time.perf_counter() - before
+            certificate_seconds += elapsed
+            checkpoints.append(self._checkpoint(completed, metrics, elapsed))
+            now = np.array([r["certificate_passed"] for r in metrics], dtype=bool)
+            accepted_iteration[~accepted & now] = completed
+            accepted |= now
+            if accepted.all() or completed == iterations:
+                break
+            before = time.perf_counter()
+            self.engine.set_accepted(accepted)
+            control_seconds += time.perf_counter() - before
+            count = min(check_interval, iterations - completed)
+            before = time.perf_counter()
+            self.engine.run(count, chunk=self.chunk_size, use_graph=self.use_graph)
+            torch.cuda.synchronize()
+            correction_seconds += time.perf_counter() - before
+            completed += count
+        # the final lifted certificate is the authority for the returned candidate
+        accepted &= now
+        accepted_iteration[~accepted] = -1
+        for i, row in enumerate(metrics):
+            row.update(accepted_iteration=int(accepted_iteration[i]), success=bool(accepted[i]), solver=self.name)
+        timing = dict(warm_start_transfer_seconds=warm_seconds, correction_seconds=correction_seconds,
+                      certificate_seconds=certificate_seconds, host_control_seconds=control_seconds,
+                      solve_total_seconds=time.perf_counter() - started)
+        x, y = self._last_lift
+        result = dict(x=x.clone(), y=y.clone(), reduced_x=zs.clone(), reduced_y=ys[:, :self.m - self._pad].clone(),
+                      accepted=accepted, all_accepted=bool(accepted.all()), iterations_run=completed, metrics=metrics,
+                      checkpoints=checkpoints, timing=timing, cpu_lp_calls=0, primal_weight=self.primal_weight,
+                      equality_reduction=True,
+                      scope="Resident FP64 reduced MI355X iterations; original-LP certification; "
+                            "host input/update/control remains; no CPU optimizer or fallback")
+        self.history.append(dict(batch=self.batch, iterations_run=completed, all_accepted=bool(accepted.all()),
+                                 accepted=accepted.tolist(), metrics=[dict(r) for r in metrics], timing=dict(timing),
+                                 cpu_lp_calls=0, update=dict(self.last_update_timing),
+                                 execution="graph" if self.use_graph else "loop"))
         return result
