@@ -41,6 +41,20 @@ def _primal_bytes(B, m, n, nnz, nvar, w):
     return B * per_lp + shared
 
 
+def _cg_update_bytes(B, m, n_rb, weighted=False):
+    """One polish CG update, both systems, every LP active: w, p, r, s of two
+    systems read, w and r written, the Jacobi diagonal read (13 m FP64 words
+    per LP), alpha / activity / selection per LP, the block partials of three
+    dot products written; the row scales once when the stop test is weighted."""
+    return B * (13 * m * 8 + 2 * 8 + 2 * 4 + 4 + 3 * n_rb * 8) + (m * 8 if weighted else 0)
+
+
+def _polish_row_blocks(m, ld):
+    """Row blocks of the polish launches (4 waves per block, rows per wave by batch)."""
+    rpw = 4 if ld >= 1024 else (2 if ld >= 256 else 1)
+    return max(1, -(-m // (4 * rpw)))
+
+
 def _dual_bytes(B, m, n, nnz, nvar, w):
     """Dual step: y r/w, rhs, ysum r/w (5m words), gathered xbar (n words)."""
     per_lp = (5 * m + n + nvar) * w + 8
@@ -135,6 +149,7 @@ def probe_kernels(args):
     eng.run(64, chunk=64, use_graph=False)
     ms_p = eng.time_kernel("primal", reps=args.probe_reps)
     ms_d = eng.time_kernel("dual", reps=args.probe_reps)
+    ms_cg = eng.time_kernel("cg_update", reps=args.probe_reps)
     n_copy = 256 << 20                                   # 256 Mi float32 = 1 GiB read + 1 GiB written
     src = torch.ones(n_copy, dtype=torch.float32, device=dev)
     dst = torch.empty_like(src)
@@ -144,7 +159,9 @@ def probe_kernels(args):
     m, n = model.shape
     print(json.dumps({"probe": "step kernels", "envs": args.envs, "rows_per_wave": args.rows_per_wave,
                       "lps_per_lane": args.lps_per_lane, "reps": args.probe_reps,
-                      "primal_ms": ms_p, "dual_ms": ms_d, "copy_bytes_read": 4 * n_copy,
+                      "primal_ms": ms_p, "dual_ms": ms_d, "cg_update_ms": ms_cg,
+                      "cg_update_algorithmic_bytes": _cg_update_bytes(args.envs, m, _polish_row_blocks(m, args.envs)),
+                      "copy_bytes_read": 4 * n_copy,
                       "copy_bytes_written": 4 * n_copy,
                       "primal_algorithmic_bytes": _primal_bytes(args.envs, m, n, model.A.nnz,
                                                                 model.var_positions.size, 4),
@@ -152,17 +169,18 @@ def probe_kernels(args):
                                                             model.var_positions.size, 4)}), flush=True)
 
 
-def measured_traffic(path, envs, lps_per_lane):
-    """HBM bytes per primal-step launch from a committed counter summary:
-    the line 'primal_step_hbm_bytes envs=E lpl=L fetch=F write=W'."""
+def measured_traffic(path, envs, lps_per_lane, key="primal_step_hbm_bytes"):
+    """HBM bytes per launch from a committed counter summary: the line
+    '<key> envs=E lpl=L fetch=F write=W' (key primal_step_hbm_bytes or
+    cg_update_hbm_bytes)."""
     try:
         with open(path) as f:
             for line in f:
                 parts = line.split()
-                if parts[:1] != ["primal_step_hbm_bytes"]:
+                if parts[:1] != [key]:
                     continue
                 kv = dict(p.split("=", 1) for p in parts[1:] if "=" in p)
-                if int(kv["envs"]) == envs and int(kv["lpl"]) == lps_per_lane:
+                if int(kv["envs"]) == envs and int(kv.get("lpl", lps_per_lane)) == lps_per_lane:
                     return int(float(kv["fetch"]) + float(kv["write"]))
     except (OSError, ValueError, KeyError):
         return None
@@ -235,13 +253,14 @@ def cooperative_stages(model, B, dev, snapshot, args):
     import torch
     from lpe.engine import SolveOptions
     from lpe.rollout import ConsortiumRollout
-    from lpe.stages import CooperativeStages
+    from lpe.stages import CooperativeStages, StageOptions
 
     ro = ConsortiumRollout(model, B, seed=args.seed, autotune=False, device=dev)
     ro.restore(snapshot)
     rhs, lower, upper, coupling = ro.lp_inputs()
     drv = CooperativeStages(model, B, precision="fp32", device=dev,
-                            options=SolveOptions(max_iterations=args.fp32_iterations))
+                            options=SolveOptions(max_iterations=args.fp32_iterations),
+                            stages=StageOptions(later_max_iterations=args.max_iterations))
     drv.solve(rhs, lower, upper, coupling, ro.biomass)          # code objects, graphs
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -333,7 +352,7 @@ def main():
         if k == 0:
             first_x, first_ok = out["x"], np.asarray(out["certified_mask"])
         step_logs.append({kk: out.get(kk) for kk in ("certified", "iterations", "restarts", "polish_attempts",
-                                                     "pdhg_ms", "check_ms", "polish_ms", "wall_seconds",
+                                                     "cg_steps", "pdhg_ms", "check_ms", "polish_ms", "wall_seconds",
                                                      "fallback_lps", "fallback_certified", "fallback_seconds")})
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
@@ -341,17 +360,37 @@ def main():
     print(f"# target: {certified}/{B * args.steps} certified in {elapsed:.2f} s", file=sys.stderr, flush=True)
     first_x_host = first_x.cpu().numpy()
 
-    # dominant kernel (primal step) and dual step, timed with hipEvents on the engine stream
+    # the step's kernels timed with hipEvents on the engine stream (every LP
+    # active): the PDHG half steps and the polish's CG update; the dominant
+    # one (largest estimated time per timed step) carries the roofline
     target_tuning = target.tuning
     eng = target.engine
     eng.reset(target.x, target.y)
     ms_primal = eng.time_kernel("primal", reps=20)
     ms_dual = eng.time_kernel("dual", reps=20)
+    ms_cg = eng.time_kernel("cg_update", reps=20)
     lpl = int((target_tuning or {}).get("lps_per_lane", 1))
     pb = _primal_bytes(B, m, n, nnz, nvar, 4)
-    traffic = measured_traffic(args.traffic_file, B, lpl)
     db = _dual_bytes(B, m, n, nnz, nvar, 4)
-    achieved = pb / (ms_primal * 1e-3) / 1e9
+    cgb = _cg_update_bytes(B, m, _polish_row_blocks(m, B))
+    mean_iters = float(np.mean([s["iterations"] for s in step_logs]))
+    mean_cg = float(np.mean([s.get("cg_steps") or 0 for s in step_logs]))
+    kernels = {
+        f"k_primal_step<float,{lpl},false>": dict(algorithmic_bytes_per_launch=pb, avg_launch_ms=round(ms_primal, 5),
+                                                  achieved=round(pb / (ms_primal * 1e-3) / 1e9, 1),
+                                                  launches_per_step=mean_iters,
+                                                  ms_per_step=round(mean_iters * ms_primal, 3),
+                                                  traffic=measured_traffic(args.traffic_file, B, lpl)),
+        f"k_dual_step<float,{lpl},false>": dict(algorithmic_bytes_per_launch=db, avg_launch_ms=round(ms_dual, 5),
+                                                achieved=round(db / (ms_dual * 1e-3) / 1e9, 1),
+                                                launches_per_step=mean_iters,
+                                                ms_per_step=round(mean_iters * ms_dual, 3)),
+        "k_cg_update": dict(algorithmic_bytes_per_launch=cgb, avg_launch_ms=round(ms_cg, 5),
+                            achieved=round(cgb / (ms_cg * 1e-3) / 1e9, 1), launches_per_step=mean_cg,
+                            ms_per_step=round(mean_cg * ms_cg, 3),
+                            traffic=measured_traffic(args.traffic_file, B, lpl, key="cg_update_hbm_bytes")),
+    }
+    dominant = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
 
     # plain form on the same LPs
     del target
@@ -404,12 +443,13 @@ def main():
         "warm_start": mlp_report or {"kind": "previous step's primal/dual"},
         "autotune": {k: v for k, v in (target_tuning or {}).items() if k != "timings"},
         "autotune_timings_ms_per_iteration": (target_tuning or {}).get("timings"),
-        "roofline": {"bound": "hbm", "kernel": f"k_primal_step<float,{lpl},false>",
-                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes_per_launch": pb, "avg_launch_ms": round(ms_primal, 5),
-                     "dual_step": {"algorithmic_bytes_per_launch": db, "avg_launch_ms": round(ms_dual, 5),
-                                   "achieved": round(db / (ms_dual * 1e-3) / 1e9, 1)},
+        "roofline": {"bound": "hbm", "kernel": dominant,
+                     "achieved": kernels[dominant]["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(kernels[dominant]["achieved"] / HBM_PEAK_GBS, 4),
+                     "traffic": kernels[dominant].get("traffic"),
+                     "algorithmic_bytes_per_launch": kernels[dominant]["algorithmic_bytes_per_launch"],
+                     "avg_launch_ms": kernels[dominant]["avg_launch_ms"],
+                     "step_kernels": kernels,
                      "primal_fp64_avg_launch_ms": None if ms_primal_fp64 is None else round(ms_primal_fp64, 5)},
     }
     if not args.no_cpu:

@@ -141,6 +141,7 @@ struct lpe_handle {
     int* d_cgact = nullptr;
     double* d_cgfloor = nullptr;
     int last_cg_steps = 0;
+    long cg_steps_total = 0;   // CG steps of every polish round so far
     double cg_ptol2 = 0.0;    // primal CG stop: |Dr (b - A x)|_2^2 <= (1e-8 min(dr))^2
     double *d_xpol = nullptr, *d_ypol = nullptr, *d_xres = nullptr, *d_yres = nullptr;
     double* d_metrics = nullptr;
@@ -575,11 +576,20 @@ int build_results(lpe_handle* h) {
     return LPE_OK;
 }
 
-constexpr int CG_BLOCK = 25;
+#ifndef LPE_CG_BLOCK
+#define LPE_CG_BLOCK 25
+#endif
+constexpr int CG_BLOCK = LPE_CG_BLOCK;
 #ifndef LPE_CG_REL
 #define LPE_CG_REL 1e-14
 #endif
-// primal CG stop: |Dr (b - A x)|_2 <= LPE_CG_PRIMAL_TOL * min(Dr)
+// primal CG stop: with LPE_CG_PRIMAL_WEIGHTED, |b - A x|_2 over the active
+// rows in original units <= LPE_CG_PRIMAL_TOL; otherwise the scaled residual
+// |Dr (b - A x)|_2 <= LPE_CG_PRIMAL_TOL * min(Dr) (the same bound, looser
+// when the row scales spread)
+#ifndef LPE_CG_PRIMAL_WEIGHTED
+#define LPE_CG_PRIMAL_WEIGHTED 0
+#endif
 #ifndef LPE_CG_PRIMAL_TOL
 #define LPE_CG_PRIMAL_TOL 1e-6
 #endif
@@ -606,7 +616,8 @@ void cg_steps(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, in
         CgUpdateArgs u;
         u.sh = sr; u.w1 = h->d_wp; u.r1 = h->d_rp_; u.p1 = h->d_pp; u.s1 = h->d_s1;
         u.w2 = h->d_wd; u.r2 = h->d_rd; u.p2 = h->d_pd; u.s2 = h->d_s2; u.alpha = h->d_alpha;
-        u.invdiag = h->d_invdiag; u.sel = v.sel; u.act = h->d_cgact; u.partial = h->d_partial;
+        u.invdiag = h->d_invdiag; u.dr = LPE_CG_PRIMAL_WEIGHTED ? h->d_dr : nullptr; u.sel = v.sel;
+        u.act = h->d_cgact; u.partial = h->d_partial;
         hipLaunchKernelGGL(k_cg_update, dim3(grid_r), dim3(BLOCK), 0, h->s, u);
         reduce_ld(h, h->d_partial, h->d_red_a, h->d_ops_sum, 3, sr.n_rb, v.ld);
         cs.mode = 2; cs.d0 = h->d_red_a; cs.d1 = h->d_red_a + v.ld; cs.plain = h->d_red_a + 2 * (long)v.ld;
@@ -665,7 +676,7 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
         a.sh = sr; a.A = op_orig_v(h, false, v); a.As = op_scaled_v(h, false, v); a.xs = h->d_xs; a.y = yin;
         a.rhs = v.rhs; a.dr = h->d_dr; a.orig = orig; a.fmask = h->d_fmask; a.invdiag = h->d_invdiag;
         a.sel = v.sel; a.rmask = h->d_rmask; a.rp = h->d_rp_; a.pp = h->d_pp; a.wp = h->d_wp; a.wd = h->d_wd;
-        a.neq = h->neq; a.tol_y = 1e-9; a.tol_s = 1e-7; a.partial = h->d_partial;
+        a.neq = h->neq; a.tol_y = 1e-9; a.tol_s = 1e-7; a.plain_orig = LPE_CG_PRIMAL_WEIGHTED; a.partial = h->d_partial;
         hipLaunchKernelGGL((k_polish_rows_init<T>), dim3(grid_r), dim3(BLOCK), 0, h->s, a);
         reduce_ld(h, h->d_partial, h->d_red_b, h->d_ops_sum, 2, sr.n_rb, v.ld);   // rho_p, |r_p|^2
         // dual setup: q1 = D (c^ - A^T wd); r_d = R A q1, p_d = r_d / diag
@@ -698,6 +709,12 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
                 HIPCHK(hipStreamSynchronize(h->s));
                 bool any = false;
                 for (long q = 0; q < 2 * ld && !any; ++q) any = act[q] != 0;
+                if (std::getenv("LPE_TRACE_CG")) {
+                    long na[2] = {0, 0};
+                    for (long q = 0; q < 2 * ld; ++q) na[q / ld] += act[q] != 0;
+                    std::fprintf(stderr, "[lpe]       cg after %d steps: active primal %ld dual %ld of %ld\n", used,
+                                 na[0], na[1], ld);
+                }
                 if (!any) break;
             }
             CHK(cg_block(h, v, sr, sc, grid_r, grid_c));
@@ -719,6 +736,7 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
             }
         }
         h->last_cg_steps = used;
+        h->cg_steps_total += used;
         if (std::getenv("LPE_TRACE")) {
             HIPCHK(hipStreamSynchronize(h->s));
             std::vector<double> rh(2 * ld), r0(2 * ld), fl(2 * ld), pl(ld);
@@ -1136,6 +1154,17 @@ int init_state(lpe_handle* h) {
 
 }  // namespace
 
+// Marks every LP of the batch selected and both CG systems active (timing of
+// the CG update over the full batch).
+__global__ void k_activate_all(int* sel, int* act, int batch, int ld) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= ld) return;
+    const int on = b < batch ? 1 : 0;
+    sel[b] = on;
+    act[b] = on;
+    act[ld + b] = on;
+}
+
 // Shared with the other translation units of the library (reduction.hip).
 namespace lpe {
 int set_error(int code, const std::string& msg) { return fail(code, msg); }
@@ -1246,7 +1275,7 @@ int lpe_create(const lpe_desc* d, lpe_handle** out) {
     {
         double mdr = INFINITY;
         for (double v : h->dr) mdr = std::min(mdr, v);
-        if (!(mdr < INFINITY)) mdr = 1.0;
+        if (!(mdr < INFINITY) || LPE_CG_PRIMAL_WEIGHTED) mdr = 1.0;
         h->cg_ptol2 = (LPE_CG_PRIMAL_TOL * mdr) * (LPE_CG_PRIMAL_TOL * mdr);
     }
     // device structure
@@ -1515,6 +1544,7 @@ int lpe_solve(lpe_handle* h, const lpe_solve_opts* o, lpe_solve_info* info, void
     bool stop = false;
     lpe_solve_info inf;
     std::memset(&inf, 0, sizeof(inf));
+    const long cg0 = h->cg_steps_total;
     float ms = 0.0f;
     long iters = 0;
     int running = 0;
@@ -1638,6 +1668,7 @@ int lpe_solve(lpe_handle* h, const lpe_solve_opts* o, lpe_solve_info* info, void
     inf.certified = certified;
     inf.restarts = restarts;
     inf.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    inf.cg_steps = h->cg_steps_total - cg0;
     if (info) *info = inf;
     return leave(h, stream);
 }
@@ -1819,13 +1850,29 @@ int lpe_time_iterations(lpe_handle* h, int64_t iterations, int32_t chunk, int32_
 
 int lpe_time_kernel(lpe_handle* h, int32_t kind, int32_t reps, double* ms_out, void* stream) {
     if (!h || !ms_out) return fail(LPE_EINVAL, "handle and ms are required");
-    if (kind != 0 && kind != 1) return fail(LPE_EINVAL, "kind must be 0 (dual step) or 1 (primal step)");
+    if (kind < 0 || kind > 2) return fail(LPE_EINVAL, "kind must be 0 (dual step), 1 (primal step) or 2 (CG update)");
     if (reps < 1) return fail(LPE_EINVAL, "reps must be positive");
     if (!h->ready) return fail(LPE_ESTATE, "reset is required after a problem update");
-    if (kind == 0 && h->m == 0) return fail(LPE_EINVAL, "no rows");
+    if (kind != 1 && h->m == 0) return fail(LPE_EINVAL, "no rows");
     CHK(enter(h, stream));
+    CgUpdateArgs u;
+    int grid_r = 0;
+    if (kind == 2) {
+        // every LP selected, both CG systems active, on the full view
+        hipLaunchKernelGGL(k_activate_all, dim3(ceil_div(h->ld, 256)), dim3(256), 0, h->s, h->d_sel, h->d_cgact,
+                           h->B, h->ld);
+        const View v = full_view(h);
+        u.sh = make_shape(h->m, v.ld, v.ld, polish_rpw(v.ld), 1, &grid_r);
+        u.w1 = h->d_wp; u.r1 = h->d_rp_; u.p1 = h->d_pp; u.s1 = h->d_s1;
+        u.w2 = h->d_wd; u.r2 = h->d_rd; u.p2 = h->d_pd; u.s2 = h->d_s2; u.alpha = h->d_alpha;
+        u.invdiag = h->d_invdiag; u.dr = LPE_CG_PRIMAL_WEIGHTED ? h->d_dr : nullptr; u.sel = v.sel;
+        u.act = h->d_cgact; u.partial = h->d_partial;
+    }
     HIPCHK(hipEventRecord(h->ev_a, h->s));
-    for (int i = 0; i < reps; ++i) launch_step(h, kind == 0 ? 1 : 2);
+    for (int i = 0; i < reps; ++i) {
+        if (kind == 2) hipLaunchKernelGGL(k_cg_update, dim3(grid_r), dim3(BLOCK), 0, h->s, u);
+        else launch_step(h, kind == 0 ? 1 : 2);
+    }
     HIPCHK(hipEventRecord(h->ev_b, h->s));
     HIPCHK(hipEventSynchronize(h->ev_b));
     float ms = 0.0f;

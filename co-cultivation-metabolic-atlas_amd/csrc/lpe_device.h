@@ -988,6 +988,7 @@ struct PolishRowsInitArgs {
     double* invdiag;         // [m][ld] 1 / (R A^ D A^T R)_ii, 0 on inactive rows
     int neq;
     double tol_y, tol_s;
+    int plain_orig;          // 1: |r|^2 in original units (r / dr), else scaled
     double* partial;         // [n_rb][2][ld]: r.D^-1 r and |r|^2 of the primal system
 };
 
@@ -1030,7 +1031,8 @@ __global__ void __launch_bounds__(BLOCK) k_polish_rows_init(PolishRowsInitArgs<T
             a.rp[i] = rv; a.pp[i] = rv * inv; a.wp[i] = 0.0;
             a.wd[i] = on ? yv : 0.0;
             v[0][0] += rv * rv * inv;
-            v[1][0] += rv * rv;
+            const double ro = a.plain_orig ? (on ? res : 0.0) : rv;
+            v[1][0] += ro * ro;
         }
     }
     const int ops[2] = {RSUM, RSUM};
@@ -1246,6 +1248,7 @@ struct CgUpdateArgs {
     double* w2; double* r2; const double* p2; const double* s2;
     const double* alpha;    // [2][ld]
     const double* invdiag;  // Jacobi preconditioner
+    const double* dr;       // row scale: |r_p|^2 in original units (r / dr), or nullptr (scaled)
     const int* sel;
     const int* act;         // [2][ld]
     double* partial;        // [n_rb][3][ld]: r_p.D^-1 r_p, r_d.D^-1 r_d, |r_p|^2
@@ -1274,7 +1277,8 @@ __global__ void __launch_bounds__(BLOCK) k_cg_update(CgUpdateArgs a) {
                 const double n1 = a.r1[i] - al1 * a.s1[i];
                 a.r1[i] = n1;
                 v[0][0] += n1 * n1 * iv;
-                v[2][0] += n1 * n1;
+                const double u = a.dr ? n1 / a.dr[row] : n1;
+                v[2][0] += u * u;
             }
             if (on2) {
                 a.w2[i] = a.w2[i] + al2 * a.p2[i];

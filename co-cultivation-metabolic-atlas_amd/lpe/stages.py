@@ -26,6 +26,7 @@ reference aborts that environment's step, src/community_solver.py:1747-1749).
 
 from __future__ import annotations
 
+import dataclasses
 import time
 from dataclasses import dataclass
 from typing import Optional
@@ -53,6 +54,8 @@ class StageOptions:
     # face of stage 1) do not reach the polish threshold from FP32 iterates;
     # they run in FP64
     later_precision: str = "fp64"
+    # iteration budget of stages 2 and 3 (None: the solve options' budget)
+    later_max_iterations: Optional[int] = None
 
 
 def exchange_terms(model: Consortium):
@@ -148,11 +151,14 @@ class CooperativeStages:
         self.term_cols = torch.as_tensor([c for _, c, _ in self.terms], device=self.device)
         self.log: list[dict] = []
 
-    def _solve(self, eng):
+    def _solve(self, eng, later: bool = False):
+        opts = self.options
+        if later and self.stages.later_max_iterations is not None:
+            opts = dataclasses.replace(opts, max_iterations=int(self.stages.later_max_iterations))
         if self.fallback and eng.precision == "fp32":
-            info, x, y, status = eng.solve_with_fallback(self.options)
+            info, x, y, status = eng.solve_with_fallback(opts)
         else:
-            info = eng.solve(self.options)
+            info = eng.solve(opts)
             x, y, status = eng.state()
         return info, x, y, status
 
@@ -187,7 +193,7 @@ class CooperativeStages:
             e2 = self.joint2
             e2.update(c2, rhs, lo2, up2, var_values=coupling)
             e2.reset(x1, y1)
-            info2, x2, y2, s2 = self._solve(e2)
+            info2, x2, y2, s2 = self._solve(e2, later=True)
             ok &= torch.as_tensor(s2 == _lib.STATUS_OPTIMAL, device=dev)
             x, y = x2, y2
             out["stage_info"]["aggregate_objective"] = info2
@@ -218,7 +224,7 @@ class CooperativeStages:
             e3 = self.stage3
             e3.update(self.c3, rhs3, lo3.contiguous(), up3.contiguous(), var_values=var3)
             e3.reset(x3.contiguous(), y3.contiguous())
-            info3, x3, y3, s3 = self._solve(e3)
+            info3, x3, y3, s3 = self._solve(e3, later=True)
             ok &= torch.as_tensor(s3 == _lib.STATUS_OPTIMAL, device=dev)
             chosen = x3[:, :n]
             out["stage_info"]["parsimonious_exchange"] = info3

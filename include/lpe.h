@@ -124,6 +124,7 @@ typedef struct lpe_solve_info {
     double check_ms;          /* device time of checks                       */
     double polish_ms;         /* device time of polish + certificates        */
     int64_t pdhg_kernel_launches;
+    int64_t cg_steps;         /* preconditioned-CG steps of the polish rounds */
 } lpe_solve_info;
 
 /* Replaces GpuPdhgCorrector.__init__ / GpuPdhgWorkspace.__init__
@@ -212,12 +213,14 @@ int lpe_set_launch(lpe_handle *h, int32_t rows_per_wave, int32_t lps_per_lane);
 int lpe_time_iterations(lpe_handle *h, int64_t iterations, int32_t chunk, int32_t use_graph,
                         double *ms, void *stream);
 
-/* Average duration (ms) of `reps` back-to-back launches of one PDHG half
- * step on the engine's stream, with hipEvents: kind 0 = dual step (the
- * fused A x SpMV + dual update), 1 = primal step (fused A^T y SpMV + primal
- * update + extrapolation + averaging).  The iterate is advanced; reset
- * before solving again.  No reference counterpart (the reference times
- * whole graph chunks, scripts/benchmark_pdhg_graph_chunk.py:217-234). */
+/* Average duration (ms) of `reps` back-to-back launches of one kernel on
+ * the engine's stream, with hipEvents: kind 0 = dual step (the fused A x
+ * SpMV + dual update), 1 = primal step (fused A^T y SpMV + primal update +
+ * extrapolation + averaging), 2 = the polish's preconditioned-CG update
+ * (both systems, every LP of the batch active; its scratch vectors are
+ * overwritten).  The iterate is advanced; reset before solving again.  No
+ * reference counterpart (the reference times whole graph chunks,
+ * scripts/benchmark_pdhg_graph_chunk.py:217-234). */
 int lpe_time_kernel(lpe_handle *h, int32_t kind, int32_t reps, double *ms, void *stream);
 
 int lpe_destroy(lpe_handle *h);
