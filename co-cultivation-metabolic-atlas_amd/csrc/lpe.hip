@@ -89,6 +89,7 @@ struct lpe_handle {
     int precision = LPE_PREC_FP64, method = LPE_METHOD_RESTARTED;
     double eta = 0.99, theta = 1.0, pw = 1.0;
     int ruiz = 10, shared_c = 0;
+    int halpern = 0;          // restarted method with reflected Halpern iterations
     int dev = 0;
 
     // host structure
@@ -311,9 +312,41 @@ Iter<double>& iter_of<double>(lpe_handle* h) { return h->it64; }
 template <>
 Iter<float>& iter_of<float>(lpe_handle* h) { return h->it32; }
 
+#ifndef LPE_HALPERN_REFLECTION
+#define LPE_HALPERN_REFLECTION 1.0
+#endif
+
 template <typename T, int LPL, bool REF>
-void launch_step_lpl(lpe_handle* h, int which) {
+void launch_step_lpl(lpe_handle* h, int which, int j, int store_t) {
     Iter<T>& it = iter_of<T>(h);
+    if (!REF && h->halpern) {
+        // primal-first reflected Halpern PDHG: x+ and xbar from (x, y), then y+
+        // from xbar; both blocks move towards the reflected point and the anchor
+        if (which & 2) {
+            int grid;
+            PrimalArgs<T> p;
+            p.sh = make_shape(h->n, h->ld, h->B, h->rpw, LPL, &grid);
+            p.AT = op_iter<T>(h, true);
+            p.y = it.y; p.x = it.x; p.xbar = it.xbar; p.c = it.c; p.lower = it.lo; p.upper = it.up;
+            p.tau = it.tau;
+            if constexpr (std::is_same<T, double>::value) p.inv_omega = h->d_iom; else p.inv_omega = h->d_iom_f;
+            p.flags = h->d_flags; p.xsum = it.xsum; p.theta = (T)h->theta; p.c_shared = h->shared_c;
+            p.anchor = it.xlast; p.hk = h->d_cnt; p.j = j; p.store_t = store_t; p.refl = (T)LPE_HALPERN_REFLECTION;
+            hipLaunchKernelGGL((k_primal_step<T, LPL, false, true>), dim3(grid), dim3(BLOCK), 0, h->s, p);
+        }
+        if (h->m > 0 && (which & 1)) {
+            int grid;
+            DualArgs<T> d;
+            d.sh = make_shape(h->m, h->ld, h->B, h->rpw, LPL, &grid);
+            d.A = op_iter<T>(h, false);
+            d.xbar = it.xbar; d.y = it.y; d.rhs = it.rhs; d.sigma = it.sigma;
+            if constexpr (std::is_same<T, double>::value) d.omega = h->d_om; else d.omega = h->d_om_f;
+            d.flags = h->d_flags; d.ysum = it.ysum; d.neq = h->neq;
+            d.anchor = it.ylast; d.hk = h->d_cnt; d.j = j; d.store_t = store_t; d.refl = (T)LPE_HALPERN_REFLECTION;
+            hipLaunchKernelGGL((k_dual_step<T, LPL, false, true>), dim3(grid), dim3(BLOCK), 0, h->s, d);
+        }
+        return;
+    }
     if (h->m > 0 && (which & 1)) {
         int grid;
         DualArgs<T> d;
@@ -339,19 +372,30 @@ void launch_step_lpl(lpe_handle* h, int which) {
 }
 
 template <typename T, bool REF>
-void launch_step_t(lpe_handle* h, int which) {
+void launch_step_t(lpe_handle* h, int which, int j, int store_t) {
     switch (h->lpl) {
-        case 4: launch_step_lpl<T, 4, REF>(h, which); break;
-        case 2: launch_step_lpl<T, 2, REF>(h, which); break;
-        default: launch_step_lpl<T, 1, REF>(h, which); break;
+        case 4: launch_step_lpl<T, 4, REF>(h, which, j, store_t); break;
+        case 2: launch_step_lpl<T, 2, REF>(h, which, j, store_t); break;
+        default: launch_step_lpl<T, 1, REF>(h, which, j, store_t); break;
     }
 }
 
-// which: 1 = dual half step, 2 = primal half step, 3 = full iteration
-void launch_step(lpe_handle* h, int which = 3) {
-    if (h->method == LPE_METHOD_FIXED) launch_step_t<double, true>(h, which);
-    else if (h->precision == LPE_PREC_FP32) launch_step_t<float, false>(h, which);
-    else launch_step_t<double, false>(h, which);
+// which: 1 = dual half step, 2 = primal half step, 3 = full iteration.
+// Halpern: j = iteration within the launch group (the kernels read the
+// per-LP iteration count cnt + j), store_t = also keep T(z) for the check.
+void launch_step(lpe_handle* h, int which = 3, int j = 0, int store_t = 0) {
+    if (h->method == LPE_METHOD_FIXED) launch_step_t<double, true>(h, which, j, store_t);
+    else if (h->precision == LPE_PREC_FP32) launch_step_t<float, false>(h, which, j, store_t);
+    else launch_step_t<double, false>(h, which, j, store_t);
+}
+
+// A group of `count` iterations (Halpern: indices 0..count-1, T(z) kept at the
+// last, then the per-LP count advances).
+void launch_group(lpe_handle* h, int count) {
+    for (int i = 0; i < count; ++i) launch_step(h, 3, i, h->halpern && i == count - 1);
+    if (h->halpern && count > 0)
+        hipLaunchKernelGGL(k_hal_advance, dim3(ceil_div(h->B, 256)), dim3(256), 0, h->s, h->d_cnt,
+                           (const int*)h->d_flags, count, h->B);
 }
 
 int kernel_launches_per_iteration(lpe_handle* h) { return h->m > 0 ? 2 : 1; }
@@ -366,7 +410,7 @@ int run_iterations(lpe_handle* h, long iters, int chunk, int use_graph) {
             if (h->graph) { hipGraphExecDestroy(h->graph); h->graph = nullptr; }
             hipGraph_t g;
             HIPCHK(hipStreamBeginCapture(h->s, hipStreamCaptureModeThreadLocal));
-            for (int i = 0; i < chunk; ++i) launch_step(h);
+            launch_group(h, chunk);
             HIPCHK(hipStreamEndCapture(h->s, &g));
             hipError_t e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
             hipGraphDestroy(g);
@@ -375,7 +419,7 @@ int run_iterations(lpe_handle* h, long iters, int chunk, int use_graph) {
         }
         for (long c = 0; c < chunks; ++c) HIPCHK(hipGraphLaunch(h->graph, h->s));
     }
-    for (long i = 0; i < tail; ++i) launch_step(h);
+    if (tail > 0) launch_group(h, (int)tail);
     HIPCHK(hipGetLastError());
     return LPE_OK;
 }
@@ -433,7 +477,7 @@ int launch_check_t(lpe_handle* h, int K, int certify, double ep, double ed, doub
         r.x = it.x; r.xsum = restarted ? it.xsum : it.x; r.y = it.y; r.ysum = restarted ? it.ysum : it.y;
         r.ylast = restarted ? it.ylast : it.y;
         r.rhs = h->d_rhs_s; r.dr = h->d_dr; r.flags = h->d_flags; r.cnt = h->d_cnt; r.K = K; r.neq = h->neq;
-        r.with_avg = restarted ? 1 : 0; r.partial = h->d_partial;
+        r.with_avg = restarted ? 1 : 0; r.hal = h->halpern; r.partial = h->d_partial;
         hipLaunchKernelGGL((k_check_rows<T>), dim3(grid_r), dim3(BLOCK), 0, h->s, r);
         reduce(h, h->d_partial, h->d_red_a, h->d_ops_cr, 2 * CR_N, sr.n_rb);
     } else {
@@ -444,7 +488,7 @@ int launch_check_t(lpe_handle* h, int K, int certify, double ep, double ed, doub
     c.x = it.x; c.xsum = restarted ? it.xsum : it.x; c.xlast = restarted ? it.xlast : it.x;
     c.y = it.y; c.ysum = restarted ? it.ysum : it.y;
     c.c = h->d_c_s; c.lower = h->d_lo_s; c.upper = h->d_up_s; c.dc = h->d_dc; c.flags = h->d_flags;
-    c.cnt = h->d_cnt; c.K = K; c.c_shared = h->shared_c; c.with_avg = restarted ? 1 : 0;
+    c.cnt = h->d_cnt; c.K = K; c.c_shared = h->shared_c; c.with_avg = restarted ? 1 : 0; c.hal = h->halpern;
     c.partial = h->d_partial;
     hipLaunchKernelGGL((k_check_cols<T>), dim3(grid_c), dim3(BLOCK), 0, h->s, c);
     reduce(h, h->d_partial, h->d_red_b, h->d_ops_cc, 2 * CC_N, sc.n_rb);
@@ -452,7 +496,7 @@ int launch_check_t(lpe_handle* h, int K, int certify, double ep, double ed, doub
     d.rows = h->d_red_a; d.cols = h->d_red_b; d.st = h->d_st; d.flags = h->d_flags; d.cnt = h->d_cnt;
     d.omega64 = h->d_om; d.omega32 = h->d_om_f; d.inv_omega64 = h->d_iom; d.inv_omega32 = h->d_iom_f;
     d.bnorm = h->d_bnorm; d.cnorm = h->d_cnorm; d.batch = h->B; d.ld = h->ld; d.K = K;
-    d.restarted = restarted ? 1 : 0; d.certify = certify; d.eps_p = ep; d.eps_d = ed; d.eps_g = eg;
+    d.restarted = restarted ? 1 : 0; d.hal = h->halpern; d.certify = certify; d.eps_p = ep; d.eps_d = ed; d.eps_g = eg;
     hipLaunchKernelGGL(k_check_decide, dim3(ceil_div(h->B, 256)), dim3(256), 0, h->s, d);
     if (h->method == LPE_METHOD_RESTARTED) {
         ApplyArgs<T> ax;
@@ -1197,13 +1241,14 @@ int lpe_create(const lpe_desc* d, lpe_handle** out) {
     if (d->m < 0 || d->n < 1 || d->batch < 1 || d->neq < 0 || d->neq > d->m || d->nnz < 0)
         return fail(LPE_EINVAL, "invalid dimensions (need n >= 1, batch >= 1, 0 <= neq <= m)");
     if (d->precision != LPE_PREC_FP64 && d->precision != LPE_PREC_FP32) return fail(LPE_EINVAL, "invalid precision");
-    if (d->method != LPE_METHOD_FIXED && d->method != LPE_METHOD_RESTARTED) return fail(LPE_EINVAL, "invalid method");
+    if (d->method != LPE_METHOD_FIXED && d->method != LPE_METHOD_RESTARTED && d->method != LPE_METHOD_HALPERN)
+        return fail(LPE_EINVAL, "invalid method");
     if (d->method == LPE_METHOD_FIXED && d->precision != LPE_PREC_FP64)
         return fail(LPE_EINVAL, "the fixed-metric method runs in FP64");
     if (!(d->step_safety > 0.0 && d->step_safety < 1.0)) return fail(LPE_EINVAL, "step_safety must be strictly between zero and one");
     if (!(d->theta >= 0.0 && d->theta <= 1.0)) return fail(LPE_EINVAL, "theta must be finite and between zero and one");
     if (!std::isfinite(d->primal_weight) || d->primal_weight < 0.0 ||
-        (d->primal_weight == 0.0 && d->method != LPE_METHOD_RESTARTED))
+        (d->primal_weight == 0.0 && d->method == LPE_METHOD_FIXED))
         return fail(LPE_EINVAL, "primal_weight must be finite and strictly positive");
     const long m = d->m, n = d->n, nnz = d->nnz;
     if ((long)d->batch * std::max(m, n) >= (1L << 31) - 1 || nnz >= (1L << 31) - 1)
@@ -1224,7 +1269,9 @@ int lpe_create(const lpe_desc* d, lpe_handle** out) {
     lpe_handle* h = new lpe_handle();
     h->m = d->m; h->n = d->n; h->neq = d->neq; h->B = d->batch; h->nnz = nnz; h->nvar = d->n_var;
     h->ld = ((d->batch + 255) / 256) * 256;
-    h->precision = d->precision; h->method = d->method; h->eta = d->step_safety; h->theta = d->theta;
+    // Halpern shares the restarted method's scaling, weights, checks and polish
+    h->precision = d->precision; h->method = d->method == LPE_METHOD_HALPERN ? LPE_METHOD_RESTARTED : d->method;
+    h->halpern = d->method == LPE_METHOD_HALPERN ? 1 : 0; h->eta = d->step_safety; h->theta = d->theta;
     h->pw = d->primal_weight; h->ruiz = std::max(0, d->ruiz_iterations); h->shared_c = d->shared_c ? 1 : 0;
     hipGetDevice(&h->dev);
     h->rp.assign(d->row_ptr, d->row_ptr + m + 1);

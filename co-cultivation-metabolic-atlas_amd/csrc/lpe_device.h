@@ -150,13 +150,22 @@ struct DualArgs {
     const T* sigma;      // [m] shared, or [m][ld] when REF
     const T* omega;      // [ld] primal weight (RESTARTED); unused when REF
     const int* flags;    // [ld] nonzero = frozen
-    T* ysum;             // [m][ld] running sum (RESTARTED)
+    T* ysum;             // [m][ld] running sum (RESTARTED); T(z)_y when HAL and store_t
     int neq;
+    // Halpern (HAL): y <- lam ((1 + refl) y+ - refl y) + (1 - lam) y0,
+    // lam = (k + 1) / (k + 2), k = hk[b] + j
+    const T* anchor;     // [m][ld] y0 (the last restart point)
+    const int* hk;       // [ld] Halpern iterations since the restart, before this launch group
+    int j;               // iteration index within the launch group
+    int store_t;         // 1: also store y+ = T(z)_y into ysum
+    T refl;
 };
 
 // y+ = y + sigma*(rhs - A xbar), inequality rows clipped to <= 0, NaN passes,
 // frozen LPs keep y (src/gpu_pdhg_workspace.py:122-127; corrector :340-344).
-template <typename T, int LPL, bool REF>
+// HAL: the reflected Halpern step of restarted Halpern PDHG (Lu & Yang,
+// "Restarted Halpern PDHG for linear programming", 2024) on the dual block.
+template <typename T, int LPL, bool REF, bool HAL = false>
 __global__ void __launch_bounds__(BLOCK) k_dual_step(DualArgs<T> a) {
     int tile, rb;
     if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
@@ -165,11 +174,15 @@ __global__ void __launch_bounds__(BLOCK) k_dual_step(DualArgs<T> a) {
     if (b0 >= a.sh.ld) return;
     const long ld = a.sh.ld;
     int fr[LPL];
-    T om[LPL];
+    T om[LPL], lam[LPL];
 #pragma unroll
     for (int t = 0; t < LPL; ++t) {
         fr[t] = a.flags[b0 + t];
         om[t] = REF ? T(1) : a.omega[b0 + t];
+        if (HAL) {
+            const int k = a.hk[b0 + t] + a.j;
+            lam[t] = T(k + 1) / T(k + 2);
+        }
     }
     bool all_frozen = true;
 #pragma unroll
@@ -201,6 +214,18 @@ __global__ void __launch_bounds__(BLOCK) k_dual_step(DualArgs<T> a) {
             if (ineq && !__builtin_isnan(cand)) nv = __builtin_fmin(cand, T(0));
             out[t] = fr[t] ? yv[t] : nv;
         }
+        if (HAL) {
+            T y0[LPL], hv[LPL];
+            load_vec<T, LPL>(a.anchor + row * ld + b0, y0);
+#pragma unroll
+            for (int t = 0; t < LPL; ++t) {
+                const T refl_pt = (T(1) + a.refl) * out[t] - a.refl * yv[t];
+                hv[t] = fr[t] ? yv[t] : lam[t] * refl_pt + (T(1) - lam[t]) * y0[t];
+            }
+            store_vec<T, LPL>(a.y + row * ld + b0, hv);
+            if (a.store_t) store_vec<T, LPL>(a.ysum + row * ld + b0, out);
+            continue;
+        }
         store_vec<T, LPL>(a.y + row * ld + b0, out);
         if (!REF) {
             T s[LPL];
@@ -225,14 +250,21 @@ struct PrimalArgs {
     const T* tau;        // [n] shared, or [n][ld] when REF
     const T* inv_omega;  // [ld]
     const int* flags;
-    T* xsum;             // [n][ld]
+    T* xsum;             // [n][ld]; T(z)_x when HAL and store_t
     T theta;
     int c_shared;
+    // Halpern (HAL): x <- lam ((1 + refl) x+ - refl x) + (1 - lam) x0; xbar
+    // still extrapolates x+ for the dual step of the same iteration
+    const T* anchor;     // [n][ld] x0
+    const int* hk;       // [ld]
+    int j;
+    int store_t;
+    T refl;
 };
 
 // x+ = clip(x - tau*(c - A^T y+), l, u); xbar = x+ + theta*(x+ - x); frozen
 // LPs keep x (src/gpu_pdhg_workspace.py:128-134; corrector :345-357).
-template <typename T, int LPL, bool REF>
+template <typename T, int LPL, bool REF, bool HAL = false>
 __global__ void __launch_bounds__(BLOCK) k_primal_step(PrimalArgs<T> a) {
     int tile, rb;
     if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
@@ -241,11 +273,15 @@ __global__ void __launch_bounds__(BLOCK) k_primal_step(PrimalArgs<T> a) {
     if (b0 >= a.sh.ld) return;
     const long ld = a.sh.ld;
     int fr[LPL];
-    T iw[LPL];
+    T iw[LPL], lam[LPL];
 #pragma unroll
     for (int t = 0; t < LPL; ++t) {
         fr[t] = a.flags[b0 + t];
         iw[t] = REF ? T(1) : a.inv_omega[b0 + t];
+        if (HAL) {
+            const int k = a.hk[b0 + t] + a.j;
+            lam[t] = T(k + 1) / T(k + 2);
+        }
     }
     bool all_frozen = true;
 #pragma unroll
@@ -291,6 +327,19 @@ __global__ void __launch_bounds__(BLOCK) k_primal_step(PrimalArgs<T> a) {
             const T ext = a.theta * (nxt - xv[t]);
             nx[t] = nxt;
             nb[t] = nxt + ext;
+        }
+        if (HAL) {
+            T x0[LPL], hv[LPL];
+            load_vec<T, LPL>(a.anchor + col * ld + b0, x0);
+#pragma unroll
+            for (int t = 0; t < LPL; ++t) {
+                const T refl_pt = (T(1) + a.refl) * nx[t] - a.refl * xv[t];
+                hv[t] = fr[t] ? xv[t] : lam[t] * refl_pt + (T(1) - lam[t]) * x0[t];
+            }
+            store_vec<T, LPL>(a.x + col * ld + b0, hv);
+            store_vec<T, LPL>(a.xbar + col * ld + b0, nb);
+            if (a.store_t) store_vec<T, LPL>(a.xsum + col * ld + b0, nx);
+            continue;
         }
         store_vec<T, LPL>(a.x + col * ld + b0, nx);
         store_vec<T, LPL>(a.xbar + col * ld + b0, nb);
@@ -441,6 +490,7 @@ struct CheckRowsArgs {
     int K;                   // iterations in this chunk
     int neq;
     int with_avg;
+    int hal;                 // 1: the second candidate (xsum, ysum) is T(z), not a sum
     double* partial;         // [n_rb][2*CR_N][ld]
 };
 
@@ -456,7 +506,7 @@ __global__ void __launch_bounds__(BLOCK) k_check_rows(CheckRowsArgs<T> a) {
     double v[2 * CR_N][1];
     for (int i = 0; i < 2 * CR_N; ++i) v[i][0] = 0.0;
     const bool active = valid && !a.flags[valid ? b0 : 0];
-    const double inv = active ? 1.0 / (double)(a.cnt[b0] + a.K) : 0.0;
+    const double inv = active ? (a.hal ? 1.0 : 1.0 / (double)(a.cnt[b0] + a.K)) : 0.0;
     if (active) {
         const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
         for (int r = 0; r < a.sh.rpw; ++r) {
@@ -519,6 +569,7 @@ struct CheckColsArgs {
     int K;
     int c_shared;
     int with_avg;
+    int hal;
     double* partial;         // [n_rb][2*CC_N][ld]
 };
 
@@ -534,7 +585,7 @@ __global__ void __launch_bounds__(BLOCK) k_check_cols(CheckColsArgs<T> a) {
     double v[2 * CC_N][1];
     for (int i = 0; i < 2 * CC_N; ++i) v[i][0] = 0.0;
     const bool active = valid && !a.flags[valid ? b0 : 0];
-    const double inv = active ? 1.0 / (double)(a.cnt[b0] + a.K) : 0.0;
+    const double inv = active ? (a.hal ? 1.0 : 1.0 / (double)(a.cnt[b0] + a.K)) : 0.0;
     if (active) {
         const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
         for (int r = 0; r < a.sh.rpw; ++r) {
@@ -637,6 +688,7 @@ struct DecideArgs {
     int ld;
     int K;
     int restarted;
+    int hal;                // Halpern: cnt already counts this chunk, candidate 2 is T(z)
     int certify;
     double eps_p, eps_d, eps_g;
 };
@@ -680,7 +732,8 @@ __global__ void k_check_decide(DecideArgs a) {
     const double sig = a.cols[CC_ASIG * ld + b] + a.rows[CR_ASIG * ld + b];
     s.asig_same = (a.K > 0 && sig == s.asig) ? s.asig_same + 1 : 0;
     s.asig = sig;
-    s.avg_div = (double)(a.cnt[b] + a.K);
+    const int since = a.hal ? a.cnt[b] : a.cnt[b] + a.K;   // iterations since the last restart
+    s.avg_div = a.hal ? 1.0 : (double)since;
     if (a.certify && (pass[0] || pass[1])) {
         s.status = 1;
         s.action = pass[0] ? ACT_ACCEPT_CUR : ACT_ACCEPT_AVG;
@@ -693,7 +746,7 @@ __global__ void k_check_decide(DecideArgs a) {
             s.kkt_last = kc;
         } else {
             restart = kc <= 0.2 * s.kkt_last || (kc <= 0.8 * s.kkt_last && kc > s.kkt_prev) ||
-                      (double)(a.cnt[b] + a.K) >= 0.36 * (double)s.iters;
+                      (double)since >= 0.36 * (double)s.iters;
         }
         s.kkt_prev = kc;
         if (restart) {
@@ -709,7 +762,7 @@ __global__ void k_check_decide(DecideArgs a) {
             s.kkt_prev = INFINITY;
             s.restarts += 1;
             a.cnt[b] = 0;
-        } else {
+        } else if (!a.hal) {
             a.cnt[b] += a.K;
         }
         a.omega64[b] = om;
@@ -1669,6 +1722,13 @@ __global__ void k_init_omega(LpState* st, double* om, float* om_f, double* iom, 
     }
     st[b].omega = w;
     om[b] = w; om_f[b] = (float)w; iom[b] = 1.0 / w; iom_f[b] = (float)(1.0 / w);
+}
+
+// Halpern: the iterations since the last restart advance by the launch
+// group's length (the kernels of the group read cnt + j).
+__global__ void k_hal_advance(int* cnt, const int* flags, int count, int batch) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < batch && !flags[b]) cnt[b] += count;
 }
 
 // Mark polished LPs optimal and frozen.

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "liblpe.so")
 
 LPE_OK, LPE_EINVAL, LPE_EDEVICE, LPE_ESTATE = 0, -1, -2, -3
 PREC_FP64, PREC_FP32 = 0, 1
-METHOD_FIXED, METHOD_RESTARTED = 0, 1
+METHOD_FIXED, METHOD_RESTARTED, METHOD_HALPERN = 0, 1, 2
 STATUS_RUNNING, STATUS_OPTIMAL, STATUS_LIMIT = 0, 1, 2
 STATUS_PRIMAL_INFEASIBLE, STATUS_DUAL_INFEASIBLE, STATUS_NUMERICAL = 3, 4, 5
 STATUS_NAMES = {0: "running", 1: "optimal", 2: "iteration_limit", 3: "primal_infeasible",

@@ -94,9 +94,11 @@ class BatchedLpEngine:
         ``"fp64"`` or ``"fp32"`` (iterate arithmetic of the PDHG bulk work;
         checks, polish and certificates are FP64 either way).
     method:
-        ``"restarted"`` (restarted-averaged PDHG on the equilibrated problem)
-        or ``"fixed"`` (the reference's fixed-metric PDHG,
-        src/gpu_pdhg_corrector.py:185-196 and :340-358).
+        ``"restarted"`` (restarted-averaged PDHG on the equilibrated problem),
+        ``"halpern"`` (restarted reflected Halpern PDHG on the equilibrated
+        problem, same checks, restarts and polish) or ``"fixed"`` (the
+        reference's fixed-metric PDHG, src/gpu_pdhg_corrector.py:185-196 and
+        :340-358).
     """
 
     def __init__(self, A, neq: int, batch: int, *, var_positions=None, precision: str = "fp64",
@@ -131,11 +133,12 @@ class BatchedLpEngine:
         d.n_var = self.n_var
         d.var_pos = self._vp.ctypes.data_as(C.POINTER(C.c_int64))
         d.precision = {"fp64": _lib.PREC_FP64, "fp32": _lib.PREC_FP32}[precision]
-        d.method = {"fixed": _lib.METHOD_FIXED, "restarted": _lib.METHOD_RESTARTED}[method]
+        d.method = {"fixed": _lib.METHOD_FIXED, "restarted": _lib.METHOD_RESTARTED,
+                    "halpern": _lib.METHOD_HALPERN}[method]
         d.step_safety = float(step_safety)
         d.theta = float(theta)
-        # restarted method: None -> ||c^||/||b^|| per LP (PDLP default); fixed: 1.0
-        d.primal_weight = float(primal_weight) if primal_weight is not None else (0.0 if method == "restarted" else 1.0)
+        # restarted / Halpern: None -> ||c^||/||b^|| per LP (PDLP default); fixed: 1.0
+        d.primal_weight = float(primal_weight) if primal_weight is not None else (0.0 if method != "fixed" else 1.0)
         d.ruiz_iterations = int(ruiz_iterations)
         d.shared_c = 1 if shared_c else 0
         handle = C.c_void_p()
@@ -218,7 +221,8 @@ class BatchedLpEngine:
             self._fallback = None
         if getattr(self, "_fallback", None) is None:
             self._fallback = BatchedLpEngine(self.A, self.neq, capacity, var_positions=self.var_positions,
-                                             precision="fp64", shared_c=self.shared_c, device=self.device)
+                                             precision="fp64", shared_c=self.shared_c, device=self.device,
+                                             method="restarted" if self.method == "fixed" else self.method)
         fb = self._fallback
         omega = self.primal_weight()
         c_t, rhs_t, lo_t, up_t, vv_t = self._vectors
@@ -229,7 +233,7 @@ class BatchedLpEngine:
             fb.update(c_t if self.shared_c else c_t[idx], rhs_t[idx], lo_t[idx], up_t[idx],
                       var_values=None if vv_t is None else vv_t[idx])
             fb.reset(x[idx], y[idx])
-            fb.set_primal_weight(omega[pad]) if self.method == "restarted" else None
+            fb.set_primal_weight(omega[pad]) if self.method != "fixed" else None
             fb_info = fb.solve(fb_opts)
             fx, fy, fs = fb.state()
             k = part.size

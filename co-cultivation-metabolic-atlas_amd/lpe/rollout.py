@@ -38,7 +38,8 @@ class ConsortiumRollout:
     def __init__(self, model: Optional[Consortium] = None, n_env: int = 4096, *, seed: int = 2,
                  precision: str = "fp32", options: Optional[SolveOptions] = None,
                  warm_start="previous", dt: float = 1.0, conc_range=(1e-2, 10.0),
-                 autotune: bool = True, fallback: bool = True, device: Optional[torch.device] = None):
+                 autotune: bool = True, fallback: bool = True, device: Optional[torch.device] = None,
+                 method: str = "restarted"):
         self.model = model if model is not None else consortium(seed)
         md = self.model
         self.n_env = int(n_env)
@@ -47,7 +48,7 @@ class ConsortiumRollout:
         self.options = options or SolveOptions()
         self.warm_start = warm_start
         self.engine = BatchedLpEngine(md.A, md.neq, self.n_env, var_positions=md.var_positions,
-                                      precision=precision, shared_c=True, device=self.device)
+                                      precision=precision, shared_c=True, device=self.device, method=method)
         rng = np.random.default_rng(seed + 7)
         lo, hi = conc_range
         dev = self.device

@@ -56,6 +56,8 @@ class StageOptions:
     later_precision: str = "fp64"
     # iteration budget of stages 2 and 3 (None: the solve options' budget)
     later_max_iterations: Optional[int] = None
+    # PDHG method of every stage engine ("restarted" or "halpern")
+    method: str = "restarted"
 
 
 def exchange_terms(model: Consortium):
@@ -132,19 +134,22 @@ class CooperativeStages:
         self.g = live_objective(model)
         self.terms = exchange_terms(model)
         self.n_aux = len(self.terms)
+        meth = self.stages.method
         self.joint = BatchedLpEngine(model.A, model.neq, self.n_env, var_positions=model.var_positions,
-                                     precision=precision, shared_c=True, device=self.device)
+                                     precision=precision, shared_c=True, device=self.device, method=meth)
         self.joint2 = self.joint
         if self.stages.optimize_live_objectives and self.stages.later_precision != precision:
             self.joint2 = BatchedLpEngine(model.A, model.neq, self.n_env, var_positions=model.var_positions,
-                                          precision=self.stages.later_precision, shared_c=True, device=self.device)
+                                          precision=self.stages.later_precision, shared_c=True, device=self.device,
+                                          method=meth)
         self.stage3 = None
         if self.stages.parsimonious_exchange and self.n_aux:
             A3, neq3, coupling3, transfer3, tr0 = stage3_pattern(model, self.stages.optimize_live_objectives)
             self.A3, self.tr0 = A3, tr0
             self.var3 = np.concatenate([coupling3.ravel(), transfer3.ravel()])
             self.stage3 = BatchedLpEngine(A3, neq3, self.n_env, var_positions=self.var3,
-                                          precision=self.stages.later_precision, shared_c=True, device=self.device)
+                                          precision=self.stages.later_precision, shared_c=True, device=self.device,
+                                          method=meth)
             self.c3 = np.zeros(A3.shape[1])
             self.c3[n:] = 1.0
         self.term_species = torch.as_tensor([s for s, _, _ in self.terms], device=self.device)

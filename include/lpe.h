@@ -53,6 +53,8 @@ extern "C" {
 /* algorithm families */
 #define LPE_METHOD_FIXED 0      /* fixed-metric PDHG, reference semantics  */
 #define LPE_METHOD_RESTARTED 1  /* restarted-averaged PDHG (Ruiz scaling)  */
+#define LPE_METHOD_HALPERN 2    /* restarted reflected Halpern PDHG (Ruiz
+                                 * scaling, same checks and restarts)       */
 
 typedef struct lpe_handle lpe_handle;
 

@@ -141,8 +141,17 @@ def _kkt_values(As, bs, cs, ls, us, ineq, x, y, dr, dc, neq):
 
 def restarted_pdhg(a, rhs, lower, upper, c, neq, *, iterations, check_interval=64, precision="fp64",
                    x0=None, y0=None, ruiz_iterations=10, eta=0.99, theta=1.0, omega0=None,
-                   certify=True, gates=GATES):
-    """Restarted-averaged PDHG for one LP, decisions as in the engine's k_check_decide."""
+                   certify=True, gates=GATES, halpern=False, reflection=1.0):
+    """Restarted-averaged PDHG for one LP, decisions as in the engine's k_check_decide.
+
+    ``halpern=True``: restarted reflected Halpern PDHG (Lu & Yang, "Restarted
+    Halpern PDHG for linear programming", 2024) with the same checks: each
+    iteration applies the primal-first PDHG operator T to z = (x, y) and moves
+    to lam ((1 + r) T(z) - r z) + (1 - lam) z0, lam = (k + 1) / (k + 2), k the
+    iterations since the last restart and z0 the restart point; at a check the
+    candidates are z and T(z) of the chunk's last iteration (instead of the
+    current iterate and the average), and a restart moves z and z0 to the
+    chosen one."""
     a = csr_matrix(a, dtype=np.float64)
     a.sort_indices()
     m, n = a.shape
@@ -186,10 +195,26 @@ def restarted_pdhg(a, rhs, lower, upper, c, neq, *, iterations, check_interval=6
     cur = evaluate(x, y)
     if certify and passes(cur):
         status, accepted_at = "optimal", 0
+    refl = T(reflection)
     while status == "running" and iters < iterations:
         k = min(check_interval, iterations - iters)
         om = T(omega); iom = T(1.0 / omega)
         for _ in range(k):
+            if halpern:
+                lam = T(cnt + 1) / T(cnt + 2)
+                aty = csr_rmatvec_ordered(AsT, y).astype(T) if m else np.zeros(n, T)
+                xp, xbar = _primal_step(x, aty, csT, tauT * iom, lsT, usT, T(theta))
+                xp, xbar = xp.astype(T), xbar.astype(T)
+                xn = (lam * ((T(1) + refl) * xp - refl * x) + (T(1) - lam) * xlast).astype(T)
+                yp = y
+                if m:
+                    ax = csr_matvec_ordered(AsT, xbar).astype(T)
+                    yp = _dual_step(y, bsT, ax, sigT * om, ineq).astype(T)
+                    y = (lam * ((T(1) + refl) * yp - refl * y) + (T(1) - lam) * ylast).astype(T)
+                x = xn
+                xsum, ysum = xp, yp          # T(z) of the latest iteration
+                cnt += 1
+                continue
             if m:
                 ax = csr_matvec_ordered(AsT, xbar).astype(T)
                 y = _dual_step(y, bsT, ax, sigT * om, ineq).astype(T)
@@ -199,8 +224,11 @@ def restarted_pdhg(a, rhs, lower, upper, c, neq, *, iterations, check_interval=6
             x, xbar = x.astype(T), xbar.astype(T)
             xsum = (xsum + x).astype(T)
         iters += k
-        div = cnt + k
-        xa, ya = (xsum.astype(np.float64) / div), (ysum.astype(np.float64) / div)
+        div = cnt if halpern else cnt + k
+        if halpern:
+            xa, ya = xsum.astype(np.float64), ysum.astype(np.float64)
+        else:
+            xa, ya = (xsum.astype(np.float64) / div), (ysum.astype(np.float64) / div)
         vc, va = evaluate(x, y), evaluate(xa, ya)
         kc, ka = kkt_w(vc), kkt_w(va)
         best = 1 if ka < kc else 0
@@ -226,10 +254,10 @@ def restarted_pdhg(a, rhs, lower, upper, c, neq, *, iterations, check_interval=6
                 omega = min(max(omega, 1e-8), 1e8)
             x, y = cx.copy(), cy.copy()
             xbar, xlast, ylast = x.copy(), x.copy(), y.copy()
-            xsum[:] = 0; ysum[:] = 0
+            xsum = np.zeros(n, T); ysum = np.zeros(m, T)
             kkt_last, kkt_prev, cnt = kcand, np.inf, 0
             restarts += 1
-        else:
+        elif not halpern:
             cnt += k
     xo = x.astype(np.float64) * dc
     yo = y.astype(np.float64) * dr

@@ -30,10 +30,11 @@ def _load_uptake(eng, ub):
     eng.update(ub.c, np.tile(ub.rhs, (ub.batch, 1)), ub.lower, ub.upper)
 
 
+@pytest.mark.parametrize("method", ["restarted", "halpern"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_restarted_iterates_match_oracle_before_restarts(device, precision):
+def test_restarted_iterates_match_oracle_before_restarts(device, precision, method):
     ub = _uptake(batch=3)
-    eng = _engine(ub.S, ub.neq, ub.batch, precision=precision, shared_c=True)
+    eng = _engine(ub.S, ub.neq, ub.batch, precision=precision, shared_c=True, method=method)
     _load_uptake(eng, ub)
     eng.reset()
     eng.run(50)
@@ -41,14 +42,15 @@ def test_restarted_iterates_match_oracle_before_restarts(device, precision):
     tol = 1e-12 if precision == "fp64" else 1e-5
     for b in range(ub.batch):
         r = opdhg.restarted_pdhg(*ub.problem(b), iterations=50, check_interval=50, precision=precision,
-                                 certify=False)
+                                 certify=False, halpern=method == "halpern")
         np.testing.assert_allclose(x.cpu().numpy()[b], r["x"], rtol=tol, atol=tol * 1e3)
         np.testing.assert_allclose(y.cpu().numpy()[b], r["y"], rtol=tol, atol=tol * 1e3)
 
 
-def test_restarted_fp64_bit_exact_single_chunk(device):
+@pytest.mark.parametrize("method", ["restarted", "halpern"])
+def test_restarted_fp64_bit_exact_single_chunk(device, method):
     ub = _uptake(batch=2)
-    eng = _engine(ub.S, ub.neq, ub.batch, precision="fp64", shared_c=True)
+    eng = _engine(ub.S, ub.neq, ub.batch, precision="fp64", shared_c=True, method=method)
     _load_uptake(eng, ub)
     eng.reset()
     eng.run(30, chunk=10, use_graph=True)
@@ -57,15 +59,18 @@ def test_restarted_fp64_bit_exact_single_chunk(device):
     eng.run(30, use_graph=False)
     lx, ly, _ = eng.state()
     assert bool((gx == lx).all()) and bool((gy == ly).all())
-    r = opdhg.restarted_pdhg(*ub.problem(0), iterations=30, check_interval=30, certify=False)
+    r = opdhg.restarted_pdhg(*ub.problem(0), iterations=30, check_interval=30, certify=False,
+                             halpern=method == "halpern")
     np.testing.assert_array_equal(r["x_scaled"], r["x_scaled"])  # oracle determinism
     np.testing.assert_allclose(gx.cpu().numpy()[0], r["x"], rtol=1e-13, atol=1e-10)
 
 
-@pytest.mark.parametrize("precision,polish", [("fp64", False), ("fp64", True), ("fp32", True)])
-def test_solve_small_batch_against_highs(device, precision, polish):
+@pytest.mark.parametrize("precision,polish,method", [("fp64", False, "restarted"), ("fp64", True, "restarted"),
+                                                     ("fp32", True, "restarted"), ("fp64", False, "halpern"),
+                                                     ("fp64", True, "halpern"), ("fp32", True, "halpern")])
+def test_solve_small_batch_against_highs(device, precision, polish, method):
     ub = _uptake(batch=8)
-    eng = _engine(ub.S, ub.neq, ub.batch, precision=precision, shared_c=True)
+    eng = _engine(ub.S, ub.neq, ub.batch, precision=precision, shared_c=True, method=method)
     _load_uptake(eng, ub)
     eng.reset()
     info = eng.solve(max_iterations=40000, check_interval=64, polish=polish)

@@ -180,6 +180,30 @@ def test_restarted_pdhg_certifies_and_matches_highs():
     assert np.all(r["x"] >= lo - 1e-12) and np.all(r["x"] <= hi + 1e-12)
 
 
+def test_halpern_pdhg_certifies_and_matches_highs():
+    # restarted reflected Halpern PDHG (Lu & Yang 2024) with the engine's checks
+    for seed in (3, 4):
+        ub = workloads.uptake_batch(batch=1, seed=seed, m=120, n=180)
+        p = ub.problem(0)
+        ref = highs.solve(*p)
+        r = pdhg.restarted_pdhg(*p, iterations=20000, halpern=True)
+        assert r["status"] == "optimal" and r["certificate"]["certificate_passed"]
+        assert cert.relative_objective_error(r["certificate"]["objective"], ref.fun) <= 1e-6
+        a, rhs, lo, hi, c, neq = p
+        assert np.all(r["x"] >= lo - 1e-12) and np.all(r["x"] <= hi + 1e-12)
+
+
+def test_halpern_steps_known_answers():
+    # box-only LP (no rows): T(x) = clip(x - c, l, u) (unit metric and weight).
+    # r = 1: z1 = T(z0) = [-1, 2]; z2 = 2/3 (2 T(z1) - z1) + 1/3 z0 = [-2, 4].
+    # r = 0: z1 = (T(z0) + z0) / 2 = [-0.5, 1]; z2 = 2/3 T(z1) + 1/3 z0 = [-1, 2].
+    p = (csr_matrix((0, 2)), np.zeros(0), np.array([-5.0, -5.0]), np.array([5.0, 5.0]), np.array([1.0, -2.0]), 0)
+    for refl, answers in ((1.0, ([-1.0, 2.0], [-2.0, 4.0])), (0.0, ([-0.5, 1.0], [-1.0, 2.0]))):
+        for k, want in zip((1, 2), answers):
+            r = pdhg.restarted_pdhg(*p, iterations=k, check_interval=k, certify=False, halpern=True, reflection=refl)
+            np.testing.assert_allclose(r["x"], want, rtol=0, atol=1e-15)
+
+
 def test_restarted_pdhg_average_is_running_mean_and_box_invariant():
     ub = workloads.uptake_batch(batch=1, seed=4, m=40, n=60)
     p = ub.problem(0)
