@@ -90,6 +90,7 @@ struct lpe_handle {
     double eta = 0.99, theta = 1.0, pw = 1.0;
     int ruiz = 10, shared_c = 0;
     int halpern = 0;          // restarted method with reflected Halpern iterations
+    bool t_valid = false;     // Halpern: xsum / ysum hold T(z) of the latest iteration
     int dev = 0;
 
     // host structure
@@ -402,6 +403,7 @@ int kernel_launches_per_iteration(lpe_handle* h) { return h->m > 0 ? 2 : 1; }
 
 int run_iterations(lpe_handle* h, long iters, int chunk, int use_graph) {
     if (iters <= 0) return LPE_OK;
+    if (h->halpern) h->t_valid = true;
     if (chunk < 1) chunk = 64;
     long chunks = use_graph ? iters / chunk : 0;
     long tail = iters - chunks * chunk;
@@ -501,12 +503,12 @@ int launch_check_t(lpe_handle* h, int K, int certify, double ep, double ed, doub
     if (h->method == LPE_METHOD_RESTARTED) {
         ApplyArgs<T> ax;
         ax.rows = h->n; ax.ld = h->ld; ax.rows_per_block = ROWS_PER_BLOCK_EW; ax.st = h->d_st;
-        ax.v = it.x; ax.vsum = it.xsum; ax.vlast = it.xlast; ax.vbar = it.xbar;
+        ax.v = it.x; ax.vsum = it.xsum; ax.vlast = it.xlast; ax.vbar = it.xbar; ax.keep_sum = h->halpern;
         hipLaunchKernelGGL((k_check_apply<T>), ew_grid(h->n, h->ld), dim3(256), 0, h->s, ax);
         if (h->m > 0) {
             ApplyArgs<T> ay;
             ay.rows = h->m; ay.ld = h->ld; ay.rows_per_block = ROWS_PER_BLOCK_EW; ay.st = h->d_st;
-            ay.v = it.y; ay.vsum = it.ysum; ay.vlast = it.ylast; ay.vbar = nullptr;
+            ay.v = it.y; ay.vsum = it.ysum; ay.vlast = it.ylast; ay.vbar = nullptr; ay.keep_sum = h->halpern;
             hipLaunchKernelGGL((k_check_apply<T>), ew_grid(h->m, h->ld), dim3(256), 0, h->s, ay);
         }
     }
@@ -969,6 +971,14 @@ int keep_results(lpe_handle* h, const View& v, const std::vector<int>& mask) {
 // full-batch polish when at most half of the batch needs it.  On return
 // h->h_sel marks the LPs that passed the certificate, their results are in
 // d_xres / d_yres, and d_sel mirrors h->h_sel.
+// The polish starts from the PDHG output: the iterate, or for Halpern the
+// operator value T(z) of the latest iteration (the Halpern iterate itself is
+// pulled towards the restart point and blurs the active set).
+template <typename T>
+T* polish_x(Iter<T>& it, const lpe_handle* h) { return h->halpern && h->t_valid ? it.xsum : it.x; }
+template <typename T>
+T* polish_y(Iter<T>& it, const lpe_handle* h) { return h->halpern && h->t_valid ? it.ysum : it.y; }
+
 int polish_selected(lpe_handle* h, int cg_iters, double tol, double ep, double ed, double eg) {
     const bool f32 = h->method != LPE_METHOD_FIXED && h->precision == LPE_PREC_FP32;
     std::vector<int> idx;
@@ -983,8 +993,8 @@ int polish_selected(lpe_handle* h, int cg_iters, double tol, double ep, double e
         std::vector<int> hsel(h->h_sel, h->h_sel + h->ld);
         std::vector<double> met;
         HIPCHK(hipMemcpyAsync(h->d_sel, h->h_sel, sizeof(int) * h->ld, hipMemcpyHostToDevice, h->s));
-        if (f32) CHK(polish_t<float>(h, v, hsel.data(), h->it32.x, h->it32.y, 0, cg_iters, tol));
-        else CHK(polish_t<double>(h, v, hsel.data(), h->it64.x, h->it64.y, 0, cg_iters, tol));
+        if (f32) CHK(polish_t<float>(h, v, hsel.data(), polish_x(h->it32, h), polish_y(h->it32, h), 0, cg_iters, tol));
+        else CHK(polish_t<double>(h, v, hsel.data(), polish_x(h->it64, h), polish_y(h->it64, h), 0, cg_iters, tol));
         CHK(certify_view(h, v, hsel, ep, ed, eg, passed, met));
         trace_failures("round 1", hsel, passed, met, ep, ed, eg);
         CHK(keep_results(h, v, passed));
@@ -1027,12 +1037,12 @@ int polish_selected(lpe_handle* h, int cg_iters, double tol, double ep, double e
     if (f32) {
         float* cx = reinterpret_cast<float*>(h->d_cx);
         float* cy = reinterpret_cast<float*>(h->d_cy);
-        gather_cols<float>(h, h->it32.x, cx, n, ldp);
-        gather_cols<float>(h, h->it32.y, cy, m, ldp);
+        gather_cols<float>(h, polish_x(h->it32, h), cx, n, ldp);
+        gather_cols<float>(h, polish_y(h->it32, h), cy, m, ldp);
         CHK(polish_rounds<float>(h, v, hsel, cx, cy, cg_iters, tol, ep, ed, eg, passed));
     } else {
-        gather_cols<double>(h, h->it64.x, h->d_cx, n, ldp);
-        gather_cols<double>(h, h->it64.y, h->d_cy, m, ldp);
+        gather_cols<double>(h, polish_x(h->it64, h), h->d_cx, n, ldp);
+        gather_cols<double>(h, polish_y(h->it64, h), h->d_cy, m, ldp);
         CHK(polish_rounds<double>(h, v, hsel, h->d_cx, h->d_cy, cg_iters, tol, ep, ed, eg, passed));
     }
     CHK(keep_results(h, v, passed));
@@ -1528,6 +1538,7 @@ int lpe_reset(lpe_handle* h, const double* x0, const double* y0, void* stream) {
     }
     HIPCHK(hipGetLastError());
     CHK(init_state(h));
+    h->t_valid = false;
     h->ready = true;
     return leave(h, stream);
 }

@@ -784,6 +784,7 @@ struct ApplyArgs {
     T* vsum;
     T* vlast;
     T* vbar;     // xbar, or nullptr for duals
+    int keep_sum;   // Halpern: vsum holds T(z) (the polish input), not a running sum
 };
 
 template <typename T>
@@ -806,7 +807,7 @@ __global__ void k_check_apply(ApplyArgs<T> a) {
         if (a.vbar) a.vbar[i] = v;
         if (act == ACT_RESTART_AVG || act == ACT_RESTART_CUR) {
             a.vlast[i] = v;
-            a.vsum[i] = T(0);
+            if (!a.keep_sum) a.vsum[i] = T(0);
         }
     }
 }

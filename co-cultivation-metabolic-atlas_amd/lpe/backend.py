@@ -36,9 +36,14 @@ class GpuLpBackend:
     name = "mi355x_batched_lp"
 
     def __init__(self, *, precision: str = "fp32", polish: bool = True, max_iterations: int = 20000,
-                 check_interval: int = 64, warm_start: bool = True, launch: tuple | None = None):
+                 check_interval: int = 64, warm_start: bool = True, launch: tuple | None = None,
+                 pdhg: str = "restarted"):
+        if pdhg not in ("restarted", "halpern"):
+            raise ValueError("pdhg must be 'restarted' or 'halpern'")
         self.precision = precision
-        self.method = f"restarted_averaged_pdhg_{precision}" + ("_polish" if polish else "")
+        self.pdhg = pdhg
+        self.method = ("restarted_averaged_pdhg_" if pdhg == "restarted" else "restarted_halpern_pdhg_") + precision \
+            + ("_polish" if polish else "")
         self.options = SolveOptions(max_iterations=max_iterations, check_interval=check_interval, polish=polish)
         if launch:
             self.options.rows_per_wave, self.options.lps_per_lane = launch
@@ -55,7 +60,7 @@ class GpuLpBackend:
         if eng is None:
             a, neq = template
             eng = BatchedLpEngine(a, neq, batch, var_positions=varying, precision=self.precision,
-                                  shared_c=shared_c)
+                                  shared_c=shared_c, method=self.pdhg)
             self._engines[cache_key] = eng
         return eng
 

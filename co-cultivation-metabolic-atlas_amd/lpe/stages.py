@@ -56,8 +56,10 @@ class StageOptions:
     later_precision: str = "fp64"
     # iteration budget of stages 2 and 3 (None: the solve options' budget)
     later_max_iterations: Optional[int] = None
-    # PDHG method of every stage engine ("restarted" or "halpern")
+    # PDHG method of stage 1 and of the later (FP64) stages ("restarted" or
+    # "halpern"; Halpern iterates need FP64 to reach the polish threshold)
     method: str = "restarted"
+    later_method: str = "restarted"
 
 
 def exchange_terms(model: Consortium):
@@ -141,7 +143,7 @@ class CooperativeStages:
         if self.stages.optimize_live_objectives and self.stages.later_precision != precision:
             self.joint2 = BatchedLpEngine(model.A, model.neq, self.n_env, var_positions=model.var_positions,
                                           precision=self.stages.later_precision, shared_c=True, device=self.device,
-                                          method=meth)
+                                          method=self.stages.later_method)
         self.stage3 = None
         if self.stages.parsimonious_exchange and self.n_aux:
             A3, neq3, coupling3, transfer3, tr0 = stage3_pattern(model, self.stages.optimize_live_objectives)
@@ -149,7 +151,7 @@ class CooperativeStages:
             self.var3 = np.concatenate([coupling3.ravel(), transfer3.ravel()])
             self.stage3 = BatchedLpEngine(A3, neq3, self.n_env, var_positions=self.var3,
                                           precision=self.stages.later_precision, shared_c=True, device=self.device,
-                                          method=meth)
+                                          method=self.stages.later_method)
             self.c3 = np.zeros(A3.shape[1])
             self.c3[n:] = 1.0
         self.term_species = torch.as_tensor([s for s, _, _ in self.terms], device=self.device)

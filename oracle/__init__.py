@@ -11,12 +11,15 @@ Modules (each function cites the reference file:line it restates):
   (src/gpu_block_lp.py:52-132, src/cpu_repeated_lp.py:150-178).
 * ``pdhg``        - fixed-metric PDHG of the reference
   (src/gpu_pdhg_corrector.py:185-196,296-358; tests/test_gpu_pdhg_workspace.py:35-50)
-  and the restarted-averaged PDHG the engine runs (published PDLP method; the
-  reference has none).
+  and the restarted-averaged and restarted reflected Halpern PDHG the engine
+  runs (published PDLP / r2HPDHG methods; the reference has neither).
 * ``polish``      - the FP64 active-set polish (no reference counterpart).
 * ``highs``       - HiGHS FP64 through SciPy, the reference's exact CPU path
   (src/cpu_repeated_lp.py:287-296, src/offline_scipy_teacher.py:32-40).
 * ``mlp``         - the warm-start network's forward/backward in FP64.
+* ``stages``      - the cooperative stage chain on HiGHS (src/community_solver.py:80-198, :1207-1844).
+* ``reduction``   - exact homogeneous equality reduction and its lift
+  (src/lp_equality_reduction.py, src/gpu_reduced_pdhg.py:81-103).
 
 Parity status of each function is recorded in its docstring ("pinned by").
 """

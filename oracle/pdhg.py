@@ -20,6 +20,10 @@ Two methods:
   Pinned by: invariants (iterates stay in the box and dual cone; an optimal
   pair is a fixed point; the average is the running mean) and by HiGHS
   optima on the synthetic workloads (tests/test_oracle.py).
+  With ``halpern=True`` the iterations are restarted reflected Halpern PDHG
+  (Lu & Yang, "Restarted Halpern PDHG for linear programming", 2024), pinned
+  by closed-form iterates on a box-only LP (plain and reflected) and by HiGHS
+  optima (tests/test_oracle.py).
 
 Both use the engine's summation order (oracle.spmv) and separate multiply /
 add roundings; ``precision='fp32'`` runs the iterate arithmetic in float32.
