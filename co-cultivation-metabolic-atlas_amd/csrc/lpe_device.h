@@ -1698,12 +1698,21 @@ __global__ void k_convert(const TI* __restrict__ in, TO* __restrict__ out, long 
 // rounding level) or (1e-12 (1 + |c^|))^2 (dual; reduced costs far below
 // what the 1e-7 relative gap gate resolves).
 // perm maps the view's columns to LPs (nullptr: identity).
+// Absolute stop of the two CG systems: rho <= (f (1 + |b^|))^2 (primal) and
+// (f (1 + |c^|))^2 (dual least squares), below which a system has reached the
+// rounding level of its right-hand side.
+#ifndef LPE_CG_FLOOR_P
+#define LPE_CG_FLOOR_P 1e-13
+#endif
+#ifndef LPE_CG_FLOOR_D
+#define LPE_CG_FLOOR_D 1e-11
+#endif
 __global__ void k_cg_floor(const double* bnorm, const double* cnorm, const int* perm, double* floor2, int ld) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= ld) return;
     const int b = perm ? perm[j] : j;
     const double bn = b >= 0 ? bnorm[b] : 0.0, cn = b >= 0 ? cnorm[b] : 0.0;
-    const double fp = 1e-13 * (1.0 + bn), fd = 1e-12 * (1.0 + cn);
+    const double fp = LPE_CG_FLOOR_P * (1.0 + bn), fd = LPE_CG_FLOOR_D * (1.0 + cn);
     floor2[j] = fp * fp;
     floor2[ld + j] = fd * fd;
 }

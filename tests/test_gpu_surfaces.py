@@ -103,13 +103,14 @@ def test_backend_solve_batch_against_highs_and_warm_reuse(device):
 
 
 @pytest.mark.gpu
-def test_backend_mixed_shapes_and_stages_grouped(device):
+@pytest.mark.parametrize("pdhg", ["restarted", "halpern"])
+def test_backend_mixed_shapes_and_stages_grouped(device, pdhg):
     from lpe.backend import GpuLpBackend
     reqs_a, ub_a = _uptake_requests(2, seed=5)
     sp = workloads.species_lp(seed=6, m=80, n=120)
     c_b, kw_b = problem_request(sp)
     kw_b2 = dict(kw_b, _stage="aggregate")
-    backend = GpuLpBackend(precision="fp64", polish=True)
+    backend = GpuLpBackend(precision="fp64", polish=True, pdhg=pdhg)
     try:
         res = backend.solve_batch(reqs_a + [(c_b, kw_b), (c_b, kw_b2)], environment_ids=["a", "b", "c", "d"])
         assert [r.success for r in res] == [True] * 4
