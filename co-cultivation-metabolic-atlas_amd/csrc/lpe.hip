@@ -473,6 +473,12 @@ int launch_check_t(lpe_handle* h, int K, int certify, double ep, double ed, doub
     int grid_r = 0, grid_c = 0;
     Shape sr = make_shape(std::max(h->m, 1), h->ld, h->B, CHECK_RPW, 1, &grid_r);
     Shape sc = make_shape(h->n, h->ld, h->B, CHECK_RPW, 1, &grid_c);
+    if (restarted && h->halpern) {
+        // T(z) of the current iterate into the sum buffers (one operator
+        // application without moving z): the second candidate of the check,
+        // the fixed-point residual and the restart point
+        launch_step(h, 3, 0, 2);
+    }
     if (h->m > 0) {
         CheckRowsArgs<T> r;
         r.sh = sr; r.A = op_scaled(h, false);
@@ -623,7 +629,7 @@ int build_results(lpe_handle* h) {
 }
 
 #ifndef LPE_CG_BLOCK
-#define LPE_CG_BLOCK 25
+#define LPE_CG_BLOCK 10
 #endif
 constexpr int CG_BLOCK = LPE_CG_BLOCK;
 #ifndef LPE_CG_REL
@@ -637,7 +643,7 @@ constexpr int CG_BLOCK = LPE_CG_BLOCK;
 #define LPE_CG_PRIMAL_WEIGHTED 0
 #endif
 #ifndef LPE_CG_PRIMAL_TOL
-#define LPE_CG_PRIMAL_TOL 1e-6
+#define LPE_CG_PRIMAL_TOL 3e-6
 #endif
 constexpr double CG_REL = LPE_CG_REL;   // rho = r.D^-1 r reduction at which a system stops
 
@@ -1114,10 +1120,10 @@ int alloc_all(lpe_handle* h) {
     CHK(dalloc(h, &h->d_partial, h->partial_elems));
     CHK(dalloc(h, &h->d_red_a, (size_t)2 * CC_N * ld)); CHK(dalloc(h, &h->d_red_b, (size_t)2 * CC_N * ld));
     // (red_a / red_b also hold the polish's 3 CG dots per LP)
-    const int ops_cr[2 * CR_N] = {RSUM, RMAX, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM,
-                                  RSUM, RMAX, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM};
-    const int ops_cc[2 * CC_N] = {RSUM, RMAX, RSUM, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM,
-                                  RSUM, RMAX, RSUM, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM};
+    const int ops_cr[2 * CR_N] = {RSUM, RMAX, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM, RSUM,
+                                  RSUM, RMAX, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM, RSUM};
+    const int ops_cc[2 * CC_N] = {RSUM, RMAX, RSUM, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM, RSUM,
+                                  RSUM, RMAX, RSUM, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM, RSUM};
     const int ops_xr[XR_N] = {RMAX, RMAX, RSUM, RMAX, RSUM};
     const int ops_xc[XC_N] = {RMAX, RMAX, RMAX, RMAX, RSUM, RSUM, RSUM, RSUM};
     const int ops_sum[3] = {RSUM, RSUM, RSUM};

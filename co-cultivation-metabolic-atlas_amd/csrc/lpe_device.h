@@ -157,7 +157,7 @@ struct DualArgs {
     const T* anchor;     // [m][ld] y0 (the last restart point)
     const int* hk;       // [ld] Halpern iterations since the restart, before this launch group
     int j;               // iteration index within the launch group
-    int store_t;         // 1: also store y+ = T(z)_y into ysum
+    int store_t;         // 1: also store y+ = T(z)_y into ysum; 2: only that (probe)
     T refl;
 };
 
@@ -214,6 +214,10 @@ __global__ void __launch_bounds__(BLOCK) k_dual_step(DualArgs<T> a) {
             if (ineq && !__builtin_isnan(cand)) nv = __builtin_fmin(cand, T(0));
             out[t] = fr[t] ? yv[t] : nv;
         }
+        if (HAL && a.store_t == 2) {        // probe: T(z)_y only
+            store_vec<T, LPL>(a.ysum + row * ld + b0, out);
+            continue;
+        }
         if (HAL) {
             T y0[LPL], hv[LPL];
             load_vec<T, LPL>(a.anchor + row * ld + b0, y0);
@@ -258,7 +262,7 @@ struct PrimalArgs {
     const T* anchor;     // [n][ld] x0
     const int* hk;       // [ld]
     int j;
-    int store_t;
+    int store_t;         // 1: also store x+ into xsum; 2: only x+ and xbar (probe)
     T refl;
 };
 
@@ -327,6 +331,11 @@ __global__ void __launch_bounds__(BLOCK) k_primal_step(PrimalArgs<T> a) {
             const T ext = a.theta * (nxt - xv[t]);
             nx[t] = nxt;
             nb[t] = nxt + ext;
+        }
+        if (HAL && a.store_t == 2) {        // probe: T(z)_x and its extrapolation only
+            store_vec<T, LPL>(a.xsum + col * ld + b0, nx);
+            store_vec<T, LPL>(a.xbar + col * ld + b0, nb);
+            continue;
         }
         if (HAL) {
             T x0[LPL], hv[LPL];
@@ -459,6 +468,7 @@ enum CheckRowVal {
     CR_DY2,        // sum (y - y_last)^2
     CR_NONFIN,     // count of non-finite activity / y
     CR_ASIG,       // active-set signature: hashed sum over inequality rows with y < 0
+    CR_FPY,        // Halpern: sum (y - T(z)_y)^2 (fixed-point residual, current iterate)
     CR_N
 };
 enum CheckColVal {
@@ -471,6 +481,7 @@ enum CheckColVal {
     CC_DX2,        // sum (x - x_last)^2
     CC_NONFIN,
     CC_ASIG,       // active-set signature: hashed sum over columns at a bound (1 lower, 2 upper)
+    CC_FPX,        // Halpern: sum (x - T(z)_x)^2
     CC_N
 };
 
@@ -543,11 +554,15 @@ __global__ void __launch_bounds__(BLOCK) k_check_rows(CheckRowsArgs<T> a) {
                 w[CR_DY2][0] += dy * dy;
                 if (!__builtin_isfinite(ax[0]) || !__builtin_isfinite(yv)) w[CR_NONFIN][0] += 1.0;
                 if (which == 0 && ineq && yv < 0.0) w[CR_ASIG][0] += asig_weight((unsigned)row + 7919u);
+                if (which == 0 && a.hal) {
+                    const double dt = yv - (double)a.ysum[row * ld + b0];
+                    w[CR_FPY][0] += dt * dt;
+                }
             }
         }
     }
-    const int ops[2 * CR_N] = {RSUM, RMAX, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM,
-                               RSUM, RMAX, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM};
+    const int ops[2 * CR_N] = {RSUM, RMAX, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM, RSUM,
+                               RSUM, RMAX, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM, RSUM};
     block_partials<2 * CR_N, 1>(v, ops, a.partial, rb, ld, b0, lds, valid);
 }
 
@@ -626,6 +641,10 @@ __global__ void __launch_bounds__(BLOCK) k_check_cols(CheckColsArgs<T> a) {
                 const double dx = xv - (double)a.xlast[col * ld + b0];
                 w[CC_DX2][0] += dx * dx;
                 if (!__builtin_isfinite(rc) || !__builtin_isfinite(xv)) w[CC_NONFIN][0] += 1.0;
+                if (which == 0 && a.hal) {
+                    const double dt = xv - (double)a.xsum[col * ld + b0];
+                    w[CC_FPX][0] += dt * dt;
+                }
                 if (which == 0) {
                     const double width = (lfin && ufin && hi - lo > 1.0) ? hi - lo : 1.0;
                     const bool at_lo = lfin && xv - lo <= 1e-6 * width;
@@ -635,8 +654,8 @@ __global__ void __launch_bounds__(BLOCK) k_check_cols(CheckColsArgs<T> a) {
             }
         }
     }
-    const int ops[2 * CC_N] = {RSUM, RMAX, RSUM, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM,
-                               RSUM, RMAX, RSUM, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM};
+    const int ops[2 * CC_N] = {RSUM, RMAX, RSUM, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM, RSUM,
+                               RSUM, RMAX, RSUM, RSUM, RSUM, RMAX, RSUM, RSUM, RSUM, RSUM};
     block_partials<2 * CC_N, 1>(v, ops, a.partial, rb, ld, b0, lds, valid);
 }
 
@@ -739,6 +758,37 @@ __global__ void k_check_decide(DecideArgs a) {
         s.action = pass[0] ? ACT_ACCEPT_CUR : ACT_ACCEPT_AVG;
         s.accepted_iter = s.iters;
         a.flags[b] = 1;
+    } else if (a.restarted && a.hal) {
+        // restarted Halpern (Lu & Yang 2024): restart to T(z) on the weighted
+        // fixed-point residual |z - T(z)|_w = sqrt(w |dx|^2 + |dy|^2 / w)
+        s.rel_kkt = fmin(rel[0], rel[1]);
+        const double fp = sqrt(om * a.cols[CC_FPX * ld + b] + a.rows[CR_FPY * ld + b] / om);
+        bool restart = false;
+        if (!(s.kkt_last < INFINITY)) {
+            s.kkt_last = fp;
+        } else {
+            restart = fp <= 0.2 * s.kkt_last || (fp <= 0.8 * s.kkt_last && fp > s.kkt_prev) ||
+                      (double)since >= 0.36 * (double)s.iters;
+        }
+        s.kkt_prev = fp;
+        if (restart) {
+            s.action = ACT_RESTART_AVG;
+            const double dx = sqrt(a.cols[(long)(CC_N + CC_DX2) * ld + b]);
+            const double dy = sqrt(a.rows[(long)(CR_N + CR_DY2) * ld + b]);
+            if (dx > 1e-10 && dy > 1e-10 && isfinite(dx) && isfinite(dy)) {
+                om = exp(0.5 * log(dy / dx) + 0.5 * log(om));
+                om = fmin(fmax(om, 1e-8), 1e8);
+            }
+            s.omega = om;
+            s.kkt_last = fp;
+            s.kkt_prev = INFINITY;
+            s.restarts += 1;
+            a.cnt[b] = 0;
+        }
+        a.omega64[b] = om;
+        a.omega32[b] = (float)om;
+        a.inv_omega64[b] = 1.0 / om;
+        a.inv_omega32[b] = (float)(1.0 / om);
     } else if (a.restarted) {
         const double kc = kkt[best];
         bool restart = false;

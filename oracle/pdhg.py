@@ -216,7 +216,6 @@ def restarted_pdhg(a, rhs, lower, upper, c, neq, *, iterations, check_interval=6
                     yp = _dual_step(y, bsT, ax, sigT * om, ineq).astype(T)
                     y = (lam * ((T(1) + refl) * yp - refl * y) + (T(1) - lam) * ylast).astype(T)
                 x = xn
-                xsum, ysum = xp, yp          # T(z) of the latest iteration
                 cnt += 1
                 continue
             if m:
@@ -229,19 +228,36 @@ def restarted_pdhg(a, rhs, lower, upper, c, neq, *, iterations, check_interval=6
             xsum = (xsum + x).astype(T)
         iters += k
         div = cnt if halpern else cnt + k
+        fp = None
         if halpern:
-            xa, ya = xsum.astype(np.float64), ysum.astype(np.float64)
+            # probe: T(z) of the current iterate (the second candidate and the
+            # restart point); fixed-point residual |z - T(z)| in the w-norm
+            aty = csr_rmatvec_ordered(AsT, y).astype(T) if m else np.zeros(n, T)
+            xp, xbar_p = _primal_step(x, aty, csT, tauT * iom, lsT, usT, T(theta))
+            xp, xbar_p = xp.astype(T), xbar_p.astype(T)
+            yp = y
+            if m:
+                ax = csr_matvec_ordered(AsT, xbar_p).astype(T)
+                yp = _dual_step(y, bsT, ax, sigT * om, ineq).astype(T)
+            xa, ya = xp.astype(np.float64), yp.astype(np.float64)
+            fx = float(np.sum((x.astype(np.float64) - xa) ** 2))
+            fy = float(np.sum((y.astype(np.float64) - ya) ** 2))
+            fp = float(np.sqrt(omega * fx + fy / omega))
         else:
             xa, ya = (xsum.astype(np.float64) / div), (ysum.astype(np.float64) / div)
         vc, va = evaluate(x, y), evaluate(xa, ya)
         kc, ka = kkt_w(vc), kkt_w(va)
         best = 1 if ka < kc else 0
-        history.append(dict(iteration=iters, kkt_cur=kc, kkt_avg=ka, rel=rel(va if best else vc), omega=omega))
+        history.append(dict(iteration=iters, kkt_cur=kc, kkt_avg=ka,
+                            rel=min(rel(vc), rel(va)) if halpern else rel(va if best else vc), omega=omega))
         if certify and (passes(vc) or passes(va)):
             if not passes(vc):
                 x, y = xa.astype(T), ya.astype(T)
             status, accepted_at = "optimal", iters
             break
+        if halpern:
+            best = 1                         # Halpern restarts to T(z), on the fixed-point residual
+            kc = ka = fp
         kcand = ka if best else kc
         restart = False
         if not np.isfinite(kkt_last):
