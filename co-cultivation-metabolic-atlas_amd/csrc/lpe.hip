@@ -518,6 +518,25 @@ int launch_check_t(lpe_handle* h, int K, int certify, double ep, double ed, doub
             hipLaunchKernelGGL((k_check_apply<T>), ew_grid(h->m, h->ld), dim3(256), 0, h->s, ay);
         }
     }
+    if (restarted && h->halpern) {
+        // restart reference at the new points: T(z0) and |z0 - T(z0)|_w
+        launch_step(h, 3, 0, 2);
+        if (h->m > 0) {
+            CheckRowsArgs<T> r;
+            r.sh = sr; r.A = op_scaled(h, false);
+            r.x = it.x; r.xsum = it.xsum; r.y = it.y; r.ysum = it.ysum; r.ylast = it.ylast;
+            r.rhs = h->d_rhs_s; r.dr = h->d_dr; r.flags = h->d_flags; r.cnt = h->d_cnt; r.K = K; r.neq = h->neq;
+            r.with_avg = 0; r.hal = 1; r.partial = h->d_partial;
+            hipLaunchKernelGGL((k_check_rows<T>), dim3(grid_r), dim3(BLOCK), 0, h->s, r);
+            reduce(h, h->d_partial, h->d_red_a, h->d_ops_cr, 2 * CR_N, sr.n_rb);
+        }
+        CheckColsArgs<T> c2 = c;
+        c2.with_avg = 0;
+        hipLaunchKernelGGL((k_check_cols<T>), dim3(grid_c), dim3(BLOCK), 0, h->s, c2);
+        reduce(h, h->d_partial, h->d_red_b, h->d_ops_cc, 2 * CC_N, sc.n_rb);
+        hipLaunchKernelGGL(k_hal_restart_ref, dim3(ceil_div(h->B, 256)), dim3(256), 0, h->s, h->d_st,
+                           (const double*)h->d_red_a, (const double*)h->d_red_b, h->B, h->ld);
+    }
     HIPCHK(hipGetLastError());
     return LPE_OK;
 }

@@ -1784,6 +1784,16 @@ __global__ void k_init_omega(LpState* st, double* om, float* om_f, double* iom, 
     om[b] = w; om_f[b] = (float)w; iom[b] = 1.0 / w; iom_f[b] = (float)(1.0 / w);
 }
 
+// Halpern: the restart reference of an LP that restarted at this check is the
+// fixed-point residual at its new point z0 = T(z) (measured by a second probe
+// and check pass): |z0 - T(z0)|_w.
+__global__ void k_hal_restart_ref(LpState* st, const double* rows, const double* cols, int batch, int ld) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch || st[b].action != ACT_RESTART_AVG) return;
+    const double om = st[b].omega;
+    st[b].kkt_last = sqrt(om * cols[(long)CC_FPX * ld + b] + rows[(long)CR_FPY * ld + b] / om);
+}
+
 // Halpern: the iterations since the last restart advance by the launch
 // group's length (the kernels of the group read cnt + j).
 __global__ void k_hal_advance(int* cnt, const int* flags, int count, int batch) {

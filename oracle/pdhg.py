@@ -276,6 +276,19 @@ def restarted_pdhg(a, rhs, lower, upper, c, neq, *, iterations, check_interval=6
             xbar, xlast, ylast = x.copy(), x.copy(), y.copy()
             xsum = np.zeros(n, T); ysum = np.zeros(m, T)
             kkt_last, kkt_prev, cnt = kcand, np.inf, 0
+            if halpern:
+                # the restart reference is the fixed-point residual at the new point
+                om2, iom2 = T(omega), T(1.0 / omega)
+                aty = csr_rmatvec_ordered(AsT, y).astype(T) if m else np.zeros(n, T)
+                xq, xbq = _primal_step(x, aty, csT, tauT * iom2, lsT, usT, T(theta))
+                xq, xbq = xq.astype(T), xbq.astype(T)
+                yq = y
+                if m:
+                    ax = csr_matvec_ordered(AsT, xbq).astype(T)
+                    yq = _dual_step(y, bsT, ax, sigT * om2, ineq).astype(T)
+                fx = float(np.sum((x.astype(np.float64) - xq.astype(np.float64)) ** 2))
+                fy = float(np.sum((y.astype(np.float64) - yq.astype(np.float64)) ** 2))
+                kkt_last = float(np.sqrt(omega * fx + fy / omega))
             restarts += 1
         elif not halpern:
             cnt += k
