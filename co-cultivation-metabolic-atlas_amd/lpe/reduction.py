@@ -22,7 +22,7 @@ from __future__ import annotations
 import ctypes as C
 import hashlib
 import time
-from typing import Any, Optional
+from typing import Optional
 
 import numpy as np
 import torch

@@ -33,11 +33,11 @@ from typing import Optional
 
 import numpy as np
 import torch
-from scipy.sparse import coo_matrix, csr_matrix
+from scipy.sparse import coo_matrix
 
 from . import _lib
 from .engine import BatchedLpEngine, SolveOptions
-from .workloads import BOUND, Consortium
+from .workloads import Consortium
 
 TRANSFER_STOICH = -1.0   # exchange reactions consume the medium metabolite
 
