@@ -298,7 +298,10 @@ def gem_reduced_leg(args, dev):
     eng.reset()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    info = eng.solve(SolveOptions(max_iterations=args.gem_iterations, check_interval=64, polish=False))
+    # primal gate 1e-7 instead of the reference's 1e-5: with row duals of order 1-5 the looser
+    # gate admits answers up to ~5e-6 away from HiGHS in objective
+    info = eng.solve(SolveOptions(max_iterations=args.gem_iterations, check_interval=64, polish=False,
+                                  eps_primal=1e-7))
     x, _, status = eng.state()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
@@ -308,7 +311,7 @@ def gem_reduced_leg(args, dev):
     eng.close()
     print(f"# real-GEM leg: {n_ok}/{B} certified in {dt:.2f} s", file=sys.stderr, flush=True)
     report = {"environments": B, "lp": f"{md.shape[0]}x{md.shape[1]} -> reduced {rm}x{rn}",
-              "method": "equality reduction + restarted Halpern PDHG, FP64", "certified": n_ok,
+              "method": "equality reduction + restarted Halpern PDHG, FP64, primal gate 1e-7", "certified": n_ok,
               "seconds": round(dt, 3), "certified_lps_per_s": round(n_ok / dt, 2), "iterations": info["iterations"],
               "cpu_baseline": None}
     if not args.no_cpu:

@@ -453,6 +453,18 @@ class ReducedLpEngine:
             x, y, _ = self.state()
         return self.original.certify(x, y, eps=eps)
 
+    def set_primal_weight(self, omega) -> None:
+        self.reduced.set_primal_weight(omega)
+
+    def primal_weight(self) -> np.ndarray:
+        return self.reduced.primal_weight()
+
+    def solve_with_fallback(self, options: Optional[SolveOptions] = None):
+        """``BatchedLpEngine.solve_with_fallback``'s interface: (info, x, y, status)."""
+        info = self.solve(options, fallback=True)
+        x, y, status = self.state()
+        return info, x, y, status
+
     def close(self) -> None:
         for part in ("reduced", "original"):
             eng = getattr(self, part, None)

@@ -39,7 +39,7 @@ class ConsortiumRollout:
                  precision: str = "fp32", options: Optional[SolveOptions] = None,
                  warm_start="previous", dt: float = 1.0, conc_range=(1e-2, 10.0),
                  autotune: bool = True, fallback: bool = True, device: Optional[torch.device] = None,
-                 method: str = "restarted"):
+                 method: str = "restarted", reduce: bool = False):
         self.model = model if model is not None else consortium(seed)
         md = self.model
         self.n_env = int(n_env)
@@ -47,8 +47,18 @@ class ConsortiumRollout:
         self.device = device or torch.device("cuda", torch.cuda.current_device())
         self.options = options or SolveOptions()
         self.warm_start = warm_start
-        self.engine = BatchedLpEngine(md.A, md.neq, self.n_env, var_positions=md.var_positions,
-                                      precision=precision, shared_c=True, device=self.device, method=method)
+        if reduce:
+            # exact equality reduction: the engine iterates on the reduced LP,
+            # accepts on the original certificate (the real-GEM path)
+            from .reduction import ReducedLpEngine
+            rhs_template = np.zeros(md.shape[0])
+            rhs_template[:md.neq] = md.rhs_species
+            self.engine = ReducedLpEngine(md.A, md.neq, self.n_env, var_positions=md.var_positions,
+                                          rhs_template=rhs_template, precision=precision, shared_c=True,
+                                          device=self.device, method=method)
+        else:
+            self.engine = BatchedLpEngine(md.A, md.neq, self.n_env, var_positions=md.var_positions,
+                                          precision=precision, shared_c=True, device=self.device, method=method)
         rng = np.random.default_rng(seed + 7)
         lo, hi = conc_range
         dev = self.device

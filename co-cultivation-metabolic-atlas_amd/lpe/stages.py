@@ -60,6 +60,9 @@ class StageOptions:
     # "halpern"; Halpern iterates need FP64 to reach the polish threshold)
     method: str = "restarted"
     later_method: str = "restarted"
+    # exact homogeneous-equality reduction of every stage LP (iterate on the
+    # reduced LP, accept on the original LP's certificate): the real-GEM path
+    reduce: bool = False
 
 
 def exchange_terms(model: Consortium):
@@ -136,22 +139,31 @@ class CooperativeStages:
         self.g = live_objective(model)
         self.terms = exchange_terms(model)
         self.n_aux = len(self.terms)
-        meth = self.stages.method
-        self.joint = BatchedLpEngine(model.A, model.neq, self.n_env, var_positions=model.var_positions,
-                                     precision=precision, shared_c=True, device=self.device, method=meth)
+        if self.stages.reduce:
+            from .reduction import ReducedLpEngine as engine_class
+        else:
+            engine_class = BatchedLpEngine
+
+        def engine(A, neq, var_positions, prec, method):
+            kw = {}
+            if self.stages.reduce:
+                # species rows keep their rhs in every stage; the rest start homogeneous
+                kw["rhs_template"] = np.zeros(A.shape[0])
+                kw["rhs_template"][:model.neq] = model.rhs_species
+            return engine_class(A, neq, self.n_env, var_positions=var_positions, precision=prec, shared_c=True,
+                                device=self.device, method=method, **kw)
+
+        self.joint = engine(model.A, model.neq, model.var_positions, precision, self.stages.method)
         self.joint2 = self.joint
         if self.stages.optimize_live_objectives and self.stages.later_precision != precision:
-            self.joint2 = BatchedLpEngine(model.A, model.neq, self.n_env, var_positions=model.var_positions,
-                                          precision=self.stages.later_precision, shared_c=True, device=self.device,
-                                          method=self.stages.later_method)
+            self.joint2 = engine(model.A, model.neq, model.var_positions, self.stages.later_precision,
+                                 self.stages.later_method)
         self.stage3 = None
         if self.stages.parsimonious_exchange and self.n_aux:
             A3, neq3, coupling3, transfer3, tr0 = stage3_pattern(model, self.stages.optimize_live_objectives)
             self.A3, self.tr0 = A3, tr0
             self.var3 = np.concatenate([coupling3.ravel(), transfer3.ravel()])
-            self.stage3 = BatchedLpEngine(A3, neq3, self.n_env, var_positions=self.var3,
-                                          precision=self.stages.later_precision, shared_c=True, device=self.device,
-                                          method=self.stages.later_method)
+            self.stage3 = engine(A3, neq3, self.var3, self.stages.later_precision, self.stages.later_method)
             self.c3 = np.zeros(A3.shape[1])
             self.c3[n:] = 1.0
         self.term_species = torch.as_tensor([s for s, _, _ in self.terms], device=self.device)
