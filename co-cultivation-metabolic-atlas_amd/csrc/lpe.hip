@@ -159,7 +159,7 @@ struct lpe_handle {
     hipStream_t s = nullptr;
     hipEvent_t ev_sync = nullptr, ev_a = nullptr, ev_b = nullptr;
     hipGraphExec_t graph = nullptr;
-    std::map<int, hipGraphExec_t> cg_graphs;
+    std::map<long long, hipGraphExec_t> cg_graphs;
     // compaction buffers of the polish
     int* d_perm = nullptr; int* d_csel = nullptr;
     double *d_clo = nullptr, *d_cup = nullptr, *d_crhs = nullptr, *d_cc = nullptr, *d_ccs = nullptr;
@@ -441,16 +441,41 @@ constexpr int CHECK_RPW = 32;
 // Rows per wave of the polish kernels: few rows per wave for narrow (compacted)
 // batches, where the kernels are latency-bound, more for wide ones, where the
 // per-block partial sums dominate.
-int polish_rpw(int ld) { return ld >= 1024 ? 4 : (ld >= 256 ? 2 : 1); }
+// LPE_POLISH_RPW / LPE_POLISH_LPL override both for launch-shape sweeps (read
+// when the handle sizes its partial buffers and at every launch).
+int env_choice(const char* name, int lo, int hi) {
+    const char* v = std::getenv(name);
+    if (!v) return 0;
+    const int k = std::atoi(v);
+    return (k >= lo && k <= hi) ? k : 0;
+}
+int polish_rpw_default(int ld) { return ld >= 1024 ? 4 : (ld >= 256 ? 2 : 1); }
+int polish_rpw(int ld) {
+    const int k = env_choice("LPE_POLISH_RPW", 1, 16);
+    return k ? k : polish_rpw_default(ld);
+}
+// The launch's rows per wave, within the partial buffers sized at create
+// (an override set after create that needs more row blocks is ignored).
+int polish_rpw_h(const lpe_handle* h, int ld) {
+    const int k = polish_rpw(ld);
+    int g;
+    const Shape s = make_shape(std::max(h->m, 1), ld, ld, k, 1, &g);
+    return (size_t)s.n_rb * 3 * ld <= h->partial_elems ? k : polish_rpw_default(ld);
+}
 // LPs per lane of the polish gather kernels (wide batches: two FP64 values per lane)
-int polish_lpl(int ld) { return ld >= 512 ? 2 : 1; }
+int polish_lpl(int ld) {
+    const int k = env_choice("LPE_POLISH_LPL", 1, 4);
+    if (k == 1 || k == 2 || (k == 4 && ld >= 256)) return k;
+    return ld >= 512 ? 2 : 1;
+}
 
 void launch_polish_cols(lpe_handle* h, const PolishColsArgs& a, int ldv) {
     int grid;
     PolishColsArgs b = a;
     const int lpl = polish_lpl(ldv);
     b.sh = make_shape(a.sh.rows, ldv, ldv, a.sh.rpw, lpl, &grid);
-    if (lpl == 2) hipLaunchKernelGGL(k_polish_cols<2>, dim3(grid), dim3(BLOCK), 0, h->s, b);
+    if (lpl == 4) hipLaunchKernelGGL(k_polish_cols<4>, dim3(grid), dim3(BLOCK), 0, h->s, b);
+    else if (lpl == 2) hipLaunchKernelGGL(k_polish_cols<2>, dim3(grid), dim3(BLOCK), 0, h->s, b);
     else hipLaunchKernelGGL(k_polish_cols<1>, dim3(grid), dim3(BLOCK), 0, h->s, b);
 }
 
@@ -460,7 +485,8 @@ int launch_polish_rows(lpe_handle* h, const PolishRowsArgs& a, int ldv) {
     PolishRowsArgs b = a;
     const int lpl = polish_lpl(ldv);
     b.sh = make_shape(a.sh.rows, ldv, ldv, a.sh.rpw, lpl, &grid);
-    if (lpl == 2) hipLaunchKernelGGL(k_polish_rows<2>, dim3(grid), dim3(BLOCK), 0, h->s, b);
+    if (lpl == 4) hipLaunchKernelGGL(k_polish_rows<4>, dim3(grid), dim3(BLOCK), 0, h->s, b);
+    else if (lpl == 2) hipLaunchKernelGGL(k_polish_rows<2>, dim3(grid), dim3(BLOCK), 0, h->s, b);
     else hipLaunchKernelGGL(k_polish_rows<1>, dim3(grid), dim3(BLOCK), 0, h->s, b);
     return b.sh.n_rb;
 }
@@ -658,6 +684,12 @@ int build_results(lpe_handle* h) {
 #define LPE_CG_BLOCK 10
 #endif
 constexpr int CG_BLOCK = LPE_CG_BLOCK;
+// CG steps per graph replay between activity checks (LPE_CG_BLOCK in the
+// environment overrides the build's value, for sweeps)
+int cg_block_steps() {
+    const int k = env_choice("LPE_CG_BLOCK", 1, 100);
+    return k ? k : CG_BLOCK;
+}
 #ifndef LPE_CG_REL
 #define LPE_CG_REL 1e-14
 #endif
@@ -708,16 +740,18 @@ void cg_steps(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, in
     }
 }
 
-// CG_BLOCK preconditioned-CG steps replayed from a hipGraph captured once per
-// view width (the kernels read the selection and all vectors from fixed
-// buffers, so the graph is reused across attempts).
+// cg_block_steps() preconditioned-CG steps replayed from a hipGraph captured once per
+// view width and launch shape (the kernels read the selection and all vectors
+// from fixed buffers, so the graph is reused across attempts).
 int cg_block(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, int grid_r, int grid_c) {
-    const int key = v.ld + (v.sel == h->d_sel ? 0 : 1 << 30);
+    const int steps = cg_block_steps();
+    const long long key = (long long)v.ld | ((long long)sr.rpw << 32) | ((long long)polish_lpl(v.ld) << 40) |
+                          ((long long)steps << 44) | (v.sel == h->d_sel ? 0LL : 1LL << 52);
     hipGraphExec_t& g_exec = h->cg_graphs[key];
     if (!g_exec) {
         hipGraph_t g;
         HIPCHK(hipStreamBeginCapture(h->s, hipStreamCaptureModeThreadLocal));
-        cg_steps(h, v, sr, sc, grid_r, grid_c, CG_BLOCK);
+        cg_steps(h, v, sr, sc, grid_r, grid_c, steps);
         HIPCHK(hipStreamEndCapture(h->s, &g));
         hipError_t e = hipGraphInstantiate(&g_exec, g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
@@ -730,15 +764,16 @@ int cg_block(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, int
 // One polish round over a view.  Input: T iterates in scaled units
 // (orig == 0) or FP64 (x, y) in original units (orig == 1, the previous
 // round's result), both in the view's column order.  CG runs in blocks of
-// CG_BLOCK steps and stops once every selected LP's preconditioned residual
+// cg_block_steps() steps and stops once every selected LP's preconditioned residual
 // has dropped by 1e-10 (rho by 1e-20).  Output: d_xpol / d_ypol (view order).
 template <typename T>
 int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const T* yin, int orig, int cg_iters,
              double tol) {
     const long ld = v.ld;
     int grid_r = 0, grid_c = 0;
-    Shape sr = make_shape(std::max(h->m, 1), v.ld, v.ld, polish_rpw(v.ld), 1, &grid_r);
-    Shape sc = make_shape(h->n, v.ld, v.ld, polish_rpw(v.ld), 1, &grid_c);
+    const int rpw = polish_rpw_h(h, v.ld);
+    Shape sr = make_shape(std::max(h->m, 1), v.ld, v.ld, rpw, 1, &grid_r);
+    Shape sc = make_shape(h->n, v.ld, v.ld, rpw, 1, &grid_c);
     {
         PolishColsInitArgs<T> a;
         a.sh = sc; a.AT = orig ? op_orig_v(h, true, v) : op_scaled_v(h, true, v); a.x = xin; a.y = yin;
@@ -779,7 +814,8 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
                            h->d_cgfloor, v.ld);
         hipLaunchKernelGGL(k_cg_scalars, dim3(ceil_div(v.ld, 256)), dim3(256), 0, h->s, cs);
         std::vector<int> act(2 * ld);
-        const int blocks = (cg_iters + CG_BLOCK - 1) / CG_BLOCK;
+        const int block_steps = cg_block_steps();
+        const int blocks = (cg_iters + block_steps - 1) / block_steps;
         int used = 0;
         for (int k = 0; k < blocks; ++k) {
             if (k > 0) {
@@ -796,7 +832,7 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
                 if (!any) break;
             }
             CHK(cg_block(h, v, sr, sc, grid_r, grid_c));
-            used += CG_BLOCK;
+            used += block_steps;
             if (std::getenv("LPE_TRACE_CG")) {
                 // residual history of the first selected LP (both systems)
                 HIPCHK(hipStreamSynchronize(h->s));
@@ -1952,7 +1988,7 @@ int lpe_time_kernel(lpe_handle* h, int32_t kind, int32_t reps, double* ms_out, v
         hipLaunchKernelGGL(k_activate_all, dim3(ceil_div(h->ld, 256)), dim3(256), 0, h->s, h->d_sel, h->d_cgact,
                            h->B, h->ld);
         const View v = full_view(h);
-        u.sh = make_shape(h->m, v.ld, v.ld, polish_rpw(v.ld), 1, &grid_r);
+        u.sh = make_shape(h->m, v.ld, v.ld, polish_rpw_h(h, v.ld), 1, &grid_r);
         u.w1 = h->d_wp; u.r1 = h->d_rp_; u.p1 = h->d_pp; u.s1 = h->d_s1;
         u.w2 = h->d_wd; u.r2 = h->d_rd; u.p2 = h->d_pd; u.s2 = h->d_s2; u.alpha = h->d_alpha;
         u.invdiag = h->d_invdiag; u.dr = LPE_CG_PRIMAL_WEIGHTED ? h->d_dr : nullptr; u.sel = v.sel;
