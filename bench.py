@@ -73,6 +73,61 @@ def _highs_worker(args):
     return r.status, r.fun, r.seconds
 
 
+def _highs_warm_worker(args):
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    from oracle import highs
+    a_data, a_ind, a_ptr, shape, rhs, lo, hi, c, neq, a_data1, rhs1, lo1, hi1 = args
+    from scipy.sparse import csr_matrix
+    model = highs.PersistentHighs(csr_matrix((a_data, a_ind, a_ptr), shape=shape), rhs, lo, hi, c, neq)
+    r0 = model.solve()                      # the previous step (untimed)
+    model.update(a_data1, rhs1, lo1, hi1)
+    r1 = model.solve()                      # the timed step, from the previous step's basis
+    return r1.status, r1.fun, r1.seconds, r1.iterations, r0.seconds
+
+
+def cpu_baseline_warm(model, snap0, snap1, gpu_x, gpu_ok, seconds_budget, cold_seconds):
+    """HiGHS as the reference's repeated-LP service runs it across dFBA steps
+    (src/cpu_repeated_lp.py: one persistent model per environment, hot updates,
+    the previous solve's basis): each sampled environment solves the first
+    timed step's LP untimed, is updated to the second step's LP and solves it
+    timed."""
+    import multiprocessing as mp
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    from oracle import highs, certificate as cert
+    from lpe.workloads import ConsortiumBatch
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    workers = max(1, min(cores, 16))
+    cb0 = ConsortiumBatch(model=model, conc=snap0[0], biomass=snap0[1])
+    cb1 = ConsortiumBatch(model=model, conc=snap1[0], biomass=snap1[1])
+    # the untimed previous-step solves dominate this leg's wall time
+    n_sample = int(max(workers, min(4 * workers, workers * seconds_budget / max(cold_seconds, 1e-3))))
+    n_sample = min(n_sample, snap0[0].shape[0])
+    jobs = []
+    for b in range(n_sample):
+        a, rhs, lo, hi, c, neq = cb0.problem(b)
+        a1, rhs1, lo1, hi1, _, _ = cb1.problem(b)
+        jobs.append((a.data, a.indices, a.indptr, a.shape, rhs, lo, hi, c, neq, a1.data, rhs1, lo1, hi1))
+    with mp.get_context("spawn").Pool(workers) as pool:
+        res = pool.map(_highs_warm_worker, jobs, chunksize=1)
+    timed = float(sum(r[2] for r in res))
+    errs = []
+    for b, r in enumerate(res):
+        if r[0] == "optimal" and gpu_x is not None and gpu_ok[b]:
+            errs.append(cert.relative_objective_error(float(model.c @ gpu_x[b]), r[1]))
+    return dict(value=n_sample * workers / timed, unit="LPs/s", cores=workers, kind="port",
+                sample=f"{n_sample} LPs of the second timed step, warm: persistent per-environment "
+                       f"{highs.version()} dual-simplex models hot-updated from the first timed step's LP and "
+                       f"started from its basis (1 thread per LP); rate = LPs x {workers} workers / summed "
+                       f"solve seconds ({timed:.2f} s)",
+                solver=highs.version(), mean_warm_lp_seconds=timed / n_sample,
+                mean_simplex_iterations=float(np.mean([r[3] for r in res]))), errs, n_sample
+
+
 def cpu_baseline(model, conc, biomass, gpu_x, gpu_ok, seconds_budget):
     """HiGHS (the reference's exact CPU path, via SciPy) on a bounded sample
     of the first timed step's LPs, one LP per worker process."""
@@ -112,7 +167,7 @@ def cpu_baseline(model, conc, biomass, gpu_x, gpu_ok, seconds_budget):
         obj = float(model.c @ gpu_x[b])
         errs.append(cert.relative_objective_error(obj, funs[b]))
     return dict(value=done / elapsed, unit="LPs/s", cores=workers, kind="port",
-                sample=f"{done} LPs of the first timed step (cold HiGHS {highs.version()} highs-ds, "
+                sample=f"{done} LPs of the first timed step (cold {highs.version()} highs-ds, "
                        f"1 thread per LP, {workers} worker processes), {elapsed:.1f} s",
                 solver=highs.version(), highs_single_lp_seconds=one), errs, n_sample
 
@@ -324,7 +379,7 @@ def gem_reduced_leg(args, dev):
             if status[b] == 1:
                 errs.append(cert.relative_objective_error(float(md.c @ x[b]), ref.fun))
         report["cpu_baseline"] = {"value": round(k / t_h, 3), "unit": "LPs/s", "cores": 1, "kind": "port",
-                                  "sample": f"{k} LPs of the batch, HiGHS {highs.version()} highs-ds, one thread"}
+                                  "sample": f"{k} LPs of the batch, {highs.version()} highs-ds, one thread"}
         report["highs_check"] = {"checked": len(errs), "max_rel_obj_err": max(errs) if errs else None,
                                  "within_1e-6": int(sum(e <= 1e-6 for e in errs))}
     return report
@@ -397,7 +452,7 @@ def main():
               f"{target.log[-1]['iterations']} iterations, {target.log[-1]['wall_seconds']:.2f} s",
               file=sys.stderr, flush=True)
     snapshots, certified, step_logs = [], 0, []
-    first_x = first_ok = None
+    first_x = first_ok = second_x = second_ok = None
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -406,6 +461,8 @@ def main():
         certified += int(out["certified"])
         if k == 0:
             first_x, first_ok = out["x"], np.asarray(out["certified_mask"])
+        elif k == 1:
+            second_x, second_ok = out["x"], np.asarray(out["certified_mask"])
         step_logs.append({kk: out.get(kk) for kk in ("certified", "iterations", "restarts", "polish_attempts",
                                                      "cg_steps", "pdhg_ms", "check_ms", "polish_ms", "wall_seconds",
                                                      "fallback_lps", "fallback_certified", "fallback_seconds")})
@@ -511,6 +568,15 @@ def main():
         cb, errs, n_sample = cpu_baseline(model, snapshots[0][0].cpu().numpy(), snapshots[0][1].cpu().numpy(),
                                           first_x_host, first_ok, args.cpu_seconds)
         result["cpu_baseline"] = cb
+        if len(snapshots) >= 2:
+            # the rollout's CPU baseline is warm (previous step's basis); the
+            # cold figure stays beside it
+            warm, errs1, n1 = cpu_baseline_warm(model, [t.cpu().numpy() for t in snapshots[0]],
+                                                [t.cpu().numpy() for t in snapshots[1]],
+                                                second_x.cpu().numpy(), second_ok, args.cpu_seconds,
+                                                cb["highs_single_lp_seconds"])
+            result["cpu_baseline"], result["cpu_baseline_cold"] = warm, cb
+            errs, n_sample = errs + errs1, n_sample + n1
         result["highs_check"] = {"sample": n_sample, "checked": len(errs),
                                  "max_rel_obj_err": max(errs) if errs else None,
                                  "within_1e-6": int(sum(e <= 1e-6 for e in errs))}

@@ -15,7 +15,9 @@ Modules (each function cites the reference file:line it restates):
   runs (published PDLP / r2HPDHG methods; the reference has neither).
 * ``polish``      - the FP64 active-set polish (no reference counterpart).
 * ``highs``       - HiGHS FP64 through SciPy, the reference's exact CPU path
-  (src/cpu_repeated_lp.py:287-296, src/offline_scipy_teacher.py:32-40).
+  (src/cpu_repeated_lp.py:287-296, src/offline_scipy_teacher.py:32-40): cold
+  ``solve`` and ``PersistentHighs`` (per-environment model, hot updates, the
+  previous solve's basis, as src/cpu_repeated_lp.py:565-655).
 * ``mlp``         - the warm-start network's forward/backward in FP64.
 * ``stages``      - the cooperative stage chain on HiGHS (src/community_solver.py:80-198, :1207-1844).
 * ``reduction``   - exact homogeneous equality reduction and its lift

@@ -130,6 +130,33 @@ def test_highs_matches_vertex_enumeration(seed):
 
 
 # ------------------------------------------------------------------ PDHG
+def test_persistent_highs_hot_updates_match_cold_solves():
+    """The warm CPU path (persistent model, hot updates, previous basis)
+    solves each updated LP to the cold linprog optimum; the updates cover
+    bounds, rhs and per-LP matrix values, and the warm solves need far fewer
+    simplex iterations than a cold start."""
+    md = workloads.consortium(2, shapes=((120, 180), (150, 220), (90, 130)), n_medium=12)
+    cb = workloads.consortium_batch(batch=3, seed=3, model=md, conc_range=(1e-2, 10.0))
+    nxt = workloads.ConsortiumBatch(model=md, conc=cb.conc * 0.7, biomass=cb.biomass * 1.05)
+    for b in range(cb.batch):
+        p0, p1 = cb.problem(b), nxt.problem(b)
+        model = highs.PersistentHighs(*p0)
+        r0 = model.solve()
+        ref0 = highs.solve(*p0)
+        assert r0.status == ref0.status == "optimal"
+        assert cert.relative_objective_error(r0.fun, ref0.fun) <= 1e-9
+        changed = model.update(p1[0].data, p1[1], p1[2], p1[3])
+        assert changed == md.var_positions.size       # every biomass coefficient moved
+        r1 = model.solve()
+        ref1 = highs.solve(*p1)
+        assert r1.status == "optimal"
+        assert cert.relative_objective_error(r1.fun, ref1.fun) <= 1e-9
+        assert np.abs(r1.x - r0.x).max() > 1e-6       # the update moved the solution
+        assert r1.iterations < max(ref1.iterations, 1) / 4
+        gates = cert.certificate(*p1, r1.x, r1.y)
+        assert gates["certificate_passed"], gates
+
+
 def test_reference_metric_known_answer():
     k = _known()["metric"]
     s, t = pdhg.reference_metric(csr_matrix(np.array(k["a"])), k["step_safety"], 1.0)
