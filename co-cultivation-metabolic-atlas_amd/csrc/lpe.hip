@@ -144,6 +144,7 @@ struct lpe_handle {
     double* d_cgfloor = nullptr;
     int last_cg_steps = 0;
     long cg_steps_total = 0;   // CG steps of every polish round so far
+    long cg_lp_steps_total = 0;  // sum over CG blocks of (LPs with an active system) x block steps
     double cg_ptol2 = 0.0;    // primal CG stop: |Dr (b - A x)|_2^2 <= (1e-8 min(dr))^2
     double *d_xpol = nullptr, *d_ypol = nullptr, *d_xres = nullptr, *d_yres = nullptr;
     double* d_metrics = nullptr;
@@ -818,11 +819,12 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
         const int blocks = (cg_iters + block_steps - 1) / block_steps;
         int used = 0;
         for (int k = 0; k < blocks; ++k) {
+            HIPCHK(hipMemcpyAsync(act.data(), h->d_cgact, sizeof(int) * 2 * ld, hipMemcpyDeviceToHost, h->s));
+            HIPCHK(hipStreamSynchronize(h->s));
+            long n_active = 0;
+            for (long q = 0; q < ld; ++q) n_active += (act[q] != 0 || act[ld + q] != 0) ? 1 : 0;
             if (k > 0) {
-                HIPCHK(hipMemcpyAsync(act.data(), h->d_cgact, sizeof(int) * 2 * ld, hipMemcpyDeviceToHost, h->s));
-                HIPCHK(hipStreamSynchronize(h->s));
-                bool any = false;
-                for (long q = 0; q < 2 * ld && !any; ++q) any = act[q] != 0;
+                const bool any = n_active > 0;
                 if (std::getenv("LPE_TRACE_CG")) {
                     long na[2] = {0, 0};
                     for (long q = 0; q < 2 * ld; ++q) na[q / ld] += act[q] != 0;
@@ -833,6 +835,7 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
             }
             CHK(cg_block(h, v, sr, sc, grid_r, grid_c));
             used += block_steps;
+            h->cg_lp_steps_total += n_active * block_steps;
             if (std::getenv("LPE_TRACE_CG")) {
                 // residual history of the first selected LP (both systems)
                 HIPCHK(hipStreamSynchronize(h->s));
@@ -1670,7 +1673,7 @@ int lpe_solve(lpe_handle* h, const lpe_solve_opts* o, lpe_solve_info* info, void
     bool stop = false;
     lpe_solve_info inf;
     std::memset(&inf, 0, sizeof(inf));
-    const long cg0 = h->cg_steps_total;
+    const long cg0 = h->cg_steps_total, cgl0 = h->cg_lp_steps_total;
     float ms = 0.0f;
     long iters = 0;
     int running = 0;
@@ -1795,6 +1798,7 @@ int lpe_solve(lpe_handle* h, const lpe_solve_opts* o, lpe_solve_info* info, void
     inf.restarts = restarts;
     inf.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     inf.cg_steps = h->cg_steps_total - cg0;
+    inf.cg_lp_steps = h->cg_lp_steps_total - cgl0;
     if (info) *info = inf;
     return leave(h, stream);
 }

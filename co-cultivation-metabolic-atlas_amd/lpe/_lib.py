@@ -67,7 +67,7 @@ class LpeSolveInfo(C.Structure):
     _fields_ = [("iterations", C.c_int64), ("certified", C.c_int32),
                 ("polish_attempts", C.c_int32), ("restarts", C.c_int32), ("chunks", C.c_int32),
                 ("seconds", C.c_double), ("pdhg_ms", C.c_double), ("check_ms", C.c_double),
-                ("polish_ms", C.c_double), ("pdhg_kernel_launches", C.c_int64), ("cg_steps", C.c_int64)]
+                ("polish_ms", C.c_double), ("pdhg_kernel_launches", C.c_int64), ("cg_steps", C.c_int64), ("cg_lp_steps", C.c_int64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -127,6 +127,8 @@ typedef struct lpe_solve_info {
     double polish_ms;         /* device time of polish + certificates        */
     int64_t pdhg_kernel_launches;
     int64_t cg_steps;         /* preconditioned-CG steps of the polish rounds */
+    int64_t cg_lp_steps;      /* sum over CG blocks of the LPs with an active
+                                 system times the block's steps (work measure) */
 } lpe_solve_info;
 
 /* Replaces GpuPdhgCorrector.__init__ / GpuPdhgWorkspace.__init__
