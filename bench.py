@@ -429,7 +429,7 @@ def main():
                     help="PDHG iterations of a warm-started step before its first check and polish "
                          "(0: half the check interval)")
     ap.add_argument("--no-gem", action="store_true", help="skip the real-GEM (reduced, Halpern FP64) leg")
-    ap.add_argument("--gem-envs", type=int, default=256)
+    ap.add_argument("--gem-envs", type=int, default=1024)
     ap.add_argument("--gem-iterations", type=int, default=100000)
     ap.add_argument("--no-stages", action="store_true",
                     help="skip the cooperative stage-driver measurement (max-min + parsimonious exchange)")
