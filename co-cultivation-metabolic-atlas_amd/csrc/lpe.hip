@@ -1946,6 +1946,37 @@ int lpe_set_primal_weight(lpe_handle* h, const double* omega, void* stream) {
     return leave(h, stream);
 }
 
+int lpe_set_metric(lpe_handle* h, const double* tau, const double* sigma, double theta, void* stream) {
+    if (!h || !tau || (h->m > 0 && !sigma)) return fail(LPE_EINVAL, "handle, tau and sigma are required");
+    if (h->method != LPE_METHOD_FIXED)
+        return fail(LPE_EINVAL, "a caller-supplied metric needs the fixed-metric method");
+    if (!(theta >= 0.0 && theta <= 1.0)) return fail(LPE_EINVAL, "theta must be finite and between zero and one");
+    CHK(enter(h, stream));
+    const long B = h->B, m = h->m, n = h->n, ld = h->ld;
+    HIPCHK(hipMemsetAsync(h->d_err, 0, sizeof(int) * 8, h->s));
+    hipLaunchKernelGGL(k_validate_metric, dim3(512), dim3(256), 0, h->s, tau, B * n, sigma, m > 0 ? B * m : 0L,
+                       h->d_err);
+    HIPCHK(hipMemcpyAsync(h->h_err, h->d_err, sizeof(int) * 8, hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    if (h->h_err[0]) return fail(LPE_EINVAL, "tau metrics must be strictly positive");
+    if (h->h_err[1]) return fail(LPE_EINVAL, "sigma metrics must be strictly positive");
+    // valid: copy (nothing has changed before this point)
+    h->ready = false;
+    hipLaunchKernelGGL((k_to_internal<double>), dim3(ceil_div(n, 64), ceil_div(ld, 64)), dim3(256), 0, h->s, tau,
+                       h->d_tauref, (int)n, (int)B, (int)ld, (const double*)nullptr, 0, 1.0);
+    if (m > 0)
+        hipLaunchKernelGGL((k_to_internal<double>), dim3(ceil_div(m, 64), ceil_div(ld, 64)), dim3(256), 0, h->s,
+                           sigma, h->d_sigref, (int)m, (int)B, (int)ld, (const double*)nullptr, 0, 1.0);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->s));
+    if (theta != h->theta) {
+        // theta is a launch argument: a captured chunk holds the old value
+        if (h->graph) { hipGraphExecDestroy(h->graph); h->graph = nullptr; }
+        h->theta = theta;
+    }
+    return leave(h, stream);
+}
+
 int lpe_get_primal_weight(lpe_handle* h, double* omega) {
     if (!h || !omega) return fail(LPE_EINVAL, "handle and omega are required");
     HIPCHK(hipMemcpyAsync(h->h_st, h->d_st, sizeof(LpState) * h->ld, hipMemcpyDeviceToHost, h->s));

@@ -1607,6 +1607,19 @@ __global__ void k_validate(const double* __restrict__ c, long nc, const double* 
     if (e_v) atomicAdd(err + 3, e_v);
 }
 
+// Caller-supplied diagonal metric: err[0] / err[1] count tau / sigma entries
+// that are not finite and strictly positive.
+__global__ void k_validate_metric(const double* __restrict__ tau, long nt, const double* __restrict__ sigma, long ns,
+                                  int* err) {
+    const long stride = (long)gridDim.x * blockDim.x;
+    const long t0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int e_t = 0, e_s = 0;
+    for (long i = t0; i < nt; i += stride) e_t += !(tau[i] > 0.0 && isfinite(tau[i]));
+    for (long i = t0; i < ns; i += stride) e_s += !(sigma[i] > 0.0 && isfinite(sigma[i]));
+    if (e_t) atomicAdd(err + 0, e_t);
+    if (e_s) atomicAdd(err + 1, e_s);
+}
+
 // Per-LP values of the variable coefficients, scaled: out[s][b] = in * dr[row_s] * dc[col_s].
 template <typename TO>
 __global__ void k_var_scale(const double* __restrict__ in, TO* __restrict__ out, int nvar, int ld,

@@ -27,7 +27,7 @@ METRIC_NAMES = ("primal_residual", "dual_violation", "relative_kkt_gap", "object
 
 EXPORTS = ("lpe_create", "lpe_update", "lpe_reset", "lpe_run", "lpe_set_accepted", "lpe_solve",
            "lpe_certify", "lpe_state", "lpe_polish", "lpe_spmv", "lpe_get_scaling",
-           "lpe_get_primal_weight", "lpe_set_primal_weight", "lpe_set_launch", "lpe_time_iterations", "lpe_time_kernel", "lpe_destroy",
+           "lpe_get_primal_weight", "lpe_set_primal_weight", "lpe_set_metric", "lpe_set_launch", "lpe_time_iterations", "lpe_time_kernel", "lpe_destroy",
            "lpe_last_error", "lpe_version",
            "lpe_reduction_create", "lpe_reduction_get_dims", "lpe_reduction_get_plan",
            "lpe_reduction_get_pattern", "lpe_reduction_get_lift_map", "lpe_reduction_reduce",
@@ -122,6 +122,7 @@ def load() -> C.CDLL:
         "lpe_get_scaling": ([P, P, P, P, P], C.c_int),
         "lpe_get_primal_weight": ([P, P], C.c_int),
         "lpe_set_primal_weight": ([P, P, P], C.c_int),
+        "lpe_set_metric": ([P, P, P, D, P], C.c_int),
         "lpe_set_launch": ([P, I32, I32], C.c_int),
         "lpe_time_iterations": ([P, I64, I32, I32, C.POINTER(D), P], C.c_int),
         "lpe_time_kernel": ([P, I32, I32, C.POINTER(D), P], C.c_int),

@@ -298,6 +298,16 @@ class BatchedLpEngine:
         with torch.cuda.device(self.device):
             check(self._lib.lpe_set_primal_weight(self._h, w.ctypes.data_as(C.c_void_p), _stream()))
 
+    def set_metric(self, tau, sigma, theta: float) -> None:
+        """Caller-supplied diagonal metric of the fixed-metric method: tau
+        [B, n], sigma [B, m] (strictly positive), theta in [0, 1]; a reset is
+        required afterwards and the next ``update`` restores the default metric."""
+        tau_t = _device_array(tau, (self.batch, self.n), "tau", self.device)
+        sig_t = _device_array(sigma, (self.batch, self.m), "sigma", self.device) if self.m else None
+        with torch.cuda.device(self.device):
+            check(self._lib.lpe_set_metric(self._h, tau_t.data_ptr(), None if sig_t is None else sig_t.data_ptr(),
+                                           float(theta), _stream()))
+
     def primal_weight(self) -> np.ndarray:
         w = np.zeros(self.batch)
         check(self._lib.lpe_get_primal_weight(self._h, w.ctypes.data_as(C.c_void_p)))

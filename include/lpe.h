@@ -200,6 +200,14 @@ int lpe_spmv(lpe_handle *h, int32_t transpose, int32_t precision, const double *
 int lpe_get_scaling(lpe_handle *h, double *row_scale, double *col_scale, double *sigma,
                     double *tau);
 
+/* Replaces the tau / sigma / theta arguments of GpuPdhgWorkspace.__init__ and
+ * update_problem (src/gpu_pdhg_workspace.py:65-68, :209-267): a caller-supplied
+ * diagonal metric for the fixed-metric method, tau [B, n] and sigma [B, m]
+ * (device, row-major per LP, finite and strictly positive; validated before
+ * anything changes) and theta in [0, 1].  Kept until the next lpe_update (which
+ * restores the default metric); lpe_reset is required afterwards. */
+int lpe_set_metric(lpe_handle *h, const double *tau, const double *sigma, double theta, void *stream);
+
 /* Per-LP primal weights [B] (host). */
 int lpe_get_primal_weight(lpe_handle *h, double *omega);
 

@@ -70,13 +70,15 @@ def _primal_step(x, aty, c, tau, lo, hi, theta, frozen=None):
 
 
 def fixed_pdhg(a, rhs, lower, upper, c, neq, *, iterations, sigma=None, tau=None, theta=1.0,
-               step_safety=0.99, primal_weight=1.0, x0=None, y0=None):
-    """Fixed-metric PDHG, x and y in original units (one LP)."""
+               step_safety=0.99, primal_weight=1.0, x0=None, y0=None, inequality_mask=None):
+    """Fixed-metric PDHG, x and y in original units (one LP).  Inequality
+    rows are rows >= neq, or the rows ``inequality_mask`` marks (the
+    workspace's ``dual_upper`` = 0 rows, src/gpu_pdhg_workspace.py:255-257)."""
     a = csr_matrix(a)
     m, n = a.shape
     if sigma is None or tau is None:
         sigma, tau = reference_metric(a, step_safety, primal_weight)
-    ineq = np.arange(m) >= neq
+    ineq = np.arange(m) >= neq if inequality_mask is None else np.asarray(inequality_mask, dtype=bool)
     x = np.zeros(n) if x0 is None else np.asarray(x0, dtype=np.float64).copy()
     y = np.zeros(m) if y0 is None else np.asarray(y0, dtype=np.float64).copy()
     x = np.minimum(np.maximum(x, lower), upper)
