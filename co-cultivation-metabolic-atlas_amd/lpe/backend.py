@@ -24,7 +24,6 @@ import time
 from types import SimpleNamespace
 
 import numpy as np
-import torch
 
 from . import _lib
 from .corrector import shared_pattern

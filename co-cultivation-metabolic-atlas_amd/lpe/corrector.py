@@ -19,7 +19,6 @@ import numpy as np
 import torch
 from scipy.sparse import csr_matrix
 
-from . import _lib
 from .engine import BatchedLpEngine
 
 

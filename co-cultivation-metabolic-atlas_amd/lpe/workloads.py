@@ -30,7 +30,7 @@ The joint consortium LP is fixed to the inequality form: 3 species blocks
 
 from __future__ import annotations
 
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import numpy as np
 from scipy.sparse import csr_matrix, coo_matrix

@@ -13,7 +13,7 @@ import os
 
 import numpy as np
 import pytest
-from scipy.sparse import csr_matrix, random as sprandom
+from scipy.sparse import csr_matrix
 
 from lpe import workloads
 from oracle import certificate as cert, highs, pdhg, polish, spmv
