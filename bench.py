@@ -55,19 +55,33 @@ def _polish_row_blocks(m, ld):
     return max(1, -(-m // (4 * rpw)))
 
 
+def _polish_cols_bytes(B, m, n, nnz, nvar):
+    """One polish columns pass q = D A^T p, both CG systems, every LP active:
+    p1, p2 gathered (2 m FP64 words per LP), q1, q2 written (2 n), the free
+    mask (n bytes), per-LP coefficients, activity / selection; the shared
+    transposed structure and scaled values once."""
+    per_lp = (2 * m + 2 * n) * 8 + n + nvar * 8 + 12
+    return B * per_lp + nnz * 16 + (n + 1) * 4
+
+
+def _polish_rows_bytes(B, m, n, nnz, nvar, n_rb):
+    """One polish rows pass s = R A q with the p.s partials, both systems:
+    q1, q2 gathered (2 n words), p1, p2 read and s1, s2 written (4 m), the row
+    mask, per-LP coefficients, two partials per row block; shared CSR once."""
+    per_lp = (2 * n + 4 * m) * 8 + m + nvar * 8 + 2 * n_rb * 8 + 12
+    return B * per_lp + nnz * 16 + (m + 1) * 4
+
+
 def _cg_step_bytes(m, n, nnz, nvar, n_rb):
     """Algorithmic bytes of one polish CG step (both systems): (per active LP,
-    shared per step).  Columns pass q = D A^T p: p1, p2 gathered, q1, q2
-    written, free mask; rows pass s = R A q with the p.s partials: q1, q2
-    gathered, p1, p2 read, s1, s2 written, row mask; the CG update (13 m
-    words); the direction p = D^-1 r + beta p (7 m words); per-LP coefficients
-    read by both gather passes of both systems."""
-    cols = (2 * m + 2 * n) * 8 + n + 2 * nvar * 8
-    rows = (2 * n + 4 * m) * 8 + m + 2 * n_rb * 8 + 2 * nvar * 8
+    shared per step) - the columns and rows passes, the CG update (13 m words)
+    and the direction p = D^-1 r + beta p (7 m words)."""
+    cols = _polish_cols_bytes(1, m, n, 0, nvar) - (n + 1) * 4
+    rows = _polish_rows_bytes(1, m, n, 0, nvar, n_rb) - (m + 1) * 4
     update = 13 * m * 8 + 3 * n_rb * 8
     direction = 7 * m * 8
     shared = 2 * (nnz * 16) + (n + 1) * 4 + (m + 1) * 4
-    return cols + rows + update + direction + 64, shared
+    return cols + rows + update + direction + 24, shared
 
 
 def _dual_bytes(B, m, n, nnz, nvar, w):
@@ -220,6 +234,8 @@ def probe_kernels(args):
     ms_p = eng.time_kernel("primal", reps=args.probe_reps)
     ms_d = eng.time_kernel("dual", reps=args.probe_reps)
     ms_cg = eng.time_kernel("cg_update", reps=args.probe_reps)
+    ms_pc = eng.time_kernel("polish_cols", reps=args.probe_reps)
+    ms_pr = eng.time_kernel("polish_rows", reps=args.probe_reps)
     n_copy = 256 << 20                                   # 256 Mi float32 = 1 GiB read + 1 GiB written
     src = torch.ones(n_copy, dtype=torch.float32, device=dev)
     dst = torch.empty_like(src)
@@ -230,7 +246,13 @@ def probe_kernels(args):
     print(json.dumps({"probe": "step kernels", "envs": args.envs, "rows_per_wave": args.rows_per_wave,
                       "lps_per_lane": args.lps_per_lane, "reps": args.probe_reps,
                       "primal_ms": ms_p, "dual_ms": ms_d, "cg_update_ms": ms_cg,
+                      "polish_cols_ms": ms_pc, "polish_rows_ms": ms_pr,
                       "cg_update_algorithmic_bytes": _cg_update_bytes(args.envs, m, _polish_row_blocks(m, args.envs)),
+                      "polish_cols_algorithmic_bytes": _polish_cols_bytes(args.envs, m, n, model.A.nnz,
+                                                                          model.var_positions.size),
+                      "polish_rows_algorithmic_bytes": _polish_rows_bytes(args.envs, m, n, model.A.nnz,
+                                                                          model.var_positions.size,
+                                                                          _polish_row_blocks(m, args.envs)),
                       "copy_bytes_read": 4 * n_copy,
                       "copy_bytes_written": 4 * n_copy,
                       "primal_algorithmic_bytes": _primal_bytes(args.envs, m, n, model.A.nnz,
@@ -501,6 +523,8 @@ def main():
     ms_primal = eng.time_kernel("primal", reps=20)
     ms_dual = eng.time_kernel("dual", reps=20)
     ms_cg = eng.time_kernel("cg_update", reps=20)
+    ms_pc = eng.time_kernel("polish_cols", reps=20)
+    ms_pr = eng.time_kernel("polish_rows", reps=20)
     lpl = int((target_tuning or {}).get("lps_per_lane", 1))
     pb = _primal_bytes(B, m, n, nnz, nvar, 4)
     db = _dual_bytes(B, m, n, nnz, nvar, 4)
@@ -522,6 +546,14 @@ def main():
                             ms_per_step=round(mean_cg * ms_cg, 3),
                             traffic=measured_traffic(args.traffic_file, B, lpl, key="cg_update_hbm_bytes")),
     }
+    pcb = _polish_cols_bytes(B, m, n, nnz, nvar)
+    prb = _polish_rows_bytes(B, m, n, nnz, nvar, _polish_row_blocks(m, B))
+    for name, nbytes, ms, key in (("k_polish_cols<2>", pcb, ms_pc, "polish_cols_hbm_bytes"),
+                                  ("k_polish_rows<2>", prb, ms_pr, "polish_rows_hbm_bytes")):
+        kernels[name] = dict(algorithmic_bytes_per_launch=nbytes, avg_launch_ms=round(ms, 5),
+                             achieved=round(nbytes / (ms * 1e-3) / 1e9, 1), launches_per_step=mean_cg,
+                             ms_per_step=round(mean_cg * ms, 3),
+                             traffic=measured_traffic(args.traffic_file, B, lpl, key=key))
     dominant = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
     # whole-step view: algorithmic bytes of every PDHG iteration (all LPs run
     # them) and of every CG step of an LP with an active system, over the

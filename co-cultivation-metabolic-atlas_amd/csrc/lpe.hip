@@ -2011,19 +2011,31 @@ int lpe_time_iterations(lpe_handle* h, int64_t iterations, int32_t chunk, int32_
 
 int lpe_time_kernel(lpe_handle* h, int32_t kind, int32_t reps, double* ms_out, void* stream) {
     if (!h || !ms_out) return fail(LPE_EINVAL, "handle and ms are required");
-    if (kind < 0 || kind > 2) return fail(LPE_EINVAL, "kind must be 0 (dual step), 1 (primal step) or 2 (CG update)");
+    if (kind < 0 || kind > 4)
+        return fail(LPE_EINVAL, "kind must be 0 (dual step), 1 (primal step), 2 (CG update), 3 (polish columns "
+                                "pass) or 4 (polish rows pass)");
     if (reps < 1) return fail(LPE_EINVAL, "reps must be positive");
     if (!h->ready) return fail(LPE_ESTATE, "reset is required after a problem update");
     if (kind != 1 && h->m == 0) return fail(LPE_EINVAL, "no rows");
     CHK(enter(h, stream));
     CgUpdateArgs u;
-    int grid_r = 0;
-    if (kind == 2) {
+    PolishColsArgs c1;
+    PolishRowsArgs c2;
+    int grid_r = 0, grid_c = 0;
+    const View v = full_view(h);
+    if (kind >= 2) {
         // every LP selected, both CG systems active, on the full view
         hipLaunchKernelGGL(k_activate_all, dim3(ceil_div(h->ld, 256)), dim3(256), 0, h->s, h->d_sel, h->d_cgact,
                            h->B, h->ld);
-        const View v = full_view(h);
-        u.sh = make_shape(h->m, v.ld, v.ld, polish_rpw_h(h, v.ld), 1, &grid_r);
+        const int rpw = polish_rpw_h(h, v.ld);
+        c1.sh = make_shape(h->n, v.ld, v.ld, rpw, 1, &grid_c);
+        c1.AT = op_scaled_v(h, true, v); c1.p1 = h->d_pp; c1.p2 = h->d_pd; c1.q1 = h->d_q1; c1.q2 = h->d_q2;
+        c1.c = nullptr; c1.c_shared = 0; c1.fmask = h->d_fmask; c1.sel = v.sel; c1.act = h->d_cgact;
+        c2.sh = make_shape(h->m, v.ld, v.ld, rpw, 1, &grid_r);
+        c2.A = op_scaled_v(h, false, v); c2.q1 = h->d_q1; c2.q2 = h->d_q2; c2.s1 = h->d_s1; c2.s2 = h->d_s2;
+        c2.p1 = h->d_pp; c2.p2 = h->d_pd; c2.setup_r = nullptr; c2.setup_p = nullptr; c2.invdiag = h->d_invdiag;
+        c2.rmask = h->d_rmask; c2.sel = v.sel; c2.act = h->d_cgact; c2.partial = h->d_partial;
+        u.sh = make_shape(h->m, v.ld, v.ld, rpw, 1, &grid_r);
         u.w1 = h->d_wp; u.r1 = h->d_rp_; u.p1 = h->d_pp; u.s1 = h->d_s1;
         u.w2 = h->d_wd; u.r2 = h->d_rd; u.p2 = h->d_pd; u.s2 = h->d_s2; u.alpha = h->d_alpha;
         u.invdiag = h->d_invdiag; u.dr = LPE_CG_PRIMAL_WEIGHTED ? h->d_dr : nullptr; u.sel = v.sel;
@@ -2032,6 +2044,8 @@ int lpe_time_kernel(lpe_handle* h, int32_t kind, int32_t reps, double* ms_out, v
     HIPCHK(hipEventRecord(h->ev_a, h->s));
     for (int i = 0; i < reps; ++i) {
         if (kind == 2) hipLaunchKernelGGL(k_cg_update, dim3(grid_r), dim3(BLOCK), 0, h->s, u);
+        else if (kind == 3) launch_polish_cols(h, c1, v.ld);
+        else if (kind == 4) launch_polish_rows(h, c2, v.ld);
         else launch_step(h, kind == 0 ? 1 : 2);
     }
     HIPCHK(hipEventRecord(h->ev_b, h->s));

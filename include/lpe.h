@@ -228,9 +228,11 @@ int lpe_time_iterations(lpe_handle *h, int64_t iterations, int32_t chunk, int32_
 /* Average duration (ms) of `reps` back-to-back launches of one kernel on
  * the engine's stream, with hipEvents: kind 0 = dual step (the fused A x
  * SpMV + dual update), 1 = primal step (fused A^T y SpMV + primal update +
- * extrapolation + averaging), 2 = the polish's preconditioned-CG update
- * (both systems, every LP of the batch active; its scratch vectors are
- * overwritten).  The iterate is advanced; reset before solving again.  No
+ * extrapolation + averaging), 2 = the polish's preconditioned-CG update,
+ * 3 = the polish's columns pass q = D A^T p, 4 = its rows pass s = R A q with
+ * the p.s partials (2-4: both CG systems, every LP of the batch active; the
+ * polish scratch vectors are overwritten).  The iterate is advanced; reset
+ * before solving again.  No
  * reference counterpart (the reference times whole graph chunks,
  * scripts/benchmark_pdhg_graph_chunk.py:217-234). */
 int lpe_time_kernel(lpe_handle *h, int32_t kind, int32_t reps, double *ms, void *stream);
