@@ -49,6 +49,11 @@ def _cg_update_bytes(B, m, n_rb, weighted=False):
     return B * (13 * m * 8 + 2 * 8 + 2 * 4 + 4 + 3 * n_rb * 8) + (m * 8 if weighted else 0)
 
 
+def _update_row_blocks(m):
+    """Row blocks of the CG update launch (4 waves per block, one row per wave)."""
+    return max(1, -(-m // 4))
+
+
 def _polish_row_blocks(m, ld):
     """Row blocks of the polish launches (4 waves per block, rows per wave by batch)."""
     rpw = 4 if ld >= 1024 else (2 if ld >= 256 else 1)
@@ -72,13 +77,13 @@ def _polish_rows_bytes(B, m, n, nnz, nvar, n_rb):
     return B * per_lp + nnz * 16 + (m + 1) * 4
 
 
-def _cg_step_bytes(m, n, nnz, nvar, n_rb):
+def _cg_step_bytes(m, n, nnz, nvar, n_rb, n_rb_update):
     """Algorithmic bytes of one polish CG step (both systems): (per active LP,
     shared per step) - the columns and rows passes, the CG update (13 m words)
     and the direction p = D^-1 r + beta p (7 m words)."""
     cols = _polish_cols_bytes(1, m, n, 0, nvar) - (n + 1) * 4
     rows = _polish_rows_bytes(1, m, n, 0, nvar, n_rb) - (m + 1) * 4
-    update = 13 * m * 8 + 3 * n_rb * 8
+    update = 13 * m * 8 + 3 * n_rb_update * 8
     direction = 7 * m * 8
     shared = 2 * (nnz * 16) + (n + 1) * 4 + (m + 1) * 4
     return cols + rows + update + direction + 24, shared
@@ -247,7 +252,7 @@ def probe_kernels(args):
                       "lps_per_lane": args.lps_per_lane, "reps": args.probe_reps,
                       "primal_ms": ms_p, "dual_ms": ms_d, "cg_update_ms": ms_cg,
                       "polish_cols_ms": ms_pc, "polish_rows_ms": ms_pr,
-                      "cg_update_algorithmic_bytes": _cg_update_bytes(args.envs, m, _polish_row_blocks(m, args.envs)),
+                      "cg_update_algorithmic_bytes": _cg_update_bytes(args.envs, m, _update_row_blocks(m)),
                       "polish_cols_algorithmic_bytes": _polish_cols_bytes(args.envs, m, n, model.A.nnz,
                                                                           model.var_positions.size),
                       "polish_rows_algorithmic_bytes": _polish_rows_bytes(args.envs, m, n, model.A.nnz,
@@ -528,7 +533,7 @@ def main():
     lpl = int((target_tuning or {}).get("lps_per_lane", 1))
     pb = _primal_bytes(B, m, n, nnz, nvar, 4)
     db = _dual_bytes(B, m, n, nnz, nvar, 4)
-    cgb = _cg_update_bytes(B, m, _polish_row_blocks(m, B))
+    cgb = _cg_update_bytes(B, m, _update_row_blocks(m))
     mean_iters = float(np.mean([s["iterations"] for s in step_logs]))
     mean_cg = float(np.mean([s.get("cg_steps") or 0 for s in step_logs]))
     kernels = {
@@ -558,7 +563,7 @@ def main():
     # whole-step view: algorithmic bytes of every PDHG iteration (all LPs run
     # them) and of every CG step of an LP with an active system, over the
     # timed wall time per step
-    per_lp_cg, shared_cg = _cg_step_bytes(m, n, nnz, nvar, _polish_row_blocks(m, B))
+    per_lp_cg, shared_cg = _cg_step_bytes(m, n, nnz, nvar, _polish_row_blocks(m, B), _update_row_blocks(m))
     pdhg_step_bytes = mean_iters * (pb + db)
     cg_step_bytes = float(np.mean([s.get("cg_lp_steps") or 0 for s in step_logs])) * per_lp_cg + mean_cg * shared_cg
     step_s = elapsed / args.steps

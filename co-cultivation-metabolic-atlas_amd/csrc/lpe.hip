@@ -463,6 +463,16 @@ int polish_rpw_h(const lpe_handle* h, int ld) {
     const Shape s = make_shape(std::max(h->m, 1), ld, ld, k, 1, &g);
     return (size_t)s.n_rb * 3 * ld <= h->partial_elems ? k : polish_rpw_default(ld);
 }
+// Rows per wave of the CG update: one (measured at 4096 LPs: 0.29 ms against
+// 0.34 ms at four rows, for 4x the block partials); LPE_CG_UPDATE_RPW
+// overrides it for sweeps, within the partial buffers sized at create.
+int cg_update_rpw(const lpe_handle* h, int ld) {
+    const int k = env_choice("LPE_CG_UPDATE_RPW", 1, 16);
+    const int r = k ? k : 1;
+    int g;
+    const Shape s = make_shape(std::max(h->m, 1), ld, ld, r, 1, &g);
+    return (size_t)s.n_rb * 3 * ld <= h->partial_elems ? r : polish_rpw_default(ld);
+}
 // LPs per lane of the polish gather kernels (wide batches: two FP64 values per lane)
 int polish_lpl(int ld) {
     const int k = env_choice("LPE_POLISH_LPL", 1, 4);
@@ -725,12 +735,14 @@ void cg_steps(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, in
         cs.mode = 1; cs.d0 = h->d_red_a; cs.d1 = h->d_red_a + v.ld; cs.plain = nullptr;
         hipLaunchKernelGGL(k_cg_scalars, dim3(ceil_div(v.ld, 256)), dim3(256), 0, h->s, cs);
         CgUpdateArgs u;
-        u.sh = sr; u.w1 = h->d_wp; u.r1 = h->d_rp_; u.p1 = h->d_pp; u.s1 = h->d_s1;
+        int grid_u = 0;
+        u.sh = make_shape(std::max(h->m, 1), v.ld, v.ld, cg_update_rpw(h, v.ld), 1, &grid_u);
+        u.w1 = h->d_wp; u.r1 = h->d_rp_; u.p1 = h->d_pp; u.s1 = h->d_s1;
         u.w2 = h->d_wd; u.r2 = h->d_rd; u.p2 = h->d_pd; u.s2 = h->d_s2; u.alpha = h->d_alpha;
         u.invdiag = h->d_invdiag; u.dr = LPE_CG_PRIMAL_WEIGHTED ? h->d_dr : nullptr; u.sel = v.sel;
         u.act = h->d_cgact; u.partial = h->d_partial;
-        hipLaunchKernelGGL(k_cg_update, dim3(grid_r), dim3(BLOCK), 0, h->s, u);
-        reduce_ld(h, h->d_partial, h->d_red_a, h->d_ops_sum, 3, sr.n_rb, v.ld);
+        hipLaunchKernelGGL(k_cg_update, dim3(grid_u), dim3(BLOCK), 0, h->s, u);
+        reduce_ld(h, h->d_partial, h->d_red_a, h->d_ops_sum, 3, u.sh.n_rb, v.ld);
         cs.mode = 2; cs.d0 = h->d_red_a; cs.d1 = h->d_red_a + v.ld; cs.plain = h->d_red_a + 2 * (long)v.ld;
         hipLaunchKernelGGL(k_cg_scalars, dim3(ceil_div(v.ld, 256)), dim3(256), 0, h->s, cs);
         CgDirArgs dd;
@@ -747,7 +759,8 @@ void cg_steps(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, in
 int cg_block(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, int grid_r, int grid_c) {
     const int steps = cg_block_steps();
     const long long key = (long long)v.ld | ((long long)sr.rpw << 32) | ((long long)polish_lpl(v.ld) << 40) |
-                          ((long long)steps << 44) | (v.sel == h->d_sel ? 0LL : 1LL << 52);
+                          ((long long)steps << 44) | (v.sel == h->d_sel ? 0LL : 1LL << 52) |
+                          ((long long)cg_update_rpw(h, v.ld) << 54);
     hipGraphExec_t& g_exec = h->cg_graphs[key];
     if (!g_exec) {
         hipGraph_t g;
@@ -1174,12 +1187,13 @@ int alloc_all(lpe_handle* h) {
     const size_t pr = (size_t)sr.n_rb * 2 * CR_N, pc = (size_t)sc.n_rb * 2 * CC_N;
     h->partial_elems = std::max(pr, pc) * ld;
     // polish rows kernels write 2 partials per row block; narrow views use 1 row per wave
+    // (the CG update runs one row per wave: the largest row-block count)
     for (int w = 64; w <= (int)ld; w *= 2) {
-        Shape sp = make_shape(std::max(h->m, 1), w, w, polish_rpw(w), 1, &g);
+        Shape sp = make_shape(std::max(h->m, 1), w, w, 1, 1, &g);
         h->partial_elems = std::max(h->partial_elems, (size_t)sp.n_rb * 3 * w);
     }
     {
-        Shape sp = make_shape(std::max(h->m, 1), ld, ld, polish_rpw(ld), 1, &g);
+        Shape sp = make_shape(std::max(h->m, 1), ld, ld, 1, 1, &g);
         h->partial_elems = std::max(h->partial_elems, (size_t)sp.n_rb * 3 * ld);
     }
     CHK(dalloc(h, &h->d_partial, h->partial_elems));
@@ -2021,7 +2035,7 @@ int lpe_time_kernel(lpe_handle* h, int32_t kind, int32_t reps, double* ms_out, v
     CgUpdateArgs u;
     PolishColsArgs c1;
     PolishRowsArgs c2;
-    int grid_r = 0, grid_c = 0;
+    int grid_r = 0, grid_c = 0, grid_u = 0;
     const View v = full_view(h);
     if (kind >= 2) {
         // every LP selected, both CG systems active, on the full view
@@ -2035,7 +2049,7 @@ int lpe_time_kernel(lpe_handle* h, int32_t kind, int32_t reps, double* ms_out, v
         c2.A = op_scaled_v(h, false, v); c2.q1 = h->d_q1; c2.q2 = h->d_q2; c2.s1 = h->d_s1; c2.s2 = h->d_s2;
         c2.p1 = h->d_pp; c2.p2 = h->d_pd; c2.setup_r = nullptr; c2.setup_p = nullptr; c2.invdiag = h->d_invdiag;
         c2.rmask = h->d_rmask; c2.sel = v.sel; c2.act = h->d_cgact; c2.partial = h->d_partial;
-        u.sh = make_shape(h->m, v.ld, v.ld, rpw, 1, &grid_r);
+        u.sh = make_shape(h->m, v.ld, v.ld, cg_update_rpw(h, v.ld), 1, &grid_u);
         u.w1 = h->d_wp; u.r1 = h->d_rp_; u.p1 = h->d_pp; u.s1 = h->d_s1;
         u.w2 = h->d_wd; u.r2 = h->d_rd; u.p2 = h->d_pd; u.s2 = h->d_s2; u.alpha = h->d_alpha;
         u.invdiag = h->d_invdiag; u.dr = LPE_CG_PRIMAL_WEIGHTED ? h->d_dr : nullptr; u.sel = v.sel;
@@ -2043,7 +2057,7 @@ int lpe_time_kernel(lpe_handle* h, int32_t kind, int32_t reps, double* ms_out, v
     }
     HIPCHK(hipEventRecord(h->ev_a, h->s));
     for (int i = 0; i < reps; ++i) {
-        if (kind == 2) hipLaunchKernelGGL(k_cg_update, dim3(grid_r), dim3(BLOCK), 0, h->s, u);
+        if (kind == 2) hipLaunchKernelGGL(k_cg_update, dim3(grid_u), dim3(BLOCK), 0, h->s, u);
         else if (kind == 3) launch_polish_cols(h, c1, v.ld);
         else if (kind == 4) launch_polish_rows(h, c2, v.ld);
         else launch_step(h, kind == 0 ? 1 : 2);
