@@ -1225,6 +1225,7 @@ struct PolishRowsArgs {
     const int* sel;
     const int* act;          // [2][ld] CG activity (nullptr: both active)
     double* partial;         // [n_rb][2][ld]
+    int reverse;             // walk the LP tiles last to first
 };
 
 template <int LPL>
@@ -1232,6 +1233,7 @@ __global__ void __launch_bounds__(BLOCK) k_polish_rows(PolishRowsArgs a) {
     __shared__ double lds[WAVES_PER_BLOCK * 2 * WAVE * LPL];
     int tile, rb;
     if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
+    if (a.reverse) tile = a.sh.n_tiles - 1 - tile;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int b0 = tile * WAVE * LPL + lane * LPL;
     const bool valid = b0 < a.sh.ld;
