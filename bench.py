@@ -77,6 +77,13 @@ def _polish_rows_bytes(B, m, n, nnz, nvar, n_rb):
     return B * per_lp + nnz * 16 + (m + 1) * 4
 
 
+def _cg_direction_bytes(B, m):
+    """One CG direction update p = D^-1 r + beta p, both systems, every LP
+    active: r1, p1, r2, p2 and the Jacobi diagonal read, p1, p2 written
+    (7 m FP64 words per LP), beta / activity / selection per LP."""
+    return B * (7 * m * 8 + 2 * 8 + 12)
+
+
 def _cg_step_bytes(m, n, nnz, nvar, n_rb, n_rb_update):
     """Algorithmic bytes of one polish CG step (both systems): (per active LP,
     shared per step) - the columns and rows passes, the CG update (13 m words)
@@ -530,6 +537,7 @@ def main():
     ms_cg = eng.time_kernel("cg_update", reps=20)
     ms_pc = eng.time_kernel("polish_cols", reps=20)
     ms_pr = eng.time_kernel("polish_rows", reps=20)
+    ms_cd = eng.time_kernel("cg_direction", reps=20)
     lpl = int((target_tuning or {}).get("lps_per_lane", 1))
     pb = _primal_bytes(B, m, n, nnz, nvar, 4)
     db = _dual_bytes(B, m, n, nnz, nvar, 4)
@@ -554,7 +562,8 @@ def main():
     pcb = _polish_cols_bytes(B, m, n, nnz, nvar)
     prb = _polish_rows_bytes(B, m, n, nnz, nvar, _polish_row_blocks(m, B))
     for name, nbytes, ms, key in (("k_polish_cols<2>", pcb, ms_pc, "polish_cols_hbm_bytes"),
-                                  ("k_polish_rows<2>", prb, ms_pr, "polish_rows_hbm_bytes")):
+                                  ("k_polish_rows<2>", prb, ms_pr, "polish_rows_hbm_bytes"),
+                                  ("k_cg_direction", _cg_direction_bytes(B, m), ms_cd, "cg_direction_hbm_bytes")):
         kernels[name] = dict(algorithmic_bytes_per_launch=nbytes, avg_launch_ms=round(ms, 5),
                              achieved=round(nbytes / (ms * 1e-3) / 1e9, 1), launches_per_step=mean_cg,
                              ms_per_step=round(mean_cg * ms, 3),

@@ -757,10 +757,12 @@ void cg_steps(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, in
         cs.mode = 2; cs.d0 = h->d_red_a; cs.d1 = h->d_red_a + v.ld; cs.plain = h->d_red_a + 2 * (long)v.ld;
         hipLaunchKernelGGL(k_cg_scalars, dim3(ceil_div(v.ld, 256)), dim3(256), 0, h->s, cs);
         CgDirArgs dd;
-        dd.rows = h->m; dd.ld = v.ld; dd.rows_per_block = ROWS_PER_BLOCK_EW; dd.p1 = h->d_pp; dd.r1 = h->d_rp_;
+        int grid_d = 0;
+        dd.sh = make_shape(std::max(h->m, 1), v.ld, v.ld, 1, 1, &grid_d);
+        dd.p1 = h->d_pp; dd.r1 = h->d_rp_;
         dd.p2 = h->d_pd; dd.r2 = h->d_rd; dd.beta = h->d_beta; dd.invdiag = h->d_invdiag; dd.sel = v.sel;
         dd.act = h->d_cgact;
-        hipLaunchKernelGGL(k_cg_direction, ew_grid(h->m, v.ld), dim3(256), 0, h->s, dd);
+        hipLaunchKernelGGL(k_cg_direction, dim3(grid_d), dim3(BLOCK), 0, h->s, dd);
     }
 }
 
@@ -2036,9 +2038,9 @@ int lpe_time_iterations(lpe_handle* h, int64_t iterations, int32_t chunk, int32_
 
 int lpe_time_kernel(lpe_handle* h, int32_t kind, int32_t reps, double* ms_out, void* stream) {
     if (!h || !ms_out) return fail(LPE_EINVAL, "handle and ms are required");
-    if (kind < 0 || kind > 4)
+    if (kind < 0 || kind > 5)
         return fail(LPE_EINVAL, "kind must be 0 (dual step), 1 (primal step), 2 (CG update), 3 (polish columns "
-                                "pass) or 4 (polish rows pass)");
+                                "pass), 4 (polish rows pass) or 5 (CG direction)");
     if (reps < 1) return fail(LPE_EINVAL, "reps must be positive");
     if (!h->ready) return fail(LPE_ESTATE, "reset is required after a problem update");
     if (kind != 1 && h->m == 0) return fail(LPE_EINVAL, "no rows");
@@ -2046,7 +2048,8 @@ int lpe_time_kernel(lpe_handle* h, int32_t kind, int32_t reps, double* ms_out, v
     CgUpdateArgs u;
     PolishColsArgs c1;
     PolishRowsArgs c2;
-    int grid_r = 0, grid_c = 0, grid_u = 0;
+    CgDirArgs dd;
+    int grid_r = 0, grid_c = 0, grid_u = 0, grid_d = 0;
     const View v = full_view(h);
     if (kind >= 2) {
         // every LP selected, both CG systems active, on the full view
@@ -2065,12 +2068,16 @@ int lpe_time_kernel(lpe_handle* h, int32_t kind, int32_t reps, double* ms_out, v
         u.w2 = h->d_wd; u.r2 = h->d_rd; u.p2 = h->d_pd; u.s2 = h->d_s2; u.alpha = h->d_alpha;
         u.invdiag = h->d_invdiag; u.dr = LPE_CG_PRIMAL_WEIGHTED ? h->d_dr : nullptr; u.sel = v.sel;
         u.act = h->d_cgact; u.partial = h->d_partial;
+        dd.sh = make_shape(h->m, v.ld, v.ld, 1, 1, &grid_d);
+        dd.p1 = h->d_pp; dd.r1 = h->d_rp_; dd.p2 = h->d_pd; dd.r2 = h->d_rd; dd.beta = h->d_beta;
+        dd.invdiag = h->d_invdiag; dd.sel = v.sel; dd.act = h->d_cgact;
     }
     HIPCHK(hipEventRecord(h->ev_a, h->s));
     for (int i = 0; i < reps; ++i) {
         if (kind == 2) hipLaunchKernelGGL(k_cg_update, dim3(grid_u), dim3(BLOCK), 0, h->s, u);
         else if (kind == 3) launch_polish_cols(h, c1, v.ld);
         else if (kind == 4) launch_polish_rows(h, c2, v.ld);
+        else if (kind == 5) hipLaunchKernelGGL(k_cg_direction, dim3(grid_d), dim3(BLOCK), 0, h->s, dd);
         else launch_step(h, kind == 0 ? 1 : 2);
     }
     HIPCHK(hipEventRecord(h->ev_b, h->s));

@@ -1400,7 +1400,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_update(CgUpdateArgs a) {
 
 // p = r + beta p (two systems).
 struct CgDirArgs {
-    int rows, ld, rows_per_block;
+    Shape sh;               // rows = m; one row per wave, 64 LPs per wave (the CG update's layout)
     double* p1; const double* r1; double* p2; const double* r2;
     const double* beta;
     const double* invdiag;
@@ -1408,17 +1408,23 @@ struct CgDirArgs {
     const int* act;
 };
 
-__global__ void k_cg_direction(CgDirArgs a) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= a.ld || !a.sel[b]) return;
-    const long ld = a.ld;
+// Launched with the CG update's block -> (LP tile, row block) mapping, so each
+// wave streams whole 512-byte row segments of the five vectors it touches.
+__global__ void __launch_bounds__(BLOCK) k_cg_direction(CgDirArgs a) {
+    int tile, rb;
+    if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int b = tile * WAVE + lane;
+    if (b >= a.sh.ld || !a.sel[b]) return;
+    const long ld = a.sh.ld;
     const bool on1 = a.act[b], on2 = a.act[ld + b];
     if (!on1 && !on2) return;
     const double be1 = a.beta[b], be2 = a.beta[ld + b];
-    const int r0 = blockIdx.y * a.rows_per_block;
-    const int r1 = min(a.rows, r0 + a.rows_per_block);
-    for (int r = r0; r < r1; ++r) {
-        const long i = r * ld + b;
+    const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
+    for (int r = 0; r < a.sh.rpw; ++r) {
+        const int row = r0 + r;
+        if (row >= a.sh.rows) break;
+        const long i = row * ld + b;
         const double iv = a.invdiag[i];
         if (on1) a.p1[i] = a.r1[i] * iv + be1 * a.p1[i];
         if (on2) a.p2[i] = a.r2[i] * iv + be2 * a.p2[i];

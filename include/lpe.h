@@ -230,7 +230,8 @@ int lpe_time_iterations(lpe_handle *h, int64_t iterations, int32_t chunk, int32_
  * SpMV + dual update), 1 = primal step (fused A^T y SpMV + primal update +
  * extrapolation + averaging), 2 = the polish's preconditioned-CG update,
  * 3 = the polish's columns pass q = D A^T p, 4 = its rows pass s = R A q with
- * the p.s partials (2-4: both CG systems, every LP of the batch active; the
+ * the p.s partials, 5 = the CG direction p = D^-1 r + beta p (2-5: both CG
+ * systems, every LP of the batch active; the
  * polish scratch vectors are overwritten).  The iterate is advanced; reset
  * before solving again.  No
  * reference counterpart (the reference times whole graph chunks,
