@@ -248,6 +248,7 @@ def probe_kernels(args):
     ms_cg = eng.time_kernel("cg_update", reps=args.probe_reps)
     ms_pc = eng.time_kernel("polish_cols", reps=args.probe_reps)
     ms_pr = eng.time_kernel("polish_rows", reps=args.probe_reps)
+    ms_cd = eng.time_kernel("cg_direction", reps=args.probe_reps)
     n_copy = 256 << 20                                   # 256 Mi float32 = 1 GiB read + 1 GiB written
     src = torch.ones(n_copy, dtype=torch.float32, device=dev)
     dst = torch.empty_like(src)
@@ -258,7 +259,8 @@ def probe_kernels(args):
     print(json.dumps({"probe": "step kernels", "envs": args.envs, "rows_per_wave": args.rows_per_wave,
                       "lps_per_lane": args.lps_per_lane, "reps": args.probe_reps,
                       "primal_ms": ms_p, "dual_ms": ms_d, "cg_update_ms": ms_cg,
-                      "polish_cols_ms": ms_pc, "polish_rows_ms": ms_pr,
+                      "polish_cols_ms": ms_pc, "polish_rows_ms": ms_pr, "cg_direction_ms": ms_cd,
+                      "cg_direction_algorithmic_bytes": _cg_direction_bytes(args.envs, m),
                       "cg_update_algorithmic_bytes": _cg_update_bytes(args.envs, m, _update_row_blocks(m)),
                       "polish_cols_algorithmic_bytes": _polish_cols_bytes(args.envs, m, n, model.A.nnz,
                                                                           model.var_positions.size),
