@@ -50,8 +50,8 @@ def _cg_update_bytes(B, m, n_rb, weighted=False):
 
 
 def _update_row_blocks(m):
-    """Row blocks of the CG update launch (4 waves per block, one row per wave)."""
-    return max(1, -(-m // 4))
+    """Row blocks of the CG update launch (16 waves per block, one row per wave)."""
+    return max(1, -(-m // 16))
 
 
 def _polish_row_blocks(m, ld):

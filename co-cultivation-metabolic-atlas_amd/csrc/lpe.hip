@@ -57,6 +57,20 @@ constexpr int ROWS_PER_BLOCK_EW = 16;   // rows per block of elementwise kernels
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 
+// As make_shape, for a kernel with `waves` waves per block.
+Shape make_shape_waves(int rows, int ld, int batch, int rpw, int lpl, int waves, int* grid) {
+    Shape s;
+    s.rows = rows;
+    s.ld = ld;
+    s.batch = batch;
+    s.rpw = rpw;
+    s.n_tiles = ceil_div(ld, WAVE * lpl);
+    s.n_rb = std::max(1, ceil_div(rows, waves * rpw));
+    s.n_real = s.n_tiles * s.n_rb;
+    *grid = ((s.n_real + 7) / 8) * 8;
+    return s;
+}
+
 Shape make_shape(int rows, int ld, int batch, int rpw, int lpl, int* grid) {
     Shape s;
     s.rows = rows;
@@ -747,12 +761,12 @@ void cg_steps(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, in
         hipLaunchKernelGGL(k_cg_scalars, dim3(ceil_div(v.ld, 256)), dim3(256), 0, h->s, cs);
         CgUpdateArgs u;
         int grid_u = 0;
-        u.sh = make_shape(std::max(h->m, 1), v.ld, v.ld, cg_update_rpw(h, v.ld), 1, &grid_u);
+        u.sh = make_shape_waves(std::max(h->m, 1), v.ld, v.ld, cg_update_rpw(h, v.ld), 1, CG_UPDATE_WAVES, &grid_u);
         u.w1 = h->d_wp; u.r1 = h->d_rp_; u.p1 = h->d_pp; u.s1 = h->d_s1;
         u.w2 = h->d_wd; u.r2 = h->d_rd; u.p2 = h->d_pd; u.s2 = h->d_s2; u.alpha = h->d_alpha;
         u.invdiag = h->d_invdiag; u.dr = LPE_CG_PRIMAL_WEIGHTED ? h->d_dr : nullptr; u.sel = v.sel;
         u.act = h->d_cgact; u.partial = h->d_partial;
-        hipLaunchKernelGGL(k_cg_update, dim3(grid_u), dim3(BLOCK), 0, h->s, u);
+        hipLaunchKernelGGL(k_cg_update, dim3(grid_u), dim3(CG_UPDATE_WAVES * WAVE), 0, h->s, u);
         reduce_ld(h, h->d_partial, h->d_red_a, h->d_ops_sum, 3, u.sh.n_rb, v.ld);
         cs.mode = 2; cs.d0 = h->d_red_a; cs.d1 = h->d_red_a + v.ld; cs.plain = h->d_red_a + 2 * (long)v.ld;
         hipLaunchKernelGGL(k_cg_scalars, dim3(ceil_div(v.ld, 256)), dim3(256), 0, h->s, cs);
@@ -2063,7 +2077,7 @@ int lpe_time_kernel(lpe_handle* h, int32_t kind, int32_t reps, double* ms_out, v
         c2.A = op_scaled_v(h, false, v); c2.q1 = h->d_q1; c2.q2 = h->d_q2; c2.s1 = h->d_s1; c2.s2 = h->d_s2;
         c2.p1 = h->d_pp; c2.p2 = h->d_pd; c2.setup_r = nullptr; c2.setup_p = nullptr; c2.invdiag = h->d_invdiag;
         c2.rmask = h->d_rmask; c2.sel = v.sel; c2.act = h->d_cgact; c2.partial = h->d_partial; c2.reverse = rows_reverse();
-        u.sh = make_shape(h->m, v.ld, v.ld, cg_update_rpw(h, v.ld), 1, &grid_u);
+        u.sh = make_shape_waves(h->m, v.ld, v.ld, cg_update_rpw(h, v.ld), 1, CG_UPDATE_WAVES, &grid_u);
         u.w1 = h->d_wp; u.r1 = h->d_rp_; u.p1 = h->d_pp; u.s1 = h->d_s1;
         u.w2 = h->d_wd; u.r2 = h->d_rd; u.p2 = h->d_pd; u.s2 = h->d_s2; u.alpha = h->d_alpha;
         u.invdiag = h->d_invdiag; u.dr = LPE_CG_PRIMAL_WEIGHTED ? h->d_dr : nullptr; u.sel = v.sel;
@@ -2074,7 +2088,7 @@ int lpe_time_kernel(lpe_handle* h, int32_t kind, int32_t reps, double* ms_out, v
     }
     HIPCHK(hipEventRecord(h->ev_a, h->s));
     for (int i = 0; i < reps; ++i) {
-        if (kind == 2) hipLaunchKernelGGL(k_cg_update, dim3(grid_u), dim3(BLOCK), 0, h->s, u);
+        if (kind == 2) hipLaunchKernelGGL(k_cg_update, dim3(grid_u), dim3(CG_UPDATE_WAVES * WAVE), 0, h->s, u);
         else if (kind == 3) launch_polish_cols(h, c1, v.ld);
         else if (kind == 4) launch_polish_rows(h, c2, v.ld);
         else if (kind == 5) hipLaunchKernelGGL(k_cg_direction, dim3(grid_d), dim3(BLOCK), 0, h->s, dd);

@@ -369,7 +369,7 @@ __global__ void __launch_bounds__(BLOCK) k_primal_step(PrimalArgs<T> a) {
 // ---------------------------------------------------------------------------
 enum RedOp { RSUM = 0, RMAX = 1 };
 
-template <int NV, int LPL>
+template <int NV, int LPL, int WPB = WAVES_PER_BLOCK>
 __device__ __forceinline__ void block_partials(double (&val)[NV][LPL], const int (&ops)[NV],
                                                double* __restrict__ partial, int rb, long ld, int b0,
                                                double* lds, bool valid) {
@@ -387,7 +387,7 @@ __device__ __forceinline__ void block_partials(double (&val)[NV][LPL], const int
 #pragma unroll
             for (int t = 0; t < LPL; ++t) {
                 double acc = lds[(0 * NV + v) * W + lane * LPL + t];
-                for (int w = 1; w < WAVES_PER_BLOCK; ++w) {
+                for (int w = 1; w < WPB; ++w) {
                     const double o = lds[(w * NV + v) * W + lane * LPL + t];
                     acc = ops[v] == RSUM ? acc + o : (o > acc || __builtin_isnan(o) ? o : acc);
                 }
@@ -1360,8 +1360,12 @@ struct CgUpdateArgs {
     double* partial;        // [n_rb][3][ld]: r_p.D^-1 r_p, r_d.D^-1 r_d, |r_p|^2
 };
 
-__global__ void __launch_bounds__(BLOCK) k_cg_update(CgUpdateArgs a) {
-    __shared__ double lds[WAVES_PER_BLOCK * 3 * WAVE];
+// 16 waves per block, one row per wave: each block covers 16 rows, so the
+// block partials of the three dots are a quarter of a 4-wave block's.
+constexpr int CG_UPDATE_WAVES = 16;
+
+__global__ void __launch_bounds__(CG_UPDATE_WAVES * WAVE) k_cg_update(CgUpdateArgs a) {
+    __shared__ double lds[CG_UPDATE_WAVES * 3 * WAVE];
     int tile, rb;
     if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
@@ -1372,7 +1376,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_update(CgUpdateArgs a) {
     const bool on1 = valid && a.act[b0], on2 = valid && a.act[ld + b0];
     if (valid && a.sel[b0] && (on1 || on2)) {
         const double al1 = a.alpha[b0], al2 = a.alpha[ld + b0];
-        const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
+        const int r0 = (rb * CG_UPDATE_WAVES + wave) * a.sh.rpw;
         for (int r = 0; r < a.sh.rpw; ++r) {
             const int row = r0 + r;
             if (row >= a.sh.rows) break;
@@ -1395,7 +1399,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_update(CgUpdateArgs a) {
         }
     }
     const int ops[3] = {RSUM, RSUM, RSUM};
-    block_partials<3, 1>(v, ops, a.partial, rb, ld, b0, lds, valid);
+    block_partials<3, 1, CG_UPDATE_WAVES>(v, ops, a.partial, rb, ld, b0, lds, valid);
 }
 
 // p = r + beta p (two systems).
