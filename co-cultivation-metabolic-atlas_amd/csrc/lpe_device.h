@@ -1410,6 +1410,7 @@ struct CgDirArgs {
     const double* invdiag;
     const int* sel;
     const int* act;
+    int reverse;            // walk the LP tiles last to first
 };
 
 // Launched with the CG update's block -> (LP tile, row block) mapping, so each
@@ -1417,6 +1418,7 @@ struct CgDirArgs {
 __global__ void __launch_bounds__(BLOCK) k_cg_direction(CgDirArgs a) {
     int tile, rb;
     if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
+    if (a.reverse) tile = a.sh.n_tiles - 1 - tile;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int b = tile * WAVE + lane;
     if (b >= a.sh.ld || !a.sel[b]) return;
