@@ -332,6 +332,8 @@ Iter<float>& iter_of<float>(lpe_handle* h) { return h->it32; }
 #define LPE_HALPERN_REFLECTION 1.0
 #endif
 
+int dual_reverse();
+
 template <typename T, int LPL, bool REF>
 void launch_step_lpl(lpe_handle* h, int which, int j, int store_t) {
     Iter<T>& it = iter_of<T>(h);
@@ -353,6 +355,7 @@ void launch_step_lpl(lpe_handle* h, int which, int j, int store_t) {
         if (h->m > 0 && (which & 1)) {
             int grid;
             DualArgs<T> d;
+        d.reverse = dual_reverse();
             d.sh = make_shape(h->m, h->ld, h->B, h->rpw, LPL, &grid);
             d.A = op_iter<T>(h, false);
             d.xbar = it.xbar; d.y = it.y; d.rhs = it.rhs; d.sigma = it.sigma;
@@ -366,6 +369,7 @@ void launch_step_lpl(lpe_handle* h, int which, int j, int store_t) {
     if (h->m > 0 && (which & 1)) {
         int grid;
         DualArgs<T> d;
+        d.reverse = dual_reverse();
         d.sh = make_shape(h->m, h->ld, h->B, h->rpw, LPL, &grid);
         d.A = op_iter<T>(h, false);
         d.xbar = it.xbar; d.y = it.y; d.rhs = it.rhs;
@@ -520,6 +524,16 @@ int rows_reverse() {
 // 73.9 ms per polish, identical results.  LPE_CG_DIR_REVERSE=0 turns it off.
 int dir_reverse() {
     const char* v = std::getenv("LPE_CG_DIR_REVERSE");
+    return (v && v[0] == '0') ? 0 : 1;
+}
+
+// The PDHG dual step walks the LP tiles last to first, after the primal
+// step's forward walk, so its x-bar gathers start on the tiles written last.
+// Measured at 4096 LPs (FP32, 2 rows/wave x 4 LPs/lane): 0.2587 against 0.2620
+// ms per iteration, identical results.  LPE_DUAL_REVERSE=0 (read when an
+// iteration chunk is launched or captured) turns it off.
+int dual_reverse() {
+    const char* v = std::getenv("LPE_DUAL_REVERSE");
     return (v && v[0] == '0') ? 0 : 1;
 }
 

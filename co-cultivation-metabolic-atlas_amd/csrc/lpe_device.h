@@ -158,6 +158,7 @@ struct DualArgs {
     const int* hk;       // [ld] Halpern iterations since the restart, before this launch group
     int j;               // iteration index within the launch group
     int store_t;         // 1: also store y+ = T(z)_y into ysum; 2: only that (probe)
+    int reverse = 0;     // walk the LP tiles last to first
     T refl;
 };
 
@@ -169,6 +170,7 @@ template <typename T, int LPL, bool REF, bool HAL = false>
 __global__ void __launch_bounds__(BLOCK) k_dual_step(DualArgs<T> a) {
     int tile, rb;
     if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
+    if (a.reverse) tile = a.sh.n_tiles - 1 - tile;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int b0 = tile * WAVE * LPL + lane * LPL;
     if (b0 >= a.sh.ld) return;
