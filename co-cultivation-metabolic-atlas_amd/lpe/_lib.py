@@ -31,7 +31,8 @@ EXPORTS = ("lpe_create", "lpe_update", "lpe_reset", "lpe_run", "lpe_set_accepted
            "lpe_last_error", "lpe_version",
            "lpe_reduction_create", "lpe_reduction_get_dims", "lpe_reduction_get_plan",
            "lpe_reduction_get_pattern", "lpe_reduction_get_lift_map", "lpe_reduction_reduce",
-           "lpe_reduction_expand", "lpe_reduction_compress", "lpe_reduction_lift", "lpe_reduction_destroy")
+           "lpe_reduction_expand", "lpe_reduction_compress", "lpe_reduction_lift", "lpe_reduction_destroy",
+           "lpe_certify_noalloc", "lpe_certify_noalloc_capacity", "lpe_certify_noalloc_need")
 
 RED_INFEASIBLE, RED_OVERFLOW, RED_INHOMOGENEOUS = 1, 2, 4
 
@@ -71,6 +72,15 @@ class LpeSolveInfo(C.Structure):
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class LpeBatchProblem(C.Structure):
+    _fields_ = [("m", C.c_int32), ("n", C.c_int32), ("neq", C.c_int32), ("batch", C.c_int32),
+                ("nnz", C.c_int64),
+                ("row_ptr", C.c_void_p), ("col_idx", C.c_void_p), ("col_ptr", C.c_void_p),
+                ("row_idx", C.c_void_p), ("t_pos", C.c_void_p), ("values", C.c_void_p),
+                ("values_per_lp", C.c_int32), ("c_per_lp", C.c_int32), ("c", C.c_void_p),
+                ("rhs", C.c_void_p), ("lower", C.c_void_p), ("upper", C.c_void_p)]
 
 
 class LpeReductionDesc(C.Structure):
@@ -139,6 +149,9 @@ def load() -> C.CDLL:
         "lpe_reduction_compress": ([P, I32, P, P, I32, P], C.c_int),
         "lpe_reduction_lift": ([P, P, I32, I32, P, P, P, P, P, P, P, P, I32, P], C.c_int),
         "lpe_reduction_destroy": ([P], C.c_int),
+        "lpe_certify_noalloc": ([C.POINTER(LpeBatchProblem), P, P, P, D, D, D, P], C.c_int),
+        "lpe_certify_noalloc_capacity": ([], I64),
+        "lpe_certify_noalloc_need": ([I32, I32], I64),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -178,6 +178,41 @@ int lpe_solve(lpe_handle *h, const lpe_solve_opts *opts, lpe_solve_info *info, v
 int lpe_certify(lpe_handle *h, const double *x, const double *y, double *metrics,
                 double eps_primal, double eps_dual, double eps_gap, void *stream);
 
+/* A caller-owned batch of B LPs sharing one sparsity pattern, all device
+ * pointers (batch-major per-LP arrays, FP64 values, int32 indices). */
+typedef struct lpe_batch_problem {
+    int32_t m, n, neq, batch;
+    int64_t nnz;
+    const int32_t *row_ptr;   /* [m+1] CSR of A, sorted columns per row       */
+    const int32_t *col_idx;   /* [nnz]                                        */
+    const int32_t *col_ptr;   /* [n+1] transposed pattern (CSR of A^T) ...    */
+    const int32_t *row_idx;   /* [nnz] ... rows ascending inside each column  */
+    const int32_t *t_pos;     /* [nnz] position in the CSR values of each A^T entry */
+    const double *values;     /* [nnz], or [B][nnz] when values_per_lp        */
+    int32_t values_per_lp;
+    int32_t c_per_lp;         /* c is [n] (0) or [B][n] (1)                   */
+    const double *c;
+    const double *rhs;        /* [B][m]                                       */
+    const double *lower;      /* [B][n]  (-inf allowed)                       */
+    const double *upper;      /* [B][n]  (+inf allowed)                       */
+} lpe_batch_problem;
+
+/* No counterpart in the reference: the variant of lpe_certify for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same FP64 certificate and gates as
+ * lpe_certify, of (x [B][n], y [B][m]) for a caller-owned problem, with no
+ * handle; metrics [B][LPE_NMETRIC] on the device.  Its block partials live
+ * in a static device array of lpe_certify_noalloc_capacity() doubles (8 MiB)
+ * per GPU, so it calls no hipMalloc/hipFree and touches no memory but its
+ * arguments and that array; a batch needing more than the array holds
+ * (batch * lpe_certify_noalloc_need(m, n) doubles) is rejected with
+ * LPE_EINVAL before any device work.  Enqueued on `stream`, not
+ * synchronised.  Calls that overlap on one GPU race on the shared array. */
+int lpe_certify_noalloc(const lpe_batch_problem *p, const double *x, const double *y, double *metrics,
+                        double eps_primal, double eps_dual, double eps_gap, void *stream);
+int64_t lpe_certify_noalloc_capacity(void);
+int64_t lpe_certify_noalloc_need(int32_t m, int32_t n);
+
 /* Replaces GpuPdhgWorkspace.state (src/gpu_pdhg_workspace.py:369-380) with
  * owned copies: x [B][n], y [B][m] in original units (FP64), status [B]. */
 int lpe_state(lpe_handle *h, double *x, double *y, int32_t *status, void *stream);

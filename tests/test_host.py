@@ -142,7 +142,7 @@ def test_ctypes_structs_match_the_c_layout(tmp_path):
     structs = {"lpe_desc": _lib.LpeDesc, "lpe_vectors": _lib.LpeVectors,
                "lpe_solve_opts": _lib.LpeSolveOpts, "lpe_solve_info": _lib.LpeSolveInfo,
                "lpe_reduction_desc": _lib.LpeReductionDesc, "lpe_reduction_dims": _lib.LpeReductionDims,
-               "lpe_reduced": _lib.LpeReduced}
+               "lpe_reduced": _lib.LpeReduced, "lpe_batch_problem": _lib.LpeBatchProblem}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "lpe_reduction.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
