@@ -147,6 +147,7 @@ struct lpe_handle {
     size_t partial_elems = 0;
     double *d_red_a = nullptr, *d_red_b = nullptr;
     int *d_ops_cr = nullptr, *d_ops_cc = nullptr, *d_ops_xr = nullptr, *d_ops_xc = nullptr, *d_ops_sum = nullptr;
+    int *d_ops_pinf = nullptr, *d_ops_pinit = nullptr;
 
     // polish / results
     double *d_xs = nullptr, *d_rp_ = nullptr, *d_pp = nullptr, *d_wp = nullptr;
@@ -498,14 +499,30 @@ int polish_lpl(int ld) {
     return ld >= 512 ? 2 : 1;
 }
 
+// LPE_POLISH_SPLIT=1: the CG gather passes run one launch per CG system
+// (half the gathered vector per LP tile in flight), bit-identical results.
+int polish_split() { return env_choice("LPE_POLISH_SPLIT", 1, 1); }
+
+void launch_polish_cols_one(lpe_handle* h, const PolishColsArgs& b, int grid, int lpl) {
+    if (lpl == 4) hipLaunchKernelGGL(k_polish_cols<4>, dim3(grid), dim3(BLOCK), 0, h->s, b);
+    else if (lpl == 2) hipLaunchKernelGGL(k_polish_cols<2>, dim3(grid), dim3(BLOCK), 0, h->s, b);
+    else hipLaunchKernelGGL(k_polish_cols<1>, dim3(grid), dim3(BLOCK), 0, h->s, b);
+}
+
 void launch_polish_cols(lpe_handle* h, const PolishColsArgs& a, int ldv) {
     int grid;
     PolishColsArgs b = a;
     const int lpl = polish_lpl(ldv);
     b.sh = make_shape(a.sh.rows, ldv, ldv, a.sh.rpw, lpl, &grid);
-    if (lpl == 4) hipLaunchKernelGGL(k_polish_cols<4>, dim3(grid), dim3(BLOCK), 0, h->s, b);
-    else if (lpl == 2) hipLaunchKernelGGL(k_polish_cols<2>, dim3(grid), dim3(BLOCK), 0, h->s, b);
-    else hipLaunchKernelGGL(k_polish_cols<1>, dim3(grid), dim3(BLOCK), 0, h->s, b);
+    if (polish_split() && !a.c && a.p1 && a.p2) {
+        PolishColsArgs b1 = b, b2 = b;
+        b1.p2 = nullptr; b1.q2 = nullptr;
+        b2.p1 = nullptr; b2.q1 = nullptr;
+        launch_polish_cols_one(h, b1, grid, lpl);
+        launch_polish_cols_one(h, b2, grid, lpl);
+        return;
+    }
+    launch_polish_cols_one(h, b, grid, lpl);
 }
 
 // The CG rows pass walks the LP tiles in the reverse of the columns pass's
@@ -538,14 +555,27 @@ int dual_reverse() {
 }
 
 // Returns the number of row blocks (partials per LP and value) of the launch.
+void launch_polish_rows_one(lpe_handle* h, const PolishRowsArgs& b, int grid, int lpl) {
+    if (lpl == 4) hipLaunchKernelGGL(k_polish_rows<4>, dim3(grid), dim3(BLOCK), 0, h->s, b);
+    else if (lpl == 2) hipLaunchKernelGGL(k_polish_rows<2>, dim3(grid), dim3(BLOCK), 0, h->s, b);
+    else hipLaunchKernelGGL(k_polish_rows<1>, dim3(grid), dim3(BLOCK), 0, h->s, b);
+}
+
 int launch_polish_rows(lpe_handle* h, const PolishRowsArgs& a, int ldv) {
     int grid;
     PolishRowsArgs b = a;
     const int lpl = polish_lpl(ldv);
     b.sh = make_shape(a.sh.rows, ldv, ldv, a.sh.rpw, lpl, &grid);
-    if (lpl == 4) hipLaunchKernelGGL(k_polish_rows<4>, dim3(grid), dim3(BLOCK), 0, h->s, b);
-    else if (lpl == 2) hipLaunchKernelGGL(k_polish_rows<2>, dim3(grid), dim3(BLOCK), 0, h->s, b);
-    else hipLaunchKernelGGL(k_polish_rows<1>, dim3(grid), dim3(BLOCK), 0, h->s, b);
+    if (polish_split() && !a.setup_r && a.q1 && a.q2) {
+        // each launch writes its own partial slot; one reduction follows
+        PolishRowsArgs b1 = b, b2 = b;
+        b1.q2 = nullptr; b1.only = 0;
+        b2.q1 = nullptr; b2.only = 1;
+        launch_polish_rows_one(h, b1, grid, lpl);
+        launch_polish_rows_one(h, b2, grid, lpl);
+        return b.sh.n_rb;
+    }
+    launch_polish_rows_one(h, b, grid, lpl);
     return b.sh.n_rb;
 }
 
@@ -763,11 +793,34 @@ int cg_block_steps() {
 #endif
 constexpr double CG_REL = LPE_CG_REL;   // rho = r.D^-1 r reduction at which a system stops
 
+// Primal CG stop measure.  Default: the infinity norm of the active rows'
+// residual in original units, max_i |b - A x|_i <= LPE_CG_PRIMAL_TOL - the
+// norm of the certificate's primal gate (1e-5), so the tolerance keeps a
+// factor 3 to it; measured on the 4096-environment consortium step it stops
+// the primal system after 30-40 CG steps where the 2-norm bound below needs
+// 50-60.  LPE_CG_PRIMAL_NORM=2 selects the 2-norm bound (LPE_CG_PRIMAL_WEIGHTED
+// as above); LPE_CG_PTOL overrides the tolerance, both for sweeps.
+bool cg_primal_inf() {
+    const char* v = std::getenv("LPE_CG_PRIMAL_NORM");
+    return !(v && v[0] == '2');
+}
+double cg_primal_tol() {
+    const char* v = std::getenv("LPE_CG_PTOL");
+    const double t = v ? std::atof(v) : 0.0;
+    return (t > 0.0 && t < 1.0) ? t : LPE_CG_PRIMAL_TOL;
+}
+double cg_ptol2_h(const lpe_handle* h) {
+    if (!cg_primal_inf()) return h->cg_ptol2;
+    const double t = cg_primal_tol();
+    return t * t;
+}
+
 void cg_steps(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, int grid_r, int grid_c, int steps) {
     CgScalarArgs cs;
     cs.rho = h->d_rho; cs.rho0 = h->d_rho0; cs.alpha = h->d_alpha; cs.beta = h->d_beta;
     cs.act = h->d_cgact; cs.floor2 = h->d_cgfloor; cs.sel = v.sel; cs.rel = CG_REL; cs.batch = v.ld; cs.ld = v.ld;
-    cs.ptol2 = h->cg_ptol2;
+    cs.ptol2 = cg_ptol2_h(h);
+    const bool pinf = cg_primal_inf();
     for (int k = 0; k < steps; ++k) {
         PolishColsArgs c1;
         c1.sh = sc; c1.AT = op_scaled_v(h, true, v); c1.p1 = h->d_pp; c1.p2 = h->d_pd; c1.q1 = h->d_q1;
@@ -787,10 +840,10 @@ void cg_steps(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, in
         u.sh = make_shape_waves(std::max(h->m, 1), v.ld, v.ld, cg_update_rpw(h, v.ld), 1, CG_UPDATE_WAVES, &grid_u);
         u.w1 = h->d_wp; u.r1 = h->d_rp_; u.p1 = h->d_pp; u.s1 = h->d_s1;
         u.w2 = h->d_wd; u.r2 = h->d_rd; u.p2 = h->d_pd; u.s2 = h->d_s2; u.alpha = h->d_alpha;
-        u.invdiag = h->d_invdiag; u.dr = LPE_CG_PRIMAL_WEIGHTED ? h->d_dr : nullptr; u.sel = v.sel;
-        u.act = h->d_cgact; u.partial = h->d_partial;
+        u.invdiag = h->d_invdiag; u.dr = (pinf || LPE_CG_PRIMAL_WEIGHTED) ? h->d_dr : nullptr; u.sel = v.sel;
+        u.act = h->d_cgact; u.partial = h->d_partial; u.inf_norm = pinf ? 1 : 0;
         hipLaunchKernelGGL(k_cg_update, dim3(grid_u), dim3(CG_UPDATE_WAVES * WAVE), 0, h->s, u);
-        reduce_ld(h, h->d_partial, h->d_red_a, h->d_ops_sum, 3, u.sh.n_rb, v.ld);
+        reduce_ld(h, h->d_partial, h->d_red_a, pinf ? h->d_ops_pinf : h->d_ops_sum, 3, u.sh.n_rb, v.ld);
         cs.mode = 2; cs.d0 = h->d_red_a; cs.d1 = h->d_red_a + v.ld; cs.plain = h->d_red_a + 2 * (long)v.ld;
         hipLaunchKernelGGL(k_cg_scalars, dim3(ceil_div(v.ld, 256)), dim3(256), 0, h->s, cs);
         CgDirArgs dd;
@@ -812,7 +865,8 @@ int cg_block(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, int
     const long long key = (long long)v.ld | ((long long)sr.rpw << 32) | ((long long)polish_lpl(v.ld) << 40) |
                           ((long long)steps << 44) | (v.sel == h->d_sel ? 0LL : 1LL << 52) |
                           ((long long)cg_update_rpw(h, v.ld) << 54) | ((long long)rows_reverse() << 60) |
-                          ((long long)dir_reverse() << 61);
+                          ((long long)dir_reverse() << 61) | ((long long)polish_split() << 62) |
+                          ((long long)(cg_primal_inf() ? 1 : 0) << 51);
     hipGraphExec_t& g_exec = h->cg_graphs[key];
     if (!g_exec) {
         hipGraph_t g;
@@ -855,9 +909,12 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
         a.sh = sr; a.A = op_orig_v(h, false, v); a.As = op_scaled_v(h, false, v); a.xs = h->d_xs; a.y = yin;
         a.rhs = v.rhs; a.dr = h->d_dr; a.orig = orig; a.fmask = h->d_fmask; a.invdiag = h->d_invdiag;
         a.sel = v.sel; a.rmask = h->d_rmask; a.rp = h->d_rp_; a.pp = h->d_pp; a.wp = h->d_wp; a.wd = h->d_wd;
-        a.neq = h->neq; a.tol_y = 1e-9; a.tol_s = 1e-7; a.plain_orig = LPE_CG_PRIMAL_WEIGHTED; a.partial = h->d_partial;
+        const bool pinf = cg_primal_inf();
+        a.neq = h->neq; a.tol_y = 1e-9; a.tol_s = 1e-7; a.plain_orig = pinf ? 2 : LPE_CG_PRIMAL_WEIGHTED;
+        a.partial = h->d_partial;
         hipLaunchKernelGGL((k_polish_rows_init<T>), dim3(grid_r), dim3(BLOCK), 0, h->s, a);
-        reduce_ld(h, h->d_partial, h->d_red_b, h->d_ops_sum, 2, sr.n_rb, v.ld);   // rho_p, |r_p|^2
+        // rho_p and the primal stop measure (the pinf reductions' first two ops: sum, max)
+        reduce_ld(h, h->d_partial, h->d_red_b, pinf ? h->d_ops_pinit : h->d_ops_sum, 2, sr.n_rb, v.ld);
         // dual setup: q1 = D (c^ - A^T wd); r_d = R A q1, p_d = r_d / diag
         PolishColsArgs pc;
         pc.sh = sc; pc.AT = op_scaled_v(h, true, v); pc.p1 = h->d_wd; pc.p2 = nullptr; pc.q1 = h->d_q1; pc.q2 = nullptr;
@@ -874,7 +931,7 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
         CgScalarArgs cs;
         cs.rho = h->d_rho; cs.rho0 = h->d_rho0; cs.alpha = h->d_alpha; cs.beta = h->d_beta;
         cs.act = h->d_cgact; cs.floor2 = h->d_cgfloor; cs.sel = v.sel; cs.rel = CG_REL; cs.batch = v.ld; cs.ld = v.ld;
-        cs.mode = 0; cs.d0 = h->d_red_b; cs.d1 = h->d_red_a; cs.plain = h->d_red_b + ld; cs.ptol2 = h->cg_ptol2;
+        cs.mode = 0; cs.d0 = h->d_red_b; cs.d1 = h->d_red_a; cs.plain = h->d_red_b + ld; cs.ptol2 = cg_ptol2_h(h);
         hipLaunchKernelGGL(k_cg_floor, dim3(ceil_div(v.ld, 256)), dim3(256), 0, h->s, (const double*)h->d_bnorm,
                            (const double*)h->d_cnorm, v.sel == h->d_sel ? (const int*)nullptr : (const int*)h->d_perm,
                            h->d_cgfloor, v.ld);
@@ -936,7 +993,7 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
                                          b, rh[q * ld + b], r0[q * ld + b], fl[q * ld + b], pl[b]);
                     }
             std::fprintf(stderr, "[lpe]     cg still active: primal %d dual %d (ptol2 %.3e)\n", na[0], na[1],
-                         h->cg_ptol2);
+                         cg_ptol2_h(h));
         }
     }
     {
@@ -1263,6 +1320,10 @@ int alloc_all(lpe_handle* h) {
     CHK(dalloc(h, &h->d_ops_xr, XR_N)); CHK(upload(h->d_ops_xr, ops_xr, XR_N));
     CHK(dalloc(h, &h->d_ops_xc, XC_N)); CHK(upload(h->d_ops_xc, ops_xc, XC_N));
     CHK(dalloc(h, &h->d_ops_sum, 3)); CHK(upload(h->d_ops_sum, ops_sum, 3));
+    const int ops_pinf[3] = {RSUM, RSUM, RMAX};      // CG update: two rho, the primal residual maximum
+    const int ops_pinit[2] = {RSUM, RMAX};           // CG setup: rho_p, the primal residual maximum
+    CHK(dalloc(h, &h->d_ops_pinf, 3)); CHK(upload(h->d_ops_pinf, ops_pinf, 3));
+    CHK(dalloc(h, &h->d_ops_pinit, 2)); CHK(upload(h->d_ops_pinit, ops_pinit, 2));
     // polish and results
     CHK(dalloc(h, &h->d_xs, n * ld)); CHK(dalloc(h, &h->d_fmask, n * ld)); CHK(dalloc(h, &h->d_rmask, m * ld));
     CHK(dalloc(h, &h->d_rp_, m * ld)); CHK(dalloc(h, &h->d_pp, m * ld)); CHK(dalloc(h, &h->d_wp, m * ld));
@@ -2105,7 +2166,9 @@ int lpe_time_kernel(lpe_handle* h, int32_t kind, int32_t reps, double* ms_out, v
         u.sh = make_shape_waves(h->m, v.ld, v.ld, cg_update_rpw(h, v.ld), 1, CG_UPDATE_WAVES, &grid_u);
         u.w1 = h->d_wp; u.r1 = h->d_rp_; u.p1 = h->d_pp; u.s1 = h->d_s1;
         u.w2 = h->d_wd; u.r2 = h->d_rd; u.p2 = h->d_pd; u.s2 = h->d_s2; u.alpha = h->d_alpha;
-        u.invdiag = h->d_invdiag; u.dr = LPE_CG_PRIMAL_WEIGHTED ? h->d_dr : nullptr; u.sel = v.sel;
+        u.invdiag = h->d_invdiag; u.sel = v.sel;
+        u.inf_norm = cg_primal_inf() ? 1 : 0;
+        u.dr = (u.inf_norm || LPE_CG_PRIMAL_WEIGHTED) ? h->d_dr : nullptr;
         u.act = h->d_cgact; u.partial = h->d_partial;
         dd.sh = make_shape(h->m, v.ld, v.ld, 1, 1, &grid_d);
         dd.p1 = h->d_pp; dd.r1 = h->d_rp_; dd.p2 = h->d_pd; dd.r2 = h->d_rd; dd.beta = h->d_beta;

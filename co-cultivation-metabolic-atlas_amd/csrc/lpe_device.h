@@ -374,7 +374,7 @@ enum RedOp { RSUM = 0, RMAX = 1 };
 template <int NV, int LPL, int WPB = WAVES_PER_BLOCK>
 __device__ __forceinline__ void block_partials(double (&val)[NV][LPL], const int (&ops)[NV],
                                                double* __restrict__ partial, int rb, long ld, int b0,
-                                               double* lds, bool valid) {
+                                               double* lds, bool valid, int only = -1) {
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     // lds: [WAVES][NV][64*LPL]
     constexpr int W = WAVE * LPL;
@@ -393,7 +393,7 @@ __device__ __forceinline__ void block_partials(double (&val)[NV][LPL], const int
                     const double o = lds[(w * NV + v) * W + lane * LPL + t];
                     acc = ops[v] == RSUM ? acc + o : (o > acc || __builtin_isnan(o) ? o : acc);
                 }
-                partial[((long)rb * NV + v) * ld + b0 + t] = acc;
+                if (only < 0 || v == only) partial[((long)rb * NV + v) * ld + b0 + t] = acc;
             }
         }
     }
@@ -1094,8 +1094,9 @@ struct PolishRowsInitArgs {
     double* invdiag;         // [m][ld] 1 / (R A^ D A^T R)_ii, 0 on inactive rows
     int neq;
     double tol_y, tol_s;
-    int plain_orig;          // 1: |r|^2 in original units (r / dr), else scaled
-    double* partial;         // [n_rb][2][ld]: r.D^-1 r and |r|^2 of the primal system
+    int plain_orig;          // 1: |r|^2 in original units (r / dr); 2: max_i r_i^2 in original
+                             // units (the infinity norm of the active rows' residual, squared); 0: |r|^2 scaled
+    double* partial;         // [n_rb][2][ld]: r.D^-1 r and the primal stop measure
 };
 
 template <typename T>
@@ -1138,10 +1139,11 @@ __global__ void __launch_bounds__(BLOCK) k_polish_rows_init(PolishRowsInitArgs<T
             a.wd[i] = on ? yv : 0.0;
             v[0][0] += rv * rv * inv;
             const double ro = a.plain_orig ? (on ? res : 0.0) : rv;
-            v[1][0] += ro * ro;
+            if (a.plain_orig == 2) v[1][0] = fmax(v[1][0], ro * ro);
+            else v[1][0] += ro * ro;
         }
     }
-    const int ops[2] = {RSUM, RSUM};
+    const int ops[2] = {RSUM, a.plain_orig == 2 ? RMAX : RSUM};
     block_partials<2, 1>(v, ops, a.partial, rb, ld, b0, lds, valid);
 }
 
@@ -1176,7 +1178,7 @@ __global__ void __launch_bounds__(BLOCK) k_polish_cols(PolishColsArgs a) {
         on2[t] = sel[t] && (!a.act || a.act[ld + b0 + t]);
         any1 = any1 || on1[t]; any2 = any2 || on2[t]; anys = anys || sel[t];
     }
-    if (a.c ? !anys : (!any1 && !(a.p2 && any2))) return;
+    if (a.c ? !anys : (!(a.p1 && any1) && !(a.p2 && any2))) return;
     const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
     for (int r = 0; r < a.sh.rpw; ++r) {
         const int col = uniform(r0 + r);
@@ -1194,7 +1196,7 @@ __global__ void __launch_bounds__(BLOCK) k_polish_cols(PolishColsArgs a) {
                 if (sel[t]) a.q1[i + t] = f[t] * (cc - s1[t]);
             }
         } else {
-            if (any1) {
+            if (a.p1 && any1) {
                 double s1[LPL];
                 row_dot<double, double, double, LPL>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.p1, ld, b0, col, s1);
 #pragma unroll
@@ -1228,6 +1230,7 @@ struct PolishRowsArgs {
     const int* act;          // [2][ld] CG activity (nullptr: both active)
     double* partial;         // [n_rb][2][ld]
     int reverse;             // walk the LP tiles last to first
+    int only = -1;           // one-system launch: the partial slot it writes (0 or 1)
 };
 
 template <int LPL>
@@ -1252,7 +1255,7 @@ __global__ void __launch_bounds__(BLOCK) k_polish_rows(PolishRowsArgs a) {
         any1 = any1 || on1[t]; any2 = any2 || on2[t]; anys = anys || sel[t];
     }
     const bool setup = a.setup_r != nullptr;
-    if (setup ? anys : (any1 || any2)) {
+    if (setup ? anys : ((a.q1 && any1) || (a.q2 && any2))) {
         const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
         for (int r = 0; r < a.sh.rpw; ++r) {
             const int row = uniform(r0 + r);
@@ -1261,7 +1264,7 @@ __global__ void __launch_bounds__(BLOCK) k_polish_rows(PolishRowsArgs a) {
             double on[LPL];
 #pragma unroll
             for (int t = 0; t < LPL; ++t) on[t] = a.rmask[i + t] ? 1.0 : 0.0;
-            if (setup || any1) {
+            if (setup || (a.q1 && any1)) {
                 double t1[LPL];
                 row_dot<double, double, double, LPL>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.q1, ld, b0, row, t1);
 #pragma unroll
@@ -1279,7 +1282,7 @@ __global__ void __launch_bounds__(BLOCK) k_polish_rows(PolishRowsArgs a) {
                     }
                 }
             }
-            if (!setup && any2) {
+            if (!setup && a.q2 && any2) {
                 double t2[LPL];
                 row_dot<double, double, double, LPL>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.q2, ld, b0, row, t2);
 #pragma unroll
@@ -1294,7 +1297,7 @@ __global__ void __launch_bounds__(BLOCK) k_polish_rows(PolishRowsArgs a) {
         }
     }
     const int ops[2] = {RSUM, RSUM};
-    block_partials<2, LPL>(v, ops, a.partial, rb, ld, b0, lds, valid);
+    block_partials<2, LPL>(v, ops, a.partial, rb, ld, b0, lds, valid, a.only);
 }
 
 // Per-LP CG scalars (Jacobi-preconditioned CG, rho = r . D^-1 r), two systems
@@ -1356,10 +1359,11 @@ struct CgUpdateArgs {
     double* w2; double* r2; const double* p2; const double* s2;
     const double* alpha;    // [2][ld]
     const double* invdiag;  // Jacobi preconditioner
-    const double* dr;       // row scale: |r_p|^2 in original units (r / dr), or nullptr (scaled)
+    const double* dr;       // row scale: the primal stop measure in original units (r / dr), or nullptr (scaled)
     const int* sel;
     const int* act;         // [2][ld]
-    double* partial;        // [n_rb][3][ld]: r_p.D^-1 r_p, r_d.D^-1 r_d, |r_p|^2
+    double* partial;        // [n_rb][3][ld]: r_p.D^-1 r_p, r_d.D^-1 r_d, primal stop measure
+    int inf_norm;           // 1: stop measure max_i (r_i / dr_i)^2, else sum_i (r_i [/ dr_i])^2
 };
 
 // 16 waves per block, one row per wave: each block covers 16 rows, so the
@@ -1390,7 +1394,8 @@ __global__ void __launch_bounds__(CG_UPDATE_WAVES * WAVE) k_cg_update(CgUpdateAr
                 a.r1[i] = n1;
                 v[0][0] += n1 * n1 * iv;
                 const double u = a.dr ? n1 / a.dr[row] : n1;
-                v[2][0] += u * u;
+                if (a.inf_norm) v[2][0] = fmax(v[2][0], u * u);
+                else v[2][0] += u * u;
             }
             if (on2) {
                 a.w2[i] = a.w2[i] + al2 * a.p2[i];
@@ -1400,7 +1405,7 @@ __global__ void __launch_bounds__(CG_UPDATE_WAVES * WAVE) k_cg_update(CgUpdateAr
             }
         }
     }
-    const int ops[3] = {RSUM, RSUM, RSUM};
+    const int ops[3] = {RSUM, RSUM, a.inf_norm ? RMAX : RSUM};
     block_partials<3, 1, CG_UPDATE_WAVES>(v, ops, a.partial, rb, ld, b0, lds, valid);
 }
 
