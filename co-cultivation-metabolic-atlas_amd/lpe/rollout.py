@@ -10,7 +10,15 @@ Each step mirrors the reference's dFBA step for the cooperative consortium
   (src/community_solver.py:1120-1149); growth rows t - mu_k <= 0 (:1114-1119);
 * Euler update   C_i <- max(0, C_i + sum_s v_ex(s,i) * biomass_s * dt)
   (src/dfba_simulator.py:1352,1425-1427);
-  biomass_s <- biomass_s * (1 + GROWTH_SCALE * v_bio_s * dt).
+* growth         mu_s = clip(v_bio_s / FLUX_UNIT, -0.5, 2) per hour and
+  X_s <- clip(X_s + mu_s X_s cap dt, 0, K), cap = max(0, 1 - sum_s X_s / K)
+  when mu_s > 0 (else 1), K = 20 g/L (src/dfba_simulator.py:1317-1321,
+  carrying_capacity default :104).  The reference reads the biomass flux
+  of a genome-scale model as mu in 1/h; the synthetic generator's fluxes
+  live on the flux-bound scale (|v| <= 1000, optimal biomass fluxes of
+  150-700), so they are read in units of FLUX_UNIT = 1000, which gives
+  mu = 0.15-0.7 per hour.  Every species is updated from the same pre-step
+  total biomass (the reference updates species one after another).
 
 All per-environment state is a torch tensor on the GPU; the LPs of one step
 are one batched solve.  Warm starts: "previous" (the last step's primal and
@@ -31,7 +39,17 @@ from . import _lib
 from .engine import BatchedLpEngine, SolveOptions
 from .workloads import BOUND, UNLIMITED_SUPPLY, Consortium, consortium
 
-GROWTH_SCALE = 1e-4
+FLUX_UNIT = 1000.0           # synthetic flux per unit of growth rate (1/h)
+MU_CLIP = (-0.5, 2.0)        # src/dfba_simulator.py:1317
+CARRYING_CAPACITY = 20.0     # g/L, src/dfba_simulator.py:104
+
+
+def active_columns(x: torch.Tensor, lower: torch.Tensor, upper: torch.Tensor, tol: float = 1e-6) -> torch.Tensor:
+    """[B, n] bool: variables at a finite bound (within tol * max(1, width))."""
+    width = torch.clamp(upper - lower, min=1.0)
+    at_lo = torch.isfinite(lower) & ((x - lower) <= tol * width)
+    at_up = torch.isfinite(upper) & ((upper - x) <= tol * width)
+    return at_lo | at_up
 
 
 class ConsortiumRollout:
@@ -39,8 +57,11 @@ class ConsortiumRollout:
                  precision: str = "fp32", options: Optional[SolveOptions] = None,
                  warm_start="previous", dt: float = 1.0, conc_range=(1e-2, 10.0),
                  autotune: bool = True, fallback: bool = True, device: Optional[torch.device] = None,
-                 method: str = "restarted", reduce: bool = False):
+                 method: str = "restarted", reduce: bool = False, flux_unit: float = FLUX_UNIT,
+                 capacity: float = CARRYING_CAPACITY):
         self.model = model if model is not None else consortium(seed)
+        self.flux_unit = float(flux_unit)
+        self.capacity = float(capacity)
         md = self.model
         self.n_env = int(n_env)
         self.dt = float(dt)
@@ -87,6 +108,8 @@ class ConsortiumRollout:
         self.tuning = None
         self.step_index = 0
         self.log: list[dict] = []
+        self.track_active = False   # record per-step active-set changes in the log
+        self._active = None
 
     def lp_inputs(self, conc=None, biomass=None):
         """(rhs [B,m], lower [B,n], upper [B,n], coupling values [B,3*K]) on the device."""
@@ -115,8 +138,10 @@ class ConsortiumRollout:
         terms[:, self._term_species, self._term_medium] = x[:, self._term_cols] * self.biomass[:, self._term_species]
         delta = terms.sum(dim=1) * self.dt                   # species summed in order (deterministic)
         conc = torch.clamp(self.conc + delta, min=0.0)
-        mu = x[:, self._biomass_cols] * GROWTH_SCALE
-        biomass = self.biomass * (1.0 + mu * self.dt)
+        mu = torch.clamp(x[:, self._biomass_cols] / self.flux_unit, MU_CLIP[0], MU_CLIP[1])
+        cap = torch.clamp(1.0 - self.biomass.sum(dim=1, keepdim=True) / self.capacity, min=0.0)
+        dx = mu * self.biomass * torch.where(mu > 0, cap, torch.ones_like(cap)) * self.dt
+        biomass = torch.clamp(self.biomass + dx, min=0.0, max=self.capacity)
         if mask is not None:
             conc = torch.where(mask[:, None], conc, self.conc)
             biomass = torch.where(mask[:, None], biomass, self.biomass)
@@ -152,6 +177,8 @@ class ConsortiumRollout:
         torch.cuda.synchronize()
         info["wall_seconds"] = time.perf_counter() - t0
         info["step"] = self.step_index
+        if self.track_active:
+            self._record_active(info, x, y, lower, upper, certified)
         self.log.append(info)
         if advance:
             ok = torch.as_tensor(certified, device=self.device)
@@ -161,6 +188,28 @@ class ConsortiumRollout:
             self.omega = eng.primal_weight()
             self.step_index += 1
         return dict(info, certified_mask=certified, x=x, y=y)
+
+    def _record_active(self, info, x, y, lower, upper, certified):
+        """Active set of each certified LP (columns at a bound, inequality rows
+        with a nonzero dual) against the previous step's: mean changed
+        entries per environment and the share of environments whose set
+        changed (environments certified at both steps)."""
+        md = self.model
+        cols = active_columns(x, lower, upper)
+        rows = y[:, md.neq:] < -1e-9
+        ok = torch.as_tensor(certified, device=self.device)
+        if self._active is not None:
+            pc, pr, pok = self._active
+            both = ok & pok
+            nb = int(both.sum())
+            if nb:
+                dc = (cols != pc)[both].sum(dim=1)
+                dr = (rows != pr)[both].sum(dim=1)
+                info["active_changed_columns"] = float(dc.double().mean())
+                info["active_changed_rows"] = float(dr.double().mean())
+                info["active_changed_share"] = float(((dc + dr) > 0).double().mean())
+        info["active_columns"] = float(cols[ok].sum(dim=1).double().mean()) if bool(ok.any()) else None
+        self._active = (cols, rows, ok)
 
     def snapshot(self):
         return self.conc.clone(), self.biomass.clone()
