@@ -1,18 +1,26 @@
 #!/usr/bin/env python
-"""Certified LPs/s of the batched dFBA step on one MI355X.
+"""Certified LPs/s of the batched dFBA rollout on one MI355X.
 
 Workload: the 3-species consortium dFBA rollout for 4096 environments - each
 step solves 4096 joint LPs (3663 rows x 5301 columns, ~20.6k nonzeros,
 shared-medium coupling coefficients per environment), then updates medium
-and biomass by explicit Euler on the device.  A bench "step" is one dFBA step.
+and biomass by explicit Euler on the device.  A bench "step" is one dFBA
+step; the timed steps are steps 0..K-1 of a rollout from its initial
+conditions (the warm-up steps run a separate rollout of another seed on the
+same engine, so graphs and the autotuned launch shape are in place).
 
 Target form: restarted-averaged PDHG with FP32 bulk arithmetic, FP64 checks,
-FP64 active-set polish and FP64 certificate, warm-started by the learned MLP
-(trained before the timed region on certified solves of rollouts with other
-seeds) or, with --warm-start previous, from the previous step's primal/dual.  Plain form: FP64 restarted-averaged PDHG, cold start, no
-polish, on the same LPs.  An LP counts as certified when it passes the
-reference's certificate (primal <= 1e-5, dual <= 1e-7, relative KKT gap <=
-1e-7); a HiGHS sample checks <= 1e-6 relative objective error.
+FP64 active-set polish and FP64 certificate, warm-started by the learned
+hybrid start (the previous step's solution where it scores within 100x the
+certificate gates on the new LP, the MLP's fused prediction elsewhere and on
+the first step; the MLP is trained before the timed region on ~200k certified
+solves of rollouts with other seeds) or, with --warm-start previous, from the
+previous step's primal/dual.  Plain form: FP64 restarted-averaged PDHG, cold
+start, no polish, on the first and last timed steps' LPs.  An LP counts as
+certified when it passes the reference's certificate (primal <= 1e-5, dual
+<= 1e-7, relative KKT gap <= 1e-7); HiGHS follows 1 % of the environments
+through every timed step (warm, the reference's repeated-LP service) and
+checks <= 1e-6 relative objective error.
 
 Prints one JSON line (rank 0).
 """
@@ -103,36 +111,40 @@ def _dual_bytes(B, m, n, nnz, nvar, w):
     return B * per_lp + shared
 
 
-def _highs_worker(args):
-    import numpy as np
+def _highs_trajectory_worker(args):
+    """One environment's LPs over the timed steps on one persistent HiGHS
+    model: the first step cold, every later step hot-updated and started from
+    the previous step's basis (the reference's repeated-LP service)."""
     sys.path.insert(0, ROOT)
-    from oracle import highs
-    a_data, a_ind, a_ptr, shape, rhs, lo, hi, c, neq = args
     from scipy.sparse import csr_matrix
-    a = csr_matrix((a_data, a_ind, a_ptr), shape=shape)
-    r = highs.solve(a, rhs, lo, hi, c, neq)
-    return r.status, r.fun, r.seconds
-
-
-def _highs_warm_worker(args):
-    import numpy as np
-    sys.path.insert(0, ROOT)
     from oracle import highs
-    a_data, a_ind, a_ptr, shape, rhs, lo, hi, c, neq, a_data1, rhs1, lo1, hi1 = args
-    from scipy.sparse import csr_matrix
-    model = highs.PersistentHighs(csr_matrix((a_data, a_ind, a_ptr), shape=shape), rhs, lo, hi, c, neq)
-    r0 = model.solve()                      # the previous step (untimed)
-    model.update(a_data1, rhs1, lo1, hi1)
-    r1 = model.solve()                      # the timed step, from the previous step's basis
-    return r1.status, r1.fun, r1.seconds, r1.iterations, r0.seconds
+    (data, ind, ptr, shape, var_pos, rhs0, lower0, upper, c, neq, medium_rows, term_cols,
+     coupling, rhs_medium, lower_terms) = args
+    out = []
+    model = None
+    for k in range(coupling.shape[0]):
+        vals = data.copy()
+        vals[var_pos] = coupling[k]
+        rhs = rhs0.copy()
+        rhs[medium_rows] = rhs_medium[k]
+        lower = lower0.copy()
+        lower[term_cols] = lower_terms[k]
+        if model is None:
+            model = highs.PersistentHighs(csr_matrix((vals, ind, ptr), shape=shape), rhs, lower, upper, c, neq)
+        else:
+            model.update(vals, rhs, lower, upper)
+        r = model.solve()
+        out.append((r.status, r.fun, r.seconds, r.iterations))
+    return out
 
 
-def cpu_baseline_warm(model, snap0, snap1, gpu_x, gpu_ok, seconds_budget, cold_seconds):
-    """HiGHS as the reference's repeated-LP service runs it across dFBA steps
-    (src/cpu_repeated_lp.py: one persistent model per environment, hot updates,
-    the previous solve's basis): each sampled environment solves the first
-    timed step's LP untimed, is updated to the second step's LP and solves it
-    timed."""
+def cpu_baseline_rollout(model, snapshots, sample, gpu_x, gpu_ok, dt=1.0):
+    """HiGHS along the rollout for a 1 % sample of the environments: each
+    sampled environment's LPs of every timed step on one persistent dual-simplex
+    model (src/cpu_repeated_lp.py: hot updates, the previous solve's basis; the
+    first step cold).  Returns the warm rate (steps after the first), the cold
+    rate (first step), simplex iterations per step and the HiGHS check of the
+    GPU answers (relative objective error of every sampled (environment, step))."""
     import multiprocessing as mp
     import numpy as np
     sys.path.insert(0, ROOT)
@@ -142,75 +154,53 @@ def cpu_baseline_warm(model, snap0, snap1, gpu_x, gpu_ok, seconds_budget, cold_s
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
-    workers = max(1, min(cores, 16))
-    cb0 = ConsortiumBatch(model=model, conc=snap0[0], biomass=snap0[1])
-    cb1 = ConsortiumBatch(model=model, conc=snap1[0], biomass=snap1[1])
-    # the untimed previous-step solves dominate this leg's wall time
-    n_sample = int(max(workers, min(4 * workers, workers * seconds_budget / max(cold_seconds, 1e-3))))
-    n_sample = min(n_sample, snap0[0].shape[0])
+    workers = max(1, min(cores, 16, len(sample)))
+    A = model.A
+    per_step = []
+    for conc, biomass in snapshots:
+        cb = ConsortiumBatch(model=model, conc=conc[sample], biomass=biomass[sample], dt=dt)
+        per_step.append((cb.coupling_values(), cb.rhs()[:, model.medium_rows], cb.lower()[:, model.term_cols]))
+    rhs0 = np.zeros(A.shape[0])
+    rhs0[:model.neq] = model.rhs_species
     jobs = []
-    for b in range(n_sample):
-        a, rhs, lo, hi, c, neq = cb0.problem(b)
-        a1, rhs1, lo1, hi1, _, _ = cb1.problem(b)
-        jobs.append((a.data, a.indices, a.indptr, a.shape, rhs, lo, hi, c, neq, a1.data, rhs1, lo1, hi1))
-    with mp.get_context("spawn").Pool(workers) as pool:
-        res = pool.map(_highs_warm_worker, jobs, chunksize=1)
-    timed = float(sum(r[2] for r in res))
-    errs = []
-    for b, r in enumerate(res):
-        if r[0] == "optimal" and gpu_x is not None and gpu_ok[b]:
-            errs.append(cert.relative_objective_error(float(model.c @ gpu_x[b]), r[1]))
-    return dict(value=n_sample * workers / timed, unit="LPs/s", cores=workers, kind="port",
-                sample=f"{n_sample} LPs of the second timed step, warm: persistent per-environment "
-                       f"{highs.version()} dual-simplex models hot-updated from the first timed step's LP and "
-                       f"started from its basis (1 thread per LP); rate = LPs x {workers} workers / summed "
-                       f"solve seconds ({timed:.2f} s)",
-                solver=highs.version(), mean_warm_lp_seconds=timed / n_sample,
-                mean_simplex_iterations=float(np.mean([r[3] for r in res]))), errs, n_sample
-
-
-def cpu_baseline(model, conc, biomass, gpu_x, gpu_ok, seconds_budget):
-    """HiGHS (the reference's exact CPU path, via SciPy) on a bounded sample
-    of the first timed step's LPs, one LP per worker process."""
-    import multiprocessing as mp
-    import numpy as np
-    sys.path.insert(0, ROOT)
-    from oracle import highs, certificate as cert
-    from lpe.workloads import ConsortiumBatch
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    workers = max(1, min(cores, 16))
-    cb = ConsortiumBatch(model=model, conc=conc, biomass=biomass)
-    # probe one LP for its duration, then size the sample to the budget
-    probs = []
-    t0 = time.perf_counter()
-    p0 = cb.problem(0)
-    r0 = highs.solve(*p0)
-    one = time.perf_counter() - t0
-    n_sample = int(max(workers, min(4 * workers, workers * seconds_budget / max(one, 1e-3))))
-    n_sample = min(n_sample, conc.shape[0])
-    for b in range(1, n_sample):
-        a, rhs, lo, hi, c, neq = cb.problem(b)
-        probs.append((a.data, a.indices, a.indptr, a.shape, rhs, lo, hi, c, neq))
+    for j in range(len(sample)):
+        jobs.append((A.data, A.indices, A.indptr, A.shape, model.var_positions, rhs0, model.lower, model.upper,
+                     model.c, model.neq, model.medium_rows, model.term_cols,
+                     np.stack([ps[0][j] for ps in per_step]), np.stack([ps[1][j] for ps in per_step]),
+                     np.stack([ps[2][j] for ps in per_step])))
     t0 = time.perf_counter()
     with mp.get_context("spawn").Pool(workers) as pool:
-        res = pool.map(_highs_worker, probs, chunksize=1)
+        res = pool.map(_highs_trajectory_worker, jobs, chunksize=1)
     elapsed = time.perf_counter() - t0
-    res = [(r0.status, r0.fun, one)] + res
-    done = n_sample - 1
-    funs = [r[1] for r in res]
-    errs = []
-    for b in range(n_sample):
-        if res[b][0] != "optimal" or gpu_x is None or not gpu_ok[b]:
-            continue
-        obj = float(model.c @ gpu_x[b])
-        errs.append(cert.relative_objective_error(obj, funs[b]))
-    return dict(value=done / elapsed, unit="LPs/s", cores=workers, kind="port",
-                sample=f"{done} LPs of the first timed step (cold {highs.version()} highs-ds, "
-                       f"1 thread per LP, {workers} worker processes), {elapsed:.1f} s",
-                solver=highs.version(), highs_single_lp_seconds=one), errs, n_sample
+    K = len(snapshots)
+    warm_s = sum(r[k][2] for r in res for k in range(1, K))
+    cold_s = sum(r[0][2] for r in res)
+    pivots = [float(np.mean([r[k][3] for r in res])) for k in range(K)]
+    errs, failed = [], 0
+    for j, b in enumerate(sample):
+        for k in range(K):
+            st, fun = res[j][k][0], res[j][k][1]
+            if st != "optimal" or not gpu_ok[k][j]:
+                failed += 1
+                continue
+            errs.append(cert.relative_objective_error(float(model.c @ gpu_x[k][j]), fun))
+    n_warm = len(sample) * (K - 1)
+    warm = None
+    if n_warm > 0 and warm_s > 0:
+        warm = dict(value=round(n_warm * workers / warm_s, 3), unit="LPs/s", cores=workers, kind="port",
+                    sample=f"{len(sample)} environments (1 %) x {K - 1} steps after the first: one persistent "
+                           f"{highs.version()} dual-simplex model per environment, hot-updated each step and "
+                           f"started from the previous step's basis (1 thread per LP); rate = LPs x {workers} "
+                           f"workers / summed solve seconds ({warm_s:.2f} s)",
+                    solver=highs.version(), mean_warm_lp_seconds=round(warm_s / n_warm, 5))
+    cold = dict(value=round(len(sample) * workers / cold_s, 3), unit="LPs/s", cores=workers, kind="port",
+                sample=f"the first timed step's LPs of the {len(sample)} sampled environments, cold "
+                       f"{highs.version()} dual simplex (1 thread per LP); rate = LPs x {workers} workers / summed "
+                       f"solve seconds ({cold_s:.2f} s)", solver=highs.version())
+    check = {"instances": len(sample) * K, "environments": len(sample),
+             "checked": len(errs), "not_compared": failed,
+             "max_rel_obj_err": max(errs) if errs else None, "within_1e-6": int(sum(e <= 1e-6 for e in errs))}
+    return warm, cold, pivots, check, elapsed
 
 
 GEM_FIXTURE = os.path.join(ROOT, "tests", "golden", "gem_consortium.npz")
@@ -300,16 +290,17 @@ def train_warm_start(model, B, dev, args, options):
     from lpe.rollout import ConsortiumRollout
     from lpe.warmstart import LearnedWarmStart
 
-    lws = LearnedWarmStart(model, device=dev, mode="predict")
+    lws = LearnedWarmStart(model, device=dev)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for seed in [int(s) for s in args.train_seeds.split(",") if s]:
-        ro = ConsortiumRollout(model, B, seed=seed, precision="fp32", options=options(), device=dev)
+        ro = ConsortiumRollout(model, args.train_envs, seed=seed, precision="fp32", options=options(), device=dev)
         for _ in range(args.train_steps):
             conc, bio = ro.conc.clone(), ro.biomass.clone()
             _, lower, upper, _ = ro.lp_inputs()
             out = ro.step()
             lws.add_solutions(conc, lower, upper, bio, out["x"], out["y"], out["certified_mask"])
+        ro.engine.close()
         del ro
         torch.cuda.empty_cache()
     torch.cuda.synchronize()
@@ -329,17 +320,24 @@ def train_warm_start(model, B, dev, args, options):
     e1.record()
     torch.cuda.synchronize()
     train_ms = e0.elapsed_time(e1) / 10
-    xi = torch.randn(B, net.n_in_p, device=dev).to(torch.bfloat16)
-    net._forward(xi, keep=False)
+    # inference as the rollout runs it: the fused path (standardisation, bias +
+    # GELU epilogues, the output layer writing the projected FP64 start)
+    fi = torch.randn(B, net.n_in, device=dev)
+    lo = torch.as_tensor(model.lower, device=dev).expand(B, -1).contiguous()
+    up = torch.as_tensor(model.upper, device=dev).expand(B, -1).contiguous()
+    net.infer_start(fi, lo, up, model.neq)
     e0.record()
     for _ in range(10):
-        net._forward(xi, keep=False)
+        net.infer_start(fi, lo, up, model.neq)
     e1.record()
     torch.cuda.synchronize()
     infer_ms = e0.elapsed_time(e1) / 10
     fps = net.flops_per_sample()
-    report = {"kind": "mlp", "layers": net.dims, "params": net.n_params, "train_samples": lws.n_samples,
-              "train_rollouts": args.train_seeds, "train_steps_per_rollout": args.train_steps,
+    report = {"kind": "mlp (hybrid: previous solution where it scores within 100x the certificate gates, "
+                      "the network's start elsewhere)",
+              "layers": net.dims, "params": net.n_params, "train_samples": lws.n_samples,
+              "train_rollouts": args.train_seeds, "train_envs": args.train_envs,
+              "train_steps_per_rollout": args.train_steps,
               "solve_seconds": round(t1 - t0, 2), "train_seconds": round(t2 - t1, 2), "epochs": args.train_epochs,
               "loss_first_last": [round(hist[0], 5), round(hist[-1], 5)],
               "mfma_roofline": {"bound": "mfma", "peak": 2500.0, "unit": "TFLOP/s",
@@ -436,15 +434,23 @@ def gem_reduced_leg(args, dev):
     return report
 
 
+def _step_record(out):
+    keys = ("certified", "iterations", "restarts", "polish_attempts", "cg_steps", "cg_lp_steps", "pdhg_ms",
+            "check_ms", "polish_ms", "wall_seconds", "fallback_lps", "fallback_certified", "fallback_seconds",
+            "start_mlp")
+    return {k: out.get(k) for k in keys}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=48, help="timed dFBA steps: steps 0..K-1 of a fresh rollout")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed steps of a separate warm-up rollout (same engine)")
     ap.add_argument("--envs", type=int, default=4096)
-    ap.add_argument("--plain-steps", type=int, default=1)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-sample", type=float, default=0.01,
+                    help="share of environments followed by HiGHS through every timed step (spot check + baseline)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-compare", action="store_true", help="skip the previous-step warm-start leg")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--max-iterations", type=int, default=30000)
     ap.add_argument("--cg-iterations", type=int, default=200, help="CG steps per polish round")
@@ -457,13 +463,14 @@ def main():
     ap.add_argument("--lps-per-lane", type=int, default=4, help="launch shape of --probe-kernels")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "pmc_step_kernels_consortium.md"))
     ap.add_argument("--warm-start", choices=("mlp", "previous"), default="mlp",
-                    help="mlp: the learned warm start (trained before the timed region); previous: last step's solution")
+                    help="mlp: the learned hybrid start (trained before the timed region); previous: last step's "
+                         "solution")
     ap.add_argument("--train-seeds", default="11,12", help="rollout seeds whose certified solves train the MLP")
-    ap.add_argument("--train-steps", type=int, default=6, help="dFBA steps per training rollout")
+    ap.add_argument("--train-envs", type=int, default=2048, help="environments per training rollout")
+    ap.add_argument("--train-steps", type=int, default=49, help="dFBA steps per training rollout")
     ap.add_argument("--train-epochs", type=int, default=20)
-    ap.add_argument("--first-check", type=int, default=0,
-                    help="PDHG iterations of a warm-started step before its first check and polish "
-                         "(0: half the check interval)")
+    ap.add_argument("--plain-steps", default="first,last",
+                    help="timed steps re-solved by the plain form (FP64 cold PDHG): 'first,last', indices, or ''")
     ap.add_argument("--no-gem", action="store_true", help="skip the real-GEM (reduced, Halpern FP64) leg")
     ap.add_argument("--gem-envs", type=int, default=1024)
     ap.add_argument("--gem-iterations", type=int, default=100000)
@@ -482,7 +489,7 @@ def main():
 
     import numpy as np
     import torch
-    from lpe.engine import SolveOptions
+    from lpe.engine import BatchedLpEngine, SolveOptions
     from lpe.rollout import ConsortiumRollout
 
     dev = torch.device("cuda", 0)
@@ -492,73 +499,114 @@ def main():
     nnz = model.A.nnz
     nvar = model.var_positions.size
     B = args.envs
+    K = args.steps
 
     target_options = lambda: SolveOptions(max_iterations=args.fp32_iterations, polish=True,  # noqa: E731
-                                          polish_cg_iterations=args.cg_iterations,
-                                          first_check_interval=args.first_check)
+                                          polish_cg_iterations=args.cg_iterations)
     warm, mlp_report = "previous", None
     if args.warm_start == "mlp":
         warm, mlp_report = train_warm_start(model, B, dev, args, target_options)
-    target = ConsortiumRollout(model, B, seed=args.seed, precision="fp32", options=target_options(),
-                               warm_start=warm, device=dev)
+    # one engine for the warm-up, timed and comparison rollouts (its graphs and
+    # autotuned launch shape carry over)
+    eng = BatchedLpEngine(model.A, model.neq, B, var_positions=model.var_positions, precision="fp32",
+                          shared_c=True, device=dev)
+    warm_ro = ConsortiumRollout(model, B, seed=args.seed + 1000, precision="fp32", options=target_options(),
+                                warm_start=warm, device=dev, engine=eng)
     for _ in range(args.warmup):
-        target.step()
-        print(f"# warmup step done: {target.log[-1]['certified']}/{B} certified, "
-              f"{target.log[-1]['iterations']} iterations, {target.log[-1]['wall_seconds']:.2f} s",
+        warm_ro.step()
+        print(f"# warm-up step done: {warm_ro.log[-1]['certified']}/{B} certified, "
+              f"{warm_ro.log[-1]['iterations']} iterations, {warm_ro.log[-1]['wall_seconds']:.2f} s",
               file=sys.stderr, flush=True)
-    snapshots, certified, step_logs = [], 0, []
-    first_x = first_ok = second_x = second_ok = None
+    tuning = warm_ro.tuning
+    if tuning is None:
+        from lpe.autotune import autotune
+        rhs, lower, upper, coupling = warm_ro.lp_inputs()
+        eng.update(model.c, rhs, lower, upper, var_values=coupling)
+        tuning = autotune(eng)
+    del warm_ro
+
+    # the timed rollout: steps 0..K-1 from the initial conditions
+    n_check = max(1, int(round(args.cpu_sample * B)))
+    sample = np.unique(np.linspace(0, B - 1, n_check).round().astype(int))
+    target = ConsortiumRollout(model, B, seed=args.seed, precision="fp32", options=target_options(),
+                               warm_start=warm, device=dev, engine=eng, tuning=tuning)
+    snapshots, certified, step_logs, sample_x, sample_ok = [], 0, [], [], []
+    sample_t = torch.as_tensor(sample, device=dev)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(K):
         snapshots.append(target.snapshot())
         out = target.step()
         certified += int(out["certified"])
-        if k == 0:
-            first_x, first_ok = out["x"], np.asarray(out["certified_mask"])
-        elif k == 1:
-            second_x, second_ok = out["x"], np.asarray(out["certified_mask"])
-        step_logs.append({kk: out.get(kk) for kk in ("certified", "iterations", "restarts", "polish_attempts",
-                                                     "cg_steps", "cg_lp_steps", "pdhg_ms", "check_ms", "polish_ms",
-                                                     "wall_seconds", "fallback_lps", "fallback_certified",
-                                                     "fallback_seconds")})
+        sample_x.append(out["x"][sample_t])
+        sample_ok.append(np.asarray(out["certified_mask"])[sample])
+        step_logs.append(_step_record(out))
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     value = certified / elapsed
-    print(f"# target: {certified}/{B * args.steps} certified in {elapsed:.2f} s", file=sys.stderr, flush=True)
-    first_x_host = first_x.cpu().numpy()
+    print(f"# target: {certified}/{B * K} certified in {elapsed:.2f} s", file=sys.stderr, flush=True)
+    sample_x = [t.cpu().numpy() for t in sample_x]
+    final_state = (target.x, target.y)
+    del target
+
+    # the same steps warm-started from the previous step's solution alone
+    compare = None
+    if not args.no_compare and args.warm_start == "mlp":
+        prev_ro = ConsortiumRollout(model, B, seed=args.seed, precision="fp32", options=target_options(),
+                                    warm_start="previous", device=dev, engine=eng, tuning=tuning)
+        p_cert, p_logs = 0, []
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for k in range(K):
+            out = prev_ro.step()
+            p_cert += int(out["certified"])
+            p_logs.append(_step_record(out))
+        torch.cuda.synchronize()
+        p_el = time.perf_counter() - t1
+        del prev_ro
+        compare = {"previous_step_start": {"value": round(p_cert / p_el, 3), "unit": "certified LPs/s",
+                                           "certified": p_cert, "seconds": round(p_el, 3),
+                                           "iterations": [l["iterations"] for l in p_logs],
+                                           "polish_attempts": [l["polish_attempts"] for l in p_logs],
+                                           "wall_ms": [round(1e3 * l["wall_seconds"], 2) for l in p_logs]},
+                   "learned_start": {"value": round(value, 3), "certified": certified, "seconds": round(elapsed, 3),
+                                     "iterations": [l["iterations"] for l in step_logs],
+                                     "polish_attempts": [l["polish_attempts"] for l in step_logs],
+                                     "wall_ms": [round(1e3 * l["wall_seconds"], 2) for l in step_logs],
+                                     "lps_started_from_the_network": [l["start_mlp"] for l in step_logs]},
+                   "gain": round(value / (p_cert / p_el), 4)}
+        print(f"# previous-step start: {p_cert}/{B * K} certified in {p_el:.2f} s", file=sys.stderr, flush=True)
 
     # the step's kernels timed with hipEvents on the engine stream (every LP
-    # active): the PDHG half steps and the polish's CG update; the dominant
-    # one (largest estimated time per timed step) carries the roofline
-    target_tuning = target.tuning
-    eng = target.engine
-    eng.reset(target.x, target.y)
+    # active); per timed step each counts as (its launches) x (the share of
+    # LPs active in them): PDHG launches run every LP, CG launches only the
+    # LPs whose CG systems are still active (cg_lp_steps / B full launches)
+    eng.reset(*final_state)
     ms_primal = eng.time_kernel("primal", reps=20)
     ms_dual = eng.time_kernel("dual", reps=20)
     ms_cg = eng.time_kernel("cg_update", reps=20)
     ms_pc = eng.time_kernel("polish_cols", reps=20)
     ms_pr = eng.time_kernel("polish_rows", reps=20)
     ms_cd = eng.time_kernel("cg_direction", reps=20)
-    lpl = int((target_tuning or {}).get("lps_per_lane", 1))
+    lpl = int((tuning or {}).get("lps_per_lane", 1))
     pb = _primal_bytes(B, m, n, nnz, nvar, 4)
     db = _dual_bytes(B, m, n, nnz, nvar, 4)
     cgb = _cg_update_bytes(B, m, _update_row_blocks(m))
     mean_iters = float(np.mean([s["iterations"] for s in step_logs]))
-    mean_cg = float(np.mean([s.get("cg_steps") or 0 for s in step_logs]))
+    full_cg = float(np.mean([(s.get("cg_lp_steps") or 0) for s in step_logs])) / B
     kernels = {
         f"k_primal_step<float,{lpl},false>": dict(algorithmic_bytes_per_launch=pb, avg_launch_ms=round(ms_primal, 5),
                                                   achieved=round(pb / (ms_primal * 1e-3) / 1e9, 1),
-                                                  launches_per_step=mean_iters,
+                                                  full_launches_per_step=round(mean_iters, 2),
                                                   ms_per_step=round(mean_iters * ms_primal, 3),
                                                   traffic=measured_traffic(args.traffic_file, B, lpl)),
         f"k_dual_step<float,{lpl},false>": dict(algorithmic_bytes_per_launch=db, avg_launch_ms=round(ms_dual, 5),
                                                 achieved=round(db / (ms_dual * 1e-3) / 1e9, 1),
-                                                launches_per_step=mean_iters,
+                                                full_launches_per_step=round(mean_iters, 2),
                                                 ms_per_step=round(mean_iters * ms_dual, 3)),
         "k_cg_update": dict(algorithmic_bytes_per_launch=cgb, avg_launch_ms=round(ms_cg, 5),
-                            achieved=round(cgb / (ms_cg * 1e-3) / 1e9, 1), launches_per_step=mean_cg,
-                            ms_per_step=round(mean_cg * ms_cg, 3),
+                            achieved=round(cgb / (ms_cg * 1e-3) / 1e9, 1), full_launches_per_step=round(full_cg, 2),
+                            ms_per_step=round(full_cg * ms_cg, 3),
                             traffic=measured_traffic(args.traffic_file, B, lpl, key="cg_update_hbm_bytes")),
     }
     pcb = _polish_cols_bytes(B, m, n, nnz, nvar)
@@ -567,33 +615,33 @@ def main():
                                   ("k_polish_rows<2>", prb, ms_pr, "polish_rows_hbm_bytes"),
                                   ("k_cg_direction", _cg_direction_bytes(B, m), ms_cd, "cg_direction_hbm_bytes")):
         kernels[name] = dict(algorithmic_bytes_per_launch=nbytes, avg_launch_ms=round(ms, 5),
-                             achieved=round(nbytes / (ms * 1e-3) / 1e9, 1), launches_per_step=mean_cg,
-                             ms_per_step=round(mean_cg * ms, 3),
+                             achieved=round(nbytes / (ms * 1e-3) / 1e9, 1), full_launches_per_step=round(full_cg, 2),
+                             ms_per_step=round(full_cg * ms, 3),
                              traffic=measured_traffic(args.traffic_file, B, lpl, key=key))
     dominant = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
-    # whole-step view: algorithmic bytes of every PDHG iteration (all LPs run
-    # them) and of every CG step of an LP with an active system, over the
-    # timed wall time per step
     per_lp_cg, shared_cg = _cg_step_bytes(m, n, nnz, nvar, _polish_row_blocks(m, B), _update_row_blocks(m))
+    mean_cg = float(np.mean([s.get("cg_steps") or 0 for s in step_logs]))
     pdhg_step_bytes = mean_iters * (pb + db)
-    cg_step_bytes = float(np.mean([s.get("cg_lp_steps") or 0 for s in step_logs])) * per_lp_cg + mean_cg * shared_cg
-    step_s = elapsed / args.steps
+    cg_step_bytes = full_cg * B * per_lp_cg + mean_cg * shared_cg
+    step_s = elapsed / K
     whole_step = {"algorithmic_bytes_per_step": int(pdhg_step_bytes + cg_step_bytes),
                   "pdhg_bytes": int(pdhg_step_bytes), "cg_bytes": int(cg_step_bytes),
                   "achieved": round((pdhg_step_bytes + cg_step_bytes) / step_s / 1e9, 1),
                   "frac": round((pdhg_step_bytes + cg_step_bytes) / step_s / 1e9 / HBM_PEAK_GBS, 4),
                   "pdhg_only_frac": round(pdhg_step_bytes / step_s / 1e9 / HBM_PEAK_GBS, 4),
                   "note": "PDHG iterations and polish CG steps over the timed wall time per step; checks, "
-                          "polish set-up and certificates are not counted"}
+                          "polish set-up, certificates and start scoring are not counted"}
 
-    # plain form on the same LPs
-    del target
-    torch.cuda.empty_cache()
+    # plain form on the same LPs: FP64 restarted PDHG, cold, no polish
+    plain_idx = []
+    for tok in [t for t in args.plain_steps.split(",") if t]:
+        plain_idx.append(0 if tok == "first" else (K - 1 if tok == "last" else int(tok)))
+    plain_idx = sorted(set(i for i in plain_idx if 0 <= i < K))
     plain = ConsortiumRollout(model, B, seed=args.seed, precision="fp64",
                               options=SolveOptions(max_iterations=args.max_iterations, polish=False),
                               warm_start="none", device=dev)
     p_cert, p_time, p_logs = 0, 0.0, []
-    for k in range(min(args.plain_steps, len(snapshots))) if args.plain_steps > 0 else []:
+    for k in plain_idx:
         plain.restore(snapshots[k])
         torch.cuda.synchronize()
         t = time.perf_counter()
@@ -601,15 +649,17 @@ def main():
         torch.cuda.synchronize()
         p_time += time.perf_counter() - t
         p_cert += int(out["certified"])
-        p_logs.append({kk: out[kk] for kk in ("certified", "iterations", "restarts", "pdhg_ms", "check_ms",
-                                              "wall_seconds")})
+        p_logs.append({"step": k, **{kk: out[kk] for kk in ("certified", "iterations", "restarts", "pdhg_ms",
+                                                             "check_ms", "wall_seconds")}})
     plain_value = p_cert / p_time if p_time > 0 else 0.0
     ms_primal_fp64 = None
     if p_logs:
         plain.engine.reset()                        # every LP active again (finished LPs skip their lanes)
         ms_primal_fp64 = plain.engine.time_kernel("primal", reps=10)
     print(f"# plain: {p_cert}/{B * len(p_logs)} certified in {p_time:.2f} s", file=sys.stderr, flush=True)
+    plain.engine.close()
     del plain
+    eng.close()
     torch.cuda.empty_cache()
 
     stages_report = None
@@ -619,24 +669,26 @@ def main():
 
     result = {
         "metric": METRIC, "value": round(value, 3), "unit": "certified LPs/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "steps": K, "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / K, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64",
         "data": "synthetic (seeded generator of the 3-species consortium LP; no reference traces)",
-        "config": {"workload": ("3-species consortium dFBA step" if args.workload == "synthetic" else
-                                "3-member real-GEM consortium dFBA step (OR16, NS21, P. freudenreichii SBML)")
-                               + f", {B} environments, LP {m}x{n} "
+        "config": {"workload": ("3-species consortium dFBA rollout" if args.workload == "synthetic" else
+                                "3-member real-GEM consortium dFBA rollout (OR16, NS21, P. freudenreichii SBML)")
+                               + f", {B} environments, steps 0..{K - 1} from the initial conditions, LP {m}x{n} "
                                f"({nnz} nnz, {nvar} per-environment coefficients), "
-                               + ("learned MLP warm start" if args.warm_start == "mlp" else
+                               + ("learned MLP warm start (hybrid)" if args.warm_start == "mlp" else
                                   "warm start from the previous step")
                                + ", FP32 PDHG + FP64 polish/certificate",
-                   "envs": B, "rows": m, "cols": n, "nnz": nnz, "rollout_steps_timed": args.steps},
-        "plain_fp64": {"value": round(plain_value, 3), "unit": "certified LPs/s", "steps": len(p_logs),
+                   "envs": B, "rows": m, "cols": n, "nnz": nnz, "rollout_steps_timed": K,
+                   "warmup_rollout_seed": args.seed + 1000},
+        "plain_fp64": {"value": round(plain_value, 3), "unit": "certified LPs/s", "steps": plain_idx,
                        "certified": p_cert, "seconds": round(p_time, 3), "log": p_logs},
         "speedup_vs_plain": round(value / plain_value, 3) if plain_value > 0 else None,
         "target_log": step_logs,
         "warm_start": mlp_report or {"kind": "previous step's primal/dual"},
-        "autotune": {k: v for k, v in (target_tuning or {}).items() if k != "timings"},
-        "autotune_timings_ms_per_iteration": (target_tuning or {}).get("timings"),
+        "warm_start_comparison": compare,
+        "autotune": {k: v for k, v in (tuning or {}).items() if k != "timings"},
+        "autotune_timings_ms_per_iteration": (tuning or {}).get("timings"),
         "roofline": {"bound": "hbm", "kernel": dominant,
                      "achieved": kernels[dominant]["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(kernels[dominant]["achieved"] / HBM_PEAK_GBS, 4),
@@ -648,21 +700,14 @@ def main():
                      "whole_step": whole_step},
     }
     if not args.no_cpu:
-        cb, errs, n_sample = cpu_baseline(model, snapshots[0][0].cpu().numpy(), snapshots[0][1].cpu().numpy(),
-                                          first_x_host, first_ok, args.cpu_seconds)
-        result["cpu_baseline"] = cb
-        if len(snapshots) >= 2:
-            # the rollout's CPU baseline is warm (previous step's basis); the
-            # cold figure stays beside it
-            warm, errs1, n1 = cpu_baseline_warm(model, [t.cpu().numpy() for t in snapshots[0]],
-                                                [t.cpu().numpy() for t in snapshots[1]],
-                                                second_x.cpu().numpy(), second_ok, args.cpu_seconds,
-                                                cb["highs_single_lp_seconds"])
-            result["cpu_baseline"], result["cpu_baseline_cold"] = warm, cb
-            errs, n_sample = errs + errs1, n_sample + n1
-        result["highs_check"] = {"sample": n_sample, "checked": len(errs),
-                                 "max_rel_obj_err": max(errs) if errs else None,
-                                 "within_1e-6": int(sum(e <= 1e-6 for e in errs))}
+        warm_cpu, cold_cpu, pivots, check, cpu_el = cpu_baseline_rollout(
+            model, [(c.cpu().numpy(), b.cpu().numpy()) for c, b in snapshots], sample, sample_x, sample_ok)
+        result["cpu_baseline"] = warm_cpu or cold_cpu
+        result["cpu_baseline_cold"] = cold_cpu
+        result["highs_simplex_iterations_per_step"] = [round(v, 1) for v in pivots]
+        result["highs_check"] = check
+        print(f"# HiGHS: {check['checked']} sampled LPs checked, max rel obj err {check['max_rel_obj_err']}, "
+              f"{cpu_el:.1f} s", file=sys.stderr, flush=True)
     else:
         result["cpu_baseline"] = None
     result["cooperative_stages"] = stages_report

@@ -102,10 +102,34 @@ __device__ __forceinline__ void stage_scalar(uint16_t (*S)[BK + PAD], const uint
     }
 }
 
-template <int TA, int TB>
+constexpr float SQRT_2_OVER_PI = 0.7978845608028654f;
+constexpr float GELU_C = 0.044715f;
+
+__device__ __forceinline__ float gelu(float z) {
+    const float u = SQRT_2_OVER_PI * (z + GELU_C * z * z * z);
+    return 0.5f * z * (1.0f + tanhf(u));
+}
+
+// Fused epilogues of the inference products (EPI 1 and 2 of k_gemm).
+struct Epilogue {
+    const float* bias;      // [N]
+    // EPI 1: H[row][col] = bf16(gelu(acc + bias))
+    uint16_t* H; int ldh;
+    // EPI 2 (warm start): columns interleave (x_j, logit_j) for j < nx, then
+    // y_i; x_j = clamp((acc + bias) * scale + shift, lower, upper), snapped to
+    // the nearer finite bound when snap and logit_j > 0; y_i = (acc + bias) *
+    // scale + shift, clipped to <= 0 on inequality rows (i >= neq); stored FP64
+    // batch-major x0[row][j], y0[row][i] (row = LP)
+    const float* scale; const float* shift;
+    const double* lower; const double* upper;
+    double* x0; double* y0;
+    int nx, ny, neq, snap;
+};
+
+template <int TA, int TB, int EPI = 0>
 __global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, const uint16_t* __restrict__ A, int lda,
                                               const uint16_t* __restrict__ B, int ldb, float* __restrict__ C,
-                                              int ldc, int accumulate, int va, int vb) {
+                                              int ldc, int accumulate, int va, int vb, Epilogue ep) {
     __shared__ __align__(16) uint16_t As[BM][BK + PAD];
     __shared__ __align__(16) uint16_t Bs[BN][BK + PAD];
     constexpr bool AK = !TA, BKM = TB != 0;   // k-contiguous operands
@@ -158,19 +182,43 @@ __global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, const uint16_
             for (int r = 0; r < 16; ++r) {
                 const int row = m0 + wm + i * 32 + (r / 4) * 8 + (lane / 32) * 4 + (r % 4);
                 const int col = n0 + wn + j * 32 + lane % 32;
-                if (row < M && col < N) {
-                    const long o = (long)row * ldc + col;
-                    C[o] = accumulate ? C[o] + acc[i][j][r] : acc[i][j][r];
+                if constexpr (EPI == 0) {
+                    if (row < M && col < N) {
+                        const long o = (long)row * ldc + col;
+                        C[o] = accumulate ? C[o] + acc[i][j][r] : acc[i][j][r];
+                    }
+                } else if constexpr (EPI == 1) {
+                    if (row < M && col < N) ep.H[(long)row * ep.ldh + col] = f2bf(gelu(acc[i][j][r] + ep.bias[col]));
+                } else {
+                    // lanes l and l ^ 1 hold columns col and col ^ 1 of the same row:
+                    // the even lane of a (x_j, logit_j) pair reads the logit from its
+                    // neighbour (every lane takes part in the exchange)
+                    const float v = col < N ? acc[i][j][r] + ep.bias[col] : 0.0f;
+                    const float other = __shfl_xor(v, 1);
+                    if (row < M && col < N) {
+                        if (col < 2 * ep.nx) {
+                            if ((col & 1) == 0) {
+                                const int xj = col >> 1;
+                                const long o = (long)row * ep.nx + xj;
+                                const double lo = ep.lower[o], hi = ep.upper[o];
+                                double xv = (double)(v * ep.scale[col] + ep.shift[col]);
+                                xv = fmin(fmax(xv, lo), hi);
+                                if (ep.snap && other > 0.0f) {
+                                    const bool flo = isfinite(lo), fhi = isfinite(hi);
+                                    if (flo && (!fhi || xv - lo <= hi - xv)) xv = lo;
+                                    else if (fhi) xv = hi;
+                                }
+                                ep.x0[o] = xv;
+                            }
+                        } else if (col < 2 * ep.nx + ep.ny) {
+                            const int yi = col - 2 * ep.nx;
+                            double yv = (double)(v * ep.scale[col] + ep.shift[col]);
+                            if (yi >= ep.neq) yv = fmin(yv, 0.0);
+                            ep.y0[(long)row * ep.ny + yi] = yv;
+                        }
+                    }
                 }
             }
-}
-
-constexpr float SQRT_2_OVER_PI = 0.7978845608028654f;
-constexpr float GELU_C = 0.044715f;
-
-__device__ __forceinline__ float gelu(float z) {
-    const float u = SQRT_2_OVER_PI * (z + GELU_C * z * z * z);
-    return 0.5f * z * (1.0f + tanhf(u));
 }
 
 __device__ __forceinline__ float gelu_grad(float z) {
@@ -262,6 +310,16 @@ __global__ void k_to_bf16(const float* x, uint16_t* y, long n) {
     if (i < n) y[i] = f2bf(x[i]);
 }
 
+// X[r][c] = bf16((F[r][c] - mean[c]) / std[c]) for c < n_in, 0 for n_in <= c < ldx.
+__global__ void k_standardize(const float* F, const float* mean, const float* sd, uint16_t* X, long M, int n_in,
+                              int ldx) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M * ldx) return;
+    const long r = i / ldx;
+    const int c = (int)(i - r * ldx);
+    X[i] = c < n_in ? f2bf((F[r * n_in + c] - mean[c]) / sd[c]) : (uint16_t)0;
+}
+
 // Y[C][R] = X[R][C]^T through a 64 x 65 LDS tile (coalesced reads and writes).
 __global__ void __launch_bounds__(256) k_transpose(const uint16_t* __restrict__ X, uint16_t* __restrict__ Y, int R,
                                                    int C) {
@@ -296,13 +354,56 @@ int ws_gemm_bf16(int M, int N, int K, const uint16_t* A, int lda, int trans_a, c
     const int va = (lda % 8 == 0 && ((uintptr_t)A & 15) == 0) ? 1 : 0;
     const int vb = (ldb % 8 == 0 && ((uintptr_t)B & 15) == 0) ? 1 : 0;
     if (!trans_a && !trans_b)
-        hipLaunchKernelGGL((k_gemm<0, 0>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate, va, vb);
+        hipLaunchKernelGGL((k_gemm<0, 0>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate, va, vb, Epilogue{});
     else if (!trans_a && trans_b)
-        hipLaunchKernelGGL((k_gemm<0, 1>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate, va, vb);
+        hipLaunchKernelGGL((k_gemm<0, 1>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate, va, vb, Epilogue{});
     else if (trans_a && !trans_b)
-        hipLaunchKernelGGL((k_gemm<1, 0>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate, va, vb);
+        hipLaunchKernelGGL((k_gemm<1, 0>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate, va, vb, Epilogue{});
     else
-        hipLaunchKernelGGL((k_gemm<1, 1>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate, va, vb);
+        hipLaunchKernelGGL((k_gemm<1, 1>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate, va, vb, Epilogue{});
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int ws_gemm_bf16_act(int M, int N, int K, const uint16_t* A, int lda, const uint16_t* B, int ldb, const float* bias,
+                     uint16_t* H, int ldh, void* stream) {
+    if (M < 0 || N < 0 || K < 0 || !A || !B || !bias || !H || ldh < N) return -1;
+    if (M == 0 || N == 0) return 0;
+    dim3 grid(blocks(N, BN), blocks(M, BM));
+    const int va = (lda % 8 == 0 && ((uintptr_t)A & 15) == 0) ? 1 : 0;
+    const int vb = (ldb % 8 == 0 && ((uintptr_t)B & 15) == 0) ? 1 : 0;
+    Epilogue ep{};
+    ep.bias = bias; ep.H = H; ep.ldh = ldh;
+    hipLaunchKernelGGL((k_gemm<0, 1, 1>), grid, dim3(256), 0, (hipStream_t)stream, M, N, K, A, lda, B, ldb,
+                       (float*)nullptr, 0, 0, va, vb, ep);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int ws_gemm_bf16_start(int M, int nx, int ny, int K, const uint16_t* A, int lda, const uint16_t* B, int ldb,
+                       const float* bias, const float* scale, const float* shift, const double* lower,
+                       const double* upper, int neq, int snap, double* x0, double* y0, void* stream) {
+    if (M < 0 || nx < 0 || ny < 0 || K < 0 || neq < 0 || neq > ny || !A || !B || !bias || !scale || !shift ||
+        (nx > 0 && (!lower || !upper || !x0)) || (ny > 0 && !y0))
+        return -1;
+    const int N = 2 * nx + ny;
+    if (M == 0 || N == 0) return 0;
+    dim3 grid(blocks(N, BN), blocks(M, BM));
+    const int va = (lda % 8 == 0 && ((uintptr_t)A & 15) == 0) ? 1 : 0;
+    const int vb = (ldb % 8 == 0 && ((uintptr_t)B & 15) == 0) ? 1 : 0;
+    Epilogue ep{};
+    ep.bias = bias; ep.scale = scale; ep.shift = shift; ep.lower = lower; ep.upper = upper;
+    ep.x0 = x0; ep.y0 = y0; ep.nx = nx; ep.ny = ny; ep.neq = neq; ep.snap = snap;
+    hipLaunchKernelGGL((k_gemm<0, 1, 2>), grid, dim3(256), 0, (hipStream_t)stream, M, N, K, A, lda, B, ldb,
+                       (float*)nullptr, 0, 0, va, vb, ep);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int ws_standardize(const float* F, const float* mean, const float* sd, uint16_t* X, int M, int n_in, int ldx,
+                   void* stream) {
+    if (!F || !mean || !sd || !X || M < 0 || n_in < 0 || ldx < n_in) return -1;
+    const long total = (long)M * ldx;
+    if (total == 0) return 0;
+    hipLaunchKernelGGL(k_standardize, dim3(blocks(total, 256)), dim3(256), 0, (hipStream_t)stream, F, mean, sd, X,
+                       (long)M, n_in, ldx);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

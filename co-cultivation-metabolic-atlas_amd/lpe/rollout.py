@@ -23,7 +23,8 @@ Each step mirrors the reference's dFBA step for the cooperative consortium
 All per-environment state is a torch tensor on the GPU; the LPs of one step
 are one batched solve.  Warm starts: "previous" (the last step's primal and
 dual), "none" (cold) or a callable (conc, lower, biomass, x_prev, y_prev) ->
-(x0, y0) such as the learned model of ``lpe.warmstart``.
+(x0, y0) such as the learned model of ``lpe.warmstart`` (which also receives
+the engine, already updated to the step's LPs, to score candidate starts).
 """
 
 from __future__ import annotations
@@ -58,7 +59,7 @@ class ConsortiumRollout:
                  warm_start="previous", dt: float = 1.0, conc_range=(1e-2, 10.0),
                  autotune: bool = True, fallback: bool = True, device: Optional[torch.device] = None,
                  method: str = "restarted", reduce: bool = False, flux_unit: float = FLUX_UNIT,
-                 capacity: float = CARRYING_CAPACITY):
+                 capacity: float = CARRYING_CAPACITY, engine=None, tuning: Optional[dict] = None):
         self.model = model if model is not None else consortium(seed)
         self.flux_unit = float(flux_unit)
         self.capacity = float(capacity)
@@ -68,7 +69,13 @@ class ConsortiumRollout:
         self.device = device or torch.device("cuda", torch.cuda.current_device())
         self.options = options or SolveOptions()
         self.warm_start = warm_start
-        if reduce:
+        if engine is not None:
+            # an existing engine of this model and batch (its graphs and launch
+            # shape carry over, e.g. from a warm-up rollout)
+            if engine.batch != self.n_env or tuple(engine.A.shape) != tuple(md.shape):
+                raise ValueError("the shared engine must match the model's shape and the number of environments")
+            self.engine = engine
+        elif reduce:
             # exact equality reduction: the engine iterates on the reduced LP,
             # accepts on the original certificate (the real-GEM path)
             from .reduction import ReducedLpEngine
@@ -105,7 +112,9 @@ class ConsortiumRollout:
         self.omega = None
         self.autotune = autotune
         self.fallback = fallback
-        self.tuning = None
+        self.tuning = tuning
+        if tuning is not None:
+            self.options.check_interval = int(tuning["chunk"])
         self.step_index = 0
         self.log: list[dict] = []
         self.track_active = False   # record per-step active-set changes in the log
@@ -157,8 +166,13 @@ class ConsortiumRollout:
             self.tuning = autotune(eng)
             self.options.check_interval = int(self.tuning["chunk"])
         x0 = y0 = None
+        start_mlp = None
         if callable(self.warm_start):
-            x0, y0 = self.warm_start(self.conc, lower, self.biomass, self.x, self.y)
+            if getattr(self.warm_start, "wants_engine", False):
+                x0, y0 = self.warm_start(self.conc, lower, self.biomass, self.x, self.y, engine=eng)
+                start_mlp = getattr(self.warm_start, "last_choice", None)
+            else:
+                x0, y0 = self.warm_start(self.conc, lower, self.biomass, self.x, self.y)
         elif self.warm_start == "previous" and self.x is not None:
             x0, y0 = self.x, self.y
         eng.reset(x0, y0)
@@ -177,6 +191,7 @@ class ConsortiumRollout:
         torch.cuda.synchronize()
         info["wall_seconds"] = time.perf_counter() - t0
         info["step"] = self.step_index
+        info["start_mlp"] = start_mlp
         if self.track_active:
             self._record_active(info, x, y, lower, upper, certified)
         self.log.append(info)

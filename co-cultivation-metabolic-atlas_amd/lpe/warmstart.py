@@ -28,7 +28,7 @@ import torch
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libwarmstart.so")
 EXPORTS = ("ws_gemm_bf16", "ws_bias_act", "ws_act_backward", "ws_loss", "ws_adam", "ws_transpose_bf16",
-           "ws_to_bf16", "ws_version")
+           "ws_to_bf16", "ws_gemm_bf16_act", "ws_gemm_bf16_start", "ws_standardize", "ws_version")
 
 _lib = None
 
@@ -49,6 +49,9 @@ def load():
     lib.ws_adam.argtypes = [P, P, P, P, P, C.c_int64, F, F, F, F, I, P]
     lib.ws_transpose_bf16.argtypes = [P, P, I, I, P]
     lib.ws_to_bf16.argtypes = [P, P, C.c_int64, P]
+    lib.ws_gemm_bf16_act.argtypes = [I, I, I, P, I, P, I, P, P, I, P]
+    lib.ws_gemm_bf16_start.argtypes = [I, I, I, I, P, I, P, I, P, P, P, P, P, I, I, P, P, P]
+    lib.ws_standardize.argtypes = [P, P, P, P, I, I, I, P]
     lib.ws_version.restype = C.c_char_p
     for name in EXPORTS[:-1]:
         getattr(lib, name).restype = C.c_int
@@ -147,6 +150,7 @@ class WarmStartNet:
         self.step_count = 0
         self.in_mean = self.in_std = self.out_mean = self.out_std = None
         self.trained = False
+        self._inf = None
 
     @property
     def n_params(self) -> int:
@@ -220,6 +224,66 @@ class WarmStartNet:
         reg = o[:, :k] * self.out_std + self.out_mean
         return reg[:, :self.nx], reg[:, self.nx:k], o[:, k:k + self.na]
 
+    # ------------------------------------------------- fused inference
+    def _inference_layout(self):
+        """Output layer for ``infer_start``: rows of the [out][in] BF16 weights
+        interleaved as (x_j, logit_j) for j < nx, then the y rows, with the
+        matching bias and the de-standardisation (scale, shift) per row."""
+        if getattr(self, "_inf_step", None) == self.step_count and self._inf is not None:
+            return self._inf
+        nx, ny = self.nx, self.ny
+        if self.na < nx:
+            raise ValueError("the fused output layer needs one bound logit per x column")
+        dev = self.device
+        jx = torch.arange(nx, device=dev)
+        order = torch.empty(2 * nx + ny, dtype=torch.long, device=dev)
+        order[0:2 * nx:2] = jx
+        order[1:2 * nx:2] = nx + ny + jx
+        order[2 * nx:] = nx + torch.arange(ny, device=dev)
+        w_out = self.wbt[-1][order].contiguous()
+        bias = self.b[-1][order].contiguous()
+        scale = torch.ones(2 * nx + ny, dtype=torch.float32, device=dev)
+        shift = torch.zeros(2 * nx + ny, dtype=torch.float32, device=dev)
+        scale[0:2 * nx:2] = self.out_std[:nx]
+        shift[0:2 * nx:2] = self.out_mean[:nx]
+        scale[2 * nx:] = self.out_std[nx:nx + ny]
+        shift[2 * nx:] = self.out_mean[nx:nx + ny]
+        self._inf = (w_out, bias, scale, shift)
+        self._inf_step = self.step_count
+        return self._inf
+
+    def infer_start(self, features: torch.Tensor, lower: torch.Tensor, upper: torch.Tensor, neq: int,
+                    snap: bool = True):
+        """Warm start (x0 [M, nx], y0 [M, ny], FP64) straight from the network:
+        input standardisation, three GEMMs with fused bias + GELU writing BF16,
+        and the output GEMM whose epilogue de-standardises, projects x onto
+        [lower, upper], snaps predicted at-bound variables and clips the
+        inequality duals (ws_standardize, ws_gemm_bf16_act, ws_gemm_bf16_start)."""
+        lib = load()
+        s = _stream()
+        mb = features.shape[0]
+        w_out, bias, scale, shift = self._inference_layout()
+        f = features.to(torch.float32).contiguous()
+        xb = torch.empty((mb, self.n_in_p), dtype=torch.bfloat16, device=self.device)
+        _ok(lib.ws_standardize(_ptr(f), _ptr(self.in_mean), _ptr(self.in_std), _ptr(xb), mb, self.n_in,
+                               self.n_in_p, s), "ws_standardize")
+        h = xb
+        for l in range(len(self.w) - 1):
+            n, k = self.dims[l + 1], self.dims[l]
+            hn = torch.empty((mb, n), dtype=torch.bfloat16, device=self.device)
+            _ok(lib.ws_gemm_bf16_act(mb, n, k, _ptr(h), k, _ptr(self.wbt[l]), k, _ptr(self.b[l]), _ptr(hn), n, s),
+                "ws_gemm_bf16_act")
+            h = hn
+        lo = lower.to(torch.float64).contiguous()
+        up = upper.to(torch.float64).contiguous()
+        x0 = torch.empty((mb, self.nx), dtype=torch.float64, device=self.device)
+        y0 = torch.empty((mb, self.ny), dtype=torch.float64, device=self.device)
+        k = self.dims[-2]
+        _ok(lib.ws_gemm_bf16_start(mb, self.nx, self.ny, k, _ptr(h), k, _ptr(w_out), k, _ptr(bias), _ptr(scale),
+                                   _ptr(shift), _ptr(lo), _ptr(up), int(neq), 1 if snap else 0, _ptr(x0), _ptr(y0),
+                                   s), "ws_gemm_bf16_start")
+        return x0, y0
+
     # ------------------------------------------------------------ training
     def loss_and_grad(self, xb: torch.Tensor, tb: torch.Tensor):
         """Forward + backward on a standardised BF16 batch; returns (loss,
@@ -292,41 +356,72 @@ class WarmStartNet:
 
 # --------------------------------------------------------------- dFBA step
 def consortium_features(model, conc: torch.Tensor, lower: torch.Tensor, biomass: torch.Tensor) -> torch.Tensor:
-    """[B, 2K + S + S*K]: log concentrations, concentrations, biomass and the
-    uptake lower bounds of every species' medium exchanges."""
+    """[B, K + S*K] network input: log medium concentrations and the exchange
+    bounds of every (species, metabolite) term scaled by the species' biomass,
+    X_s * lb_(s,i) - the bound of that term in the medium row, in mmol/L/h.
+    Biomass enters the LP only through those rows' coefficients and is
+    recoverable from these inputs (lb depends on C alone), so 60 + 180 = 240
+    values determine the consortium LP."""
+    s_idx, _ = model.terms()
     mc = torch.as_tensor(model.term_cols, device=lower.device, dtype=torch.long)
-    return torch.cat([torch.log(conc + 1e-6), conc, biomass, lower[:, mc]], dim=1).to(torch.float32)
+    si = torch.as_tensor(s_idx, device=lower.device, dtype=torch.long)
+    return torch.cat([torch.log(conc + 1e-6), biomass[:, si] * lower[:, mc]], dim=1).to(torch.float32)
 
 
 def bound_labels(x: torch.Tensor, lower: torch.Tensor, upper: torch.Tensor, tol: float = 1e-6) -> torch.Tensor:
-    width = torch.clamp(upper - lower, min=1.0)
-    at = ((x - lower) <= tol * width) | ((upper - x) <= tol * width)
+    """1 where x sits at a finite bound (within tol * max(1, width)), else 0."""
+    width = torch.clamp(torch.nan_to_num(upper - lower, posinf=1.0), min=1.0)
+    at = (torch.isfinite(lower) & ((x - lower) <= tol * width)) | (torch.isfinite(upper) & ((upper - x) <= tol * width))
     return at.to(torch.float32)
+
+
+def start_score(metrics: torch.Tensor) -> torch.Tensor:
+    """Distance of a candidate start from optimality, per LP, from its FP64
+    certificate [B, 8]: the largest of the primal residual, the dual violation
+    and the relative complementarity gap, each over its gate (1e-5, 1e-7,
+    1e-7); non-finite candidates score +inf."""
+    primal, dual, gap, finite = metrics[:, 0] / 1e-5, metrics[:, 1] / 1e-7, metrics[:, 2] / 1e-7, metrics[:, 5]
+    sc = torch.maximum(primal, torch.maximum(dual, gap))
+    return torch.where((finite > 0.5) & torch.isfinite(sc), sc, torch.full_like(sc, float("inf")))
 
 
 class LearnedWarmStart:
     """Warm-start callable for ``ConsortiumRollout``.
 
-    mode "predict": x0, y0 = the network's prediction for this step's LPs,
-    variables the network places at a bound snapped there.
-    mode "correct": the previous step's solution moved by the network's
-    predicted change, x0 = x_prev + f(p_t) - f(p_{t-1}) (the systematic error
-    of f cancels); the first step falls back to "predict".
+    mode "hybrid" (default): an LP starts from the previous step's certified
+    solution when that solution, scored by ``start_score`` of its FP64
+    certificate on the new LP (``engine.certify``), is within ``keep_score``
+    times the certificate gates, and from the network's prediction otherwise
+    (or when it has no previous solution).  The network is only evaluated
+    when some LP needs it.  The rule and the default of 100 are measured
+    (1024 consortium environments): the previous solution scores ~5e3
+    (median) one step after the start of a rollout, when biomass has grown by
+    tens of percent, and needs 352 PDHG iterations and 3 polish rounds where
+    the network's start needs 64 and 1; from the second step on it scores
+    below ~10 and needs 32 iterations (0 once the rollout settles) where the
+    network's start needs 64.  The network's own score (1e4-3e4, a larger
+    residual but the right active set) does not rank its starts, so it is
+    not used for the choice.
+    mode "predict": the network's prediction for every LP.
     Before training it passes the previous solution through (None on the
-    first step: a cold start)."""
+    first step: a cold start, the method without the network).  The last
+    call's choice is in ``last_choice`` (LPs started from the network)."""
+
+    wants_engine = True
 
     def __init__(self, model, *, hidden: int = 1024, layers: int = 3, device=None, seed: int = 0,
-                 snap: bool = True, mode: str = "predict"):
-        if mode not in ("predict", "correct"):
-            raise ValueError("mode must be 'predict' or 'correct'")
+                 snap: bool = True, mode: str = "hybrid", keep_score: float = 100.0):
+        if mode not in ("hybrid", "predict"):
+            raise ValueError("mode must be 'hybrid' or 'predict'")
         self.model = model
         m, n = model.shape
-        self.n_features = 2 * model.n_medium + model.n_species + len(model.term_cols)
+        self.n_features = model.n_medium + len(model.term_cols)
         self.net = WarmStartNet(self.n_features, n, m, n, hidden=hidden, layers=layers, device=device, seed=seed)
         self.snap = snap
         self.mode = mode
+        self.keep_score = float(keep_score)
         self.features, self.targets = [], []
-        self._last = None
+        self.last_choice = None
 
     def add_solutions(self, conc, lower, upper, biomass, x, y, certified) -> int:
         """Keep the certified LPs of one solved step as training samples."""
@@ -347,27 +442,33 @@ class LearnedWarmStart:
     def train(self, **kwargs) -> list:
         f = torch.cat(self.features)
         t = torch.cat(self.targets)
-        return self.net.fit(f, t, **kwargs)
+        self.features, self.targets = [f], [t]
+        hist = self.net.fit(f, t, **kwargs)
+        # load the scoring kernels now: their first use costs ~0.1 s, which
+        # would otherwise land on the first step that scores previous solutions
+        start_score(torch.zeros((1, 8), dtype=torch.float64, device=self.net.device))
+        torch.cuda.synchronize(self.net.device)
+        return hist
 
     def predict(self, conc, lower, biomass):
+        """(x0, y0) of the network for every LP (fused inference)."""
         md = self.model
         upper = torch.as_tensor(md.upper, device=lower.device).expand_as(lower)
-        x, y, logits = self.net.predict(consortium_features(md, conc, lower, biomass))
-        x = torch.minimum(torch.maximum(x.to(torch.float64), lower), upper)
-        if self.snap:
-            near_lo = (x - lower) <= (upper - x)
-            at = logits > 0
-            x = torch.where(at & near_lo, lower, torch.where(at, upper, x))
-        return x.contiguous(), y.to(torch.float64).contiguous()
+        return self.net.infer_start(consortium_features(md, conc, lower, biomass), lower, upper, md.neq,
+                                    snap=self.snap)
 
-    def __call__(self, conc, lower, biomass, x_prev=None, y_prev=None):
+    def __call__(self, conc, lower, biomass, x_prev=None, y_prev=None, engine=None):
         if not self.net.trained:
+            self.last_choice = None
             return x_prev, y_prev
-        x, y = self.predict(conc, lower, biomass)
-        last, self._last = self._last, (x, y)
-        if self.mode == "correct" and last is not None and x_prev is not None:
-            md = self.model
-            upper = torch.as_tensor(md.upper, device=lower.device).expand_as(lower)
-            xc = torch.minimum(torch.maximum(x_prev + (x - last[0]), lower), upper)
-            return xc.contiguous(), (y_prev + (y - last[1])).contiguous()
-        return x, y
+        B = conc.shape[0]
+        if self.mode == "predict" or x_prev is None or engine is None or not hasattr(engine, "certify"):
+            self.last_choice = B
+            return self.predict(conc, lower, biomass)
+        use = start_score(engine.certify(x_prev, y_prev)) > self.keep_score
+        self.last_choice = int(use.sum())
+        if self.last_choice == 0:
+            return x_prev, y_prev
+        x_m, y_m = self.predict(conc, lower, biomass)
+        return (torch.where(use[:, None], x_m, x_prev).contiguous(),
+                torch.where(use[:, None], y_m, y_prev).contiguous())

@@ -26,6 +26,25 @@ extern "C" {
 int ws_gemm_bf16(int M, int N, int K, const uint16_t *A, int lda, int trans_a, const uint16_t *B, int ldb,
                  int trans_b, float *C, int ldc, int accumulate, void *stream);
 
+/* Inference products with fused epilogues (op(A) = A[M][K], op(B) = the
+ * transpose of B[N][K], both k-contiguous BF16):
+ * ws_gemm_bf16_act: H[M][N] (leading dimension ldh) = bf16(gelu(A B^T + bias)).
+ * ws_gemm_bf16_start: the warm-start output layer.  B's N = 2 nx + ny rows
+ * interleave (x_j, logit_j) for j < nx, then y_i; with v = (A B^T + bias)
+ * per column, x0[r][j] = clamp(v * scale + shift, lower[r][j], upper[r][j]),
+ * moved to the nearer finite bound when snap and logit_j > 0, and y0[r][i] =
+ * v * scale + shift (clipped to <= 0 for i >= neq), stored FP64 batch-major
+ * (lower, upper: FP64 [M][nx]).  Returns 0, -1 on invalid arguments. */
+int ws_gemm_bf16_act(int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb,
+                     const float *bias, uint16_t *H, int ldh, void *stream);
+int ws_gemm_bf16_start(int M, int nx, int ny, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb,
+                       const float *bias, const float *scale, const float *shift, const double *lower,
+                       const double *upper, int neq, int snap, double *x0, double *y0, void *stream);
+
+/* X[M][ldx] = bf16((F[M][n_in] - mean) / std), columns n_in..ldx-1 zero. */
+int ws_standardize(const float *F, const float *mean, const float *std, uint16_t *X, int M, int n_in, int ldx,
+                   void *stream);
+
 /* Z[M][N] += bias[N]; H = gelu(Z) as BF16 (tanh form) into H[M][N]; Z keeps
  * the pre-activation for the backward pass.  gelu == 0: H = bf16(Z).  H may be
  * NULL (output layer: only Z is needed). */

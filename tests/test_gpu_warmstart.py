@@ -145,3 +145,50 @@ def test_training_reduces_loss(ws, device):
     hist = net.fit(torch.tensor(feats, dtype=torch.float32, device=device),
                    torch.tensor(targets, dtype=torch.float32, device=device), epochs=30, batch=256, lr=3e-3)
     assert hist[-1] < 0.35 * hist[0], hist
+
+
+def _snap_reference(x, logits, lower, upper):
+    """Projection onto [lower, upper], then predicted at-bound variables moved
+    to the nearer finite bound (the rule of the fused output epilogue)."""
+    x = torch.minimum(torch.maximum(x, lower), upper)
+    flo, fhi = torch.isfinite(lower), torch.isfinite(upper)
+    to_lo = flo & (~fhi | ((x - lower) <= (upper - x)))
+    at = logits > 0
+    return torch.where(at & to_lo, lower, torch.where(at & fhi, upper, x))
+
+
+def test_fused_inference_equals_the_unfused_forward(ws, device):
+    """infer_start (standardisation, bias + GELU epilogues, the interleaved
+    output layer with de-standardisation, projection and snapping) equals the
+    unfused forward followed by the same host-side steps, bit for bit - with
+    free, one-sided and boxed columns."""
+    nx, ny, na, neq = 50, 30, 50, 22
+    net = ws.WarmStartNet(37, nx, ny, na, hidden=128, layers=2, device=device, seed=5)
+    g = torch.Generator(device="cpu").manual_seed(7)
+    feats = torch.randn(300, 37, generator=g).to(device)
+    targ = torch.randn(300, nx + ny + na, generator=g).to(device) * 3.0 + 1.0
+    net.fit_normalization(feats, targ)
+    for l in range(len(net.w)):                   # a trained-looking network: larger random weights
+        net.w[l].copy_(torch.randn(net.w[l].shape, generator=g).to(device) / np.sqrt(net.w[l].shape[0]))
+        net.b[l].copy_(torch.randn(net.b[l].shape, generator=g).to(device) * 0.1)
+        ws.to_bf16(net.w[l], net.wb[l])
+        ws.transpose(net.wb[l], net.wbt[l])
+    net.trained = True
+    lower = torch.randn(300, nx, generator=g, dtype=torch.float64).to(device) - 2.0
+    upper = lower + torch.rand(300, nx, generator=g, dtype=torch.float64).to(device) * 6.0
+    lower[:, :5] = -float("inf"); upper[:, :5] = float("inf")      # free columns
+    lower[:, 5:10] = -float("inf")                                 # upper bound only
+    upper[:, 10:15] = float("inf")                                 # lower bound only
+    x0, y0 = net.infer_start(feats, lower, upper, neq)
+    xr, yr, logits = net.predict(feats)
+    x_ref = _snap_reference(xr.to(torch.float64), logits, lower, upper)
+    y_ref = yr.to(torch.float64)
+    y_ref[:, neq:] = torch.clamp(y_ref[:, neq:], max=0.0)
+    assert torch.isfinite(x0).all() and torch.isfinite(y0).all()
+    assert torch.equal(x0, x_ref)
+    assert torch.equal(y0, y_ref)
+    # predicted at-bound variables sit exactly on a finite bound; free ones are never snapped
+    at = logits > 0
+    on_bound = (x0 == lower) | (x0 == upper)
+    assert bool(on_bound[:, 5:][at[:, 5:]].all())
+    assert bool((x0[:, :5] == xr[:, :5].to(torch.float64)).all())
