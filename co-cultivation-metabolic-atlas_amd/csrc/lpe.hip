@@ -147,7 +147,7 @@ struct lpe_handle {
     size_t partial_elems = 0;
     double *d_red_a = nullptr, *d_red_b = nullptr;
     int *d_ops_cr = nullptr, *d_ops_cc = nullptr, *d_ops_xr = nullptr, *d_ops_xc = nullptr, *d_ops_sum = nullptr;
-    int *d_ops_pinf = nullptr, *d_ops_pinit = nullptr;
+    int *d_ops_pinf = nullptr, *d_ops_pinit = nullptr, *d_ops_ir = nullptr, *d_ops_ic = nullptr;
 
     // polish / results
     double *d_xs = nullptr, *d_rp_ = nullptr, *d_pp = nullptr, *d_wp = nullptr;
@@ -653,6 +653,49 @@ int launch_check_t(lpe_handle* h, int K, int certify, double ep, double ed, doub
     }
     HIPCHK(hipGetLastError());
     return LPE_OK;
+}
+
+// Infeasibility certificates of the running LPs from the iterate
+// differences since their last restart (k_infeas_*): a certificate marks an
+// LP primal or dual infeasible and freezes it.  Restarted and Halpern methods.
+// The dual ray is the dual iterate's change since the last restart; the
+// primal ray is the last step x_k - x_(k-1) (= (xbar - x) / theta for the
+// restarted-averaged method, whose extrapolated point holds it), which tends
+// to the primal ray while the converging components' steps vanish (the
+// change since the restart keeps their whole displacement), or the change
+// since the restart for Halpern iterations.
+template <typename T>
+int launch_infeasibility_t(lpe_handle* h, double eps) {
+    Iter<T>& it = iter_of<T>(h);
+    const bool step = !h->halpern && h->theta > 0.0;
+    const T* xa = step ? it.xbar : it.x;      // primal ray = xa - xb
+    const T* xb = step ? it.x : it.xlast;
+    int grid_r = 0, grid_c = 0;
+    Shape sr = make_shape(std::max(h->m, 1), h->ld, h->B, CHECK_RPW, 1, &grid_r);
+    Shape sc = make_shape(h->n, h->ld, h->B, CHECK_RPW, 1, &grid_c);
+    if (h->m > 0) {
+        InfRowsArgs<T> r;
+        r.sh = sr; r.A = op_scaled(h, false); r.x = xa; r.xlast = xb; r.y = it.y; r.ylast = it.ylast;
+        r.rhs = h->d_rhs_s; r.flags = h->d_flags; r.neq = h->neq; r.partial = h->d_partial;
+        hipLaunchKernelGGL((k_infeas_rows<T>), dim3(grid_r), dim3(BLOCK), 0, h->s, r);
+        reduce(h, h->d_partial, h->d_red_a, h->d_ops_ir, IR_N, sr.n_rb);
+    }
+    InfColsArgs<T> c;
+    c.sh = sc; c.AT = op_scaled(h, true); c.x = xa; c.xlast = xb;
+    c.y = h->m > 0 ? it.y : nullptr; c.ylast = h->m > 0 ? it.ylast : nullptr;
+    c.c = h->d_c_s; c.c_shared = h->shared_c; c.lower = h->d_lo_s; c.upper = h->d_up_s; c.flags = h->d_flags;
+    c.partial = h->d_partial;
+    hipLaunchKernelGGL((k_infeas_cols<T>), dim3(grid_c), dim3(BLOCK), 0, h->s, c);
+    reduce(h, h->d_partial, h->d_red_b, h->d_ops_ic, IC_N, sc.n_rb);
+    hipLaunchKernelGGL(k_infeas_decide, dim3(ceil_div(h->B, 256)), dim3(256), 0, h->s, h->d_st, h->d_flags,
+                       (const double*)h->d_red_a, (const double*)h->d_red_b, h->B, h->ld, h->m > 0 ? 1 : 0, eps);
+    HIPCHK(hipGetLastError());
+    return LPE_OK;
+}
+
+int launch_infeasibility(lpe_handle* h, double eps) {
+    if (h->method != LPE_METHOD_FIXED && h->precision == LPE_PREC_FP32) return launch_infeasibility_t<float>(h, eps);
+    return launch_infeasibility_t<double>(h, eps);
 }
 
 int launch_check(lpe_handle* h, int K, int certify, double ep, double ed, double eg) {
@@ -1324,6 +1367,10 @@ int alloc_all(lpe_handle* h) {
     const int ops_pinit[2] = {RSUM, RMAX};           // CG setup: rho_p, the primal residual maximum
     CHK(dalloc(h, &h->d_ops_pinf, 3)); CHK(upload(h->d_ops_pinf, ops_pinf, 3));
     CHK(dalloc(h, &h->d_ops_pinit, 2)); CHK(upload(h->d_ops_pinit, ops_pinit, 2));
+    const int ops_ir[IR_N] = {RSUM, RMAX, RMAX, RMAX};
+    const int ops_ic[IC_N] = {RSUM, RMAX, RSUM, RMAX};
+    CHK(dalloc(h, &h->d_ops_ir, IR_N)); CHK(upload(h->d_ops_ir, ops_ir, IR_N));
+    CHK(dalloc(h, &h->d_ops_ic, IC_N)); CHK(upload(h->d_ops_ic, ops_ic, IC_N));
     // polish and results
     CHK(dalloc(h, &h->d_xs, n * ld)); CHK(dalloc(h, &h->d_fmask, n * ld)); CHK(dalloc(h, &h->d_rmask, m * ld));
     CHK(dalloc(h, &h->d_rp_, m * ld)); CHK(dalloc(h, &h->d_pp, m * ld)); CHK(dalloc(h, &h->d_wp, m * ld));
@@ -1809,8 +1856,15 @@ int lpe_solve(lpe_handle* h, const lpe_solve_opts* o, lpe_solve_info* info, void
     HIPCHK(hipMemcpyAsync(h->h_st, h->d_st, sizeof(LpState) * h->ld, hipMemcpyDeviceToHost, h->s));
     HIPCHK(hipStreamSynchronize(h->s));
     for (int b = 0; b < h->B; ++b) running += h->h_st[b].status == 0;
+    // infeasibility certificates every infeas_every-th check from the 8th on
+    // (the iterate differences since the last restart must have grown first)
+    const int infeas_every = o->infeasibility_interval < 0 ? 0 : (o->infeasibility_interval > 0 ? o->infeasibility_interval : 4);
+    const double eps_inf = o->eps_infeasible > 0 ? o->eps_infeasible : 1e-8;
     auto after_chunk = [&](int k) -> int {
         HIPCHK(hipEventRecord(e0, h->s));
+        if (k > 0 && infeas_every > 0 && h->method == LPE_METHOD_RESTARTED && inf.chunks >= 8 &&
+            inf.chunks % infeas_every == 0)
+            CHK(launch_infeasibility(h, eps_inf));
         CHK(launch_check(h, k, 1, ep, ed, eg));
         HIPCHK(hipMemcpyAsync(h->h_st, h->d_st, sizeof(LpState) * h->ld, hipMemcpyDeviceToHost, h->s));
         HIPCHK(hipEventRecord(e1, h->s));

@@ -663,6 +663,124 @@ __global__ void __launch_bounds__(BLOCK) k_check_cols(CheckColsArgs<T> a) {
 
 
 // ---------------------------------------------------------------------------
+// Infeasibility certificates from the iterate differences since the last
+// restart (PDLP: Applegate et al., "Practical large-scale linear programming
+// using primal-dual hybrid gradient", 2021, sec. 5; cuPDLP: Lu & Yang, 2023),
+// on the scaled LP, in FP64.  With dx = x - x_last, dy = y - y_last (the
+// launcher passes the last step's end points as x and x_last for dx):
+//  * dual ray (primal infeasible): r = -A^T dy, objective
+//      F = b.dy + sum_j (r_j > 0 ? r_j l_j : r_j u_j)   (finite bounds only)
+//    violations: dy_i > 0 on inequality rows, r_j > 0 with l_j = -inf,
+//    r_j < 0 with u_j = +inf;
+//  * primal ray (dual infeasible, unbounded): objective c.dx < 0, violations:
+//    |A dx| on equality rows, (A dx)+ on inequality rows, dx_j < 0 with l_j
+//    finite, dx_j > 0 with u_j finite.
+// A ray is a certificate when its objective has the right sign and its
+// largest violation is at most eps times |objective| (both scale with the ray).
+// ---------------------------------------------------------------------------
+enum InfRowVal { IR_BDY = 0, IR_DYPOS, IR_EQV, IR_UBV, IR_N };
+enum InfColVal { IC_BT = 0, IC_RV, IC_CDX, IC_DXV, IC_N };
+
+template <typename T>
+struct InfRowsArgs {
+    Shape sh;
+    Op<double> A;            // scaled FP64
+    const T* x; const T* xlast; const T* y; const T* ylast;
+    const double* rhs;       // scaled [m][ld]
+    const int* flags;
+    int neq;
+    double* partial;         // [n_rb][IR_N][ld]
+};
+
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_infeas_rows(InfRowsArgs<T> a) {
+    __shared__ double lds[WAVES_PER_BLOCK * IR_N * WAVE];
+    int tile, rb;
+    if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int b0 = tile * WAVE + lane;
+    const bool valid = b0 < a.sh.ld;
+    const long ld = a.sh.ld;
+    double v[IR_N][1] = {{0.0}, {0.0}, {0.0}, {0.0}};
+    if (valid && !a.flags[b0]) {
+        const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
+        for (int r = 0; r < a.sh.rpw; ++r) {
+            const int row = uniform(r0 + r);
+            if (row >= a.sh.rows) break;
+            const long i = row * ld + b0;
+            double ax[1], axl[1];
+            row_dot<double, T, double, 1>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.x, ld, b0, row, ax);
+            row_dot<double, T, double, 1>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.xlast, ld, b0, row, axl);
+            const double adx = ax[0] - axl[0];
+            const double dy = (double)a.y[i] - (double)a.ylast[i];
+            v[IR_BDY][0] += a.rhs[i] * dy;
+            if (row < a.neq) {
+                v[IR_EQV][0] = fmax(v[IR_EQV][0], __builtin_fabs(adx));
+            } else {
+                v[IR_DYPOS][0] = fmax(v[IR_DYPOS][0], dy);
+                v[IR_UBV][0] = fmax(v[IR_UBV][0], adx);
+            }
+        }
+    }
+    const int ops[IR_N] = {RSUM, RMAX, RMAX, RMAX};
+    block_partials<IR_N, 1>(v, ops, a.partial, rb, ld, b0, lds, valid);
+}
+
+template <typename T>
+struct InfColsArgs {
+    Shape sh;
+    Op<double> AT;           // scaled FP64
+    const T* x; const T* xlast; const T* y; const T* ylast;
+    const double* c;         // scaled, [n] or [n][ld]
+    const double* lower; const double* upper;   // scaled [n][ld]
+    const int* flags;
+    int c_shared;
+    double* partial;         // [n_rb][IC_N][ld]
+};
+
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_infeas_cols(InfColsArgs<T> a) {
+    __shared__ double lds[WAVES_PER_BLOCK * IC_N * WAVE];
+    int tile, rb;
+    if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int b0 = tile * WAVE + lane;
+    const bool valid = b0 < a.sh.ld;
+    const long ld = a.sh.ld;
+    double v[IC_N][1] = {{0.0}, {0.0}, {0.0}, {0.0}};
+    if (valid && !a.flags[b0]) {
+        const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
+        for (int r = 0; r < a.sh.rpw; ++r) {
+            const int col = uniform(r0 + r);
+            if (col >= a.sh.rows) break;
+            const long i = col * ld + b0;
+            double aty[1] = {0.0}, atyl[1] = {0.0};
+            if (a.y) {
+                row_dot<double, T, double, 1>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.y, ld, b0, col, aty);
+                row_dot<double, T, double, 1>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.ylast, ld, b0, col, atyl);
+            }
+            const double rr = -(aty[0] - atyl[0]);
+            const double lo = a.lower[i], hi = a.upper[i];
+            const bool lfin = __builtin_isfinite(lo), ufin = __builtin_isfinite(hi);
+            if (rr > 0.0) {
+                if (lfin) v[IC_BT][0] += rr * lo;
+                else v[IC_RV][0] = fmax(v[IC_RV][0], rr);
+            } else if (rr < 0.0) {
+                if (ufin) v[IC_BT][0] += rr * hi;
+                else v[IC_RV][0] = fmax(v[IC_RV][0], -rr);
+            }
+            const double dx = (double)a.x[i] - (double)a.xlast[i];
+            const double cc = a.c_shared ? a.c[col] : a.c[i];
+            v[IC_CDX][0] += cc * dx;
+            if (lfin && dx < 0.0) v[IC_DXV][0] = fmax(v[IC_DXV][0], -dx);
+            if (ufin && dx > 0.0) v[IC_DXV][0] = fmax(v[IC_DXV][0], dx);
+        }
+    }
+    const int ops[IC_N] = {RSUM, RMAX, RSUM, RMAX};
+    block_partials<IC_N, 1>(v, ops, a.partial, rb, ld, b0, lds, valid);
+}
+
+// ---------------------------------------------------------------------------
 // Per-LP restart / acceptance decisions (one thread per LP).
 //
 // Restart rule (PDLP: Applegate et al., NeurIPS 2021, sec. 3; cuPDLP.jl, Lu &
@@ -692,6 +810,30 @@ struct LpState {
 };
 
 enum Action { ACT_NONE = 0, ACT_RESTART_CUR = 1, ACT_RESTART_AVG = 2, ACT_ACCEPT_CUR = 3, ACT_ACCEPT_AVG = 4 };
+
+// One thread per LP: a certificate marks a running LP primal (3) or dual (4)
+// infeasible and freezes it.
+__global__ void k_infeas_decide(LpState* st, int* flags, const double* rows, const double* cols, int batch, int ld,
+                                int have_rows, double eps) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch || flags[b] || st[b].status != 0) return;
+    const double bdy = have_rows ? rows[IR_BDY * (long)ld + b] : 0.0;
+    const double dypos = have_rows ? rows[IR_DYPOS * (long)ld + b] : 0.0;
+    const double eqv = have_rows ? rows[IR_EQV * (long)ld + b] : 0.0;
+    const double ubv = have_rows ? rows[IR_UBV * (long)ld + b] : 0.0;
+    const double fobj = bdy + cols[IC_BT * (long)ld + b];
+    const double dviol = fmax(dypos, cols[IC_RV * (long)ld + b]);
+    const double cdx = cols[IC_CDX * (long)ld + b];
+    const double pviol = fmax(fmax(eqv, ubv), cols[IC_DXV * (long)ld + b]);
+    int status = 0;
+    if (isfinite(fobj) && isfinite(dviol) && fobj > 0.0 && dviol <= eps * fobj) status = 3;
+    else if (isfinite(cdx) && isfinite(pviol) && cdx < 0.0 && pviol <= eps * (-cdx)) status = 4;
+    if (status) {
+        st[b].status = status;
+        flags[b] = 1;
+    }
+}
+
 
 struct DecideArgs {
     const double* rows;     // [2*CR_N][ld] reduced row values

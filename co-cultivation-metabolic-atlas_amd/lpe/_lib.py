@@ -61,7 +61,8 @@ class LpeSolveOpts(C.Structure):
                 ("eps_certify_primal", C.c_double), ("eps_certify_dual", C.c_double),
                 ("eps_certify_gap", C.c_double), ("stop_after_failed_polishes", C.c_int32),
                 ("polish_kkt_start", C.c_double), ("polish_stable_checks", C.c_int32),
-                ("first_check_interval", C.c_int32)]
+                ("first_check_interval", C.c_int32), ("infeasibility_interval", C.c_int32),
+                ("eps_infeasible", C.c_double)]
 
 
 class LpeSolveInfo(C.Structure):

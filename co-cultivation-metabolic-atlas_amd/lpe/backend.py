@@ -53,10 +53,21 @@ class GpuLpBackend:
         self.closed = False
 
     # ---------------------------------------------------------------- engines
+    MAX_ENGINES = 4
+
     def _engine_for(self, key, template, varying, batch, shared_c):
+        """The cached engine of this pattern and batch size; at most
+        MAX_ENGINES stay resident (least recently used ones are closed, so a
+        microbatched rollout whose pending batch shrinks does not keep one
+        engine per size)."""
         cache_key = (key, tuple(varying.tolist()), batch, shared_c)
-        eng = self._engines.get(cache_key)
-        if eng is None:
+        eng = self._engines.pop(cache_key, None)
+        if eng is not None:
+            self._engines[cache_key] = eng            # most recently used last
+        else:
+            while len(self._engines) >= self.MAX_ENGINES:
+                old = next(iter(self._engines))
+                self._engines.pop(old).close()
             a, neq = template
             eng = BatchedLpEngine(a, neq, batch, var_positions=varying, precision=self.precision,
                                   shared_c=shared_c, method=self.pdhg)
@@ -97,11 +108,15 @@ class GpuLpBackend:
                         cpu_solver_runs=0, numerical_retry_count=0)
             if ok and self.warm_start:
                 self._warm[(e, stage, key)] = (x[b].copy(), y[b].copy())
+            if ok:
+                message = "optimal"
+            elif int(status[b]) in (_lib.STATUS_PRIMAL_INFEASIBLE, _lib.STATUS_DUAL_INFEASIBLE):
+                message = "model status " + diag["status"] + " (ray certificate)"
+            else:
+                message = "original LP certificate failed: " + diag["status"]
             out.append(SimpleNamespace(success=ok, x=x[b].copy() if ok else None,
                                        fun=float(metrics[b, 3]) if ok else None, nit=int(info["iterations"]),
-                                       diagnostics=diag,
-                                       message="optimal" if ok else "original LP certificate failed: "
-                                       + diag["status"]))
+                                       diagnostics=diag, message=message))
         return out, info
 
     # ------------------------------------------------------------- interfaces

@@ -58,6 +58,8 @@ class SolveOptions:
     polish_kkt_start: float = 0.0
     polish_stable_checks: int = 0
     first_check_interval: int = 0
+    infeasibility_interval: int = 0
+    eps_infeasible: float = 0.0
 
     def to_c(self) -> _lib.LpeSolveOpts:
         o = _lib.LpeSolveOpts()
@@ -76,6 +78,8 @@ class SolveOptions:
         o.polish_kkt_start = float(self.polish_kkt_start)
         o.polish_stable_checks = int(self.polish_stable_checks)
         o.first_check_interval = int(self.first_check_interval)
+        o.infeasibility_interval = int(self.infeasibility_interval)
+        o.eps_infeasible = float(self.eps_infeasible)
         return o
 
 
@@ -210,7 +214,8 @@ class BatchedLpEngine:
             options = dataclasses.replace(options, stop_after_failed_polishes=2)
         info = self.solve(options)
         x, y, status = self.state()
-        bad = np.flatnonzero(status != _lib.STATUS_OPTIMAL)
+        bad = np.flatnonzero((status != _lib.STATUS_OPTIMAL) & (status != _lib.STATUS_PRIMAL_INFEASIBLE)
+                             & (status != _lib.STATUS_DUAL_INFEASIBLE))
         info["fallback_lps"] = int(bad.size)
         info["fallback_certified"] = 0
         if bad.size == 0 or self._vectors is None:

@@ -346,3 +346,148 @@ def consortium_batch(batch: int = 64, seed: int = 2, model: Consortium | None = 
     conc = np.exp(rng.uniform(np.log(lo), np.log(hi), size=(batch, model.medium_cols.shape[1])))
     biomass = rng.uniform(0.05, 0.5, size=(batch, model.medium_cols.shape[0]))
     return ConsortiumBatch(model=model, conc=conc, biomass=biomass)
+
+
+@dataclass
+class StatusBatch:
+    """LPs of one pattern with known status: ``kind[b]`` is "optimal",
+    "infeasible" or "unbounded"."""
+
+    A: csr_matrix
+    rhs: np.ndarray          # [B, m]
+    lower: np.ndarray        # [B, n]
+    upper: np.ndarray        # [B, n]
+    c: np.ndarray            # [n] shared
+    neq: int
+    kind: list
+
+    @property
+    def batch(self):
+        return self.lower.shape[0]
+
+    def problem(self, b: int):
+        return (self.A.copy(), self.rhs[b].copy(), self.lower[b].copy(), self.upper[b].copy(), self.c.copy(),
+                self.neq)
+
+
+def status_batch(n_each: int = 4, seed: int = 5, m: int = 200, n: int = 300) -> StatusBatch:
+    """Feasible, primal infeasible and unbounded LPs sharing one pattern.
+
+    A species LP (``species_model``) gets two extra columns p, q with +1 and
+    -1 in one row, costs -0.01 each, lower bound 0.  Feasible LPs: uptake
+    bounds of the exchanges drawn per LP, p and q bounded by 1000.  Primal
+    infeasible LPs: one row's right-hand side moved one unit past the largest
+    activity the box allows (a single-row Farkas ray).  Unbounded LPs: p and
+    q without upper bound (the ray e_p + e_q keeps A x and lowers c.x)."""
+    sp = species_model(m, n, seed=seed, exchange_v0_nonnegative=True)
+    rng = np.random.default_rng(seed + 31)
+    S = sp.S.tocoo()
+    r_pq = int(rng.integers(0, m))
+    rows = np.concatenate([S.row, [r_pq, r_pq]])
+    cols = np.concatenate([S.col, [n, n + 1]])
+    vals = np.concatenate([S.data, [1.0, -1.0]])
+    A = coo_matrix((vals, (rows, cols)), shape=(m, n + 2)).tocsr()
+    A.sum_duplicates(); A.sort_indices()
+    c = np.zeros(n + 2); c[sp.biomass_col] = -1.0; c[n] = c[n + 1] = -0.01
+    B = 3 * n_each
+    lower = np.tile(np.r_[sp.lower, 0.0, 0.0], (B, 1))
+    upper = np.tile(np.r_[sp.upper, BOUND, BOUND], (B, 1))
+    rhs = np.tile(sp.rhs, (B, 1))
+    k = sp.exchange_cols.size
+    lower[:, sp.exchange_cols] = -rng.uniform(0.5, 20.0, size=(B, k))
+    kind = []
+    bad_rows = rng.choice(m, size=n_each, replace=False)
+    for b in range(B):
+        if b % 3 == 0:
+            kind.append("optimal")
+        elif b % 3 == 1:
+            kind.append("infeasible")
+            i = int(bad_rows[(b // 3) % n_each])
+            seg = slice(A.indptr[i], A.indptr[i + 1])
+            a_i, j = A.data[seg], A.indices[seg]
+            top = np.sum(np.maximum(a_i * lower[b, j], a_i * upper[b, j]))
+            rhs[b, i] = top + 1.0
+        else:
+            kind.append("unbounded")
+            upper[b, n] = upper[b, n + 1] = np.inf
+    return StatusBatch(A=A, rhs=rhs, lower=lower, upper=upper, c=c, neq=m, kind=kind)
+
+
+@dataclass
+class KnownOptimumBatch:
+    """LPs of one pattern, each built around a chosen unique optimum: x_star
+    (basic columns strictly inside their bounds, the others at a bound with
+    a nonzero reduced cost of the right sign) and y_star (nonzero on the
+    equality rows and the active inequality rows, zero on inactive ones)."""
+
+    A: csr_matrix
+    rhs: np.ndarray          # [B, m]
+    lower: np.ndarray        # [B, n]
+    upper: np.ndarray        # [B, n]
+    c: np.ndarray            # [B, n]
+    neq: int
+    x_star: np.ndarray       # [B, n]
+    y_star: np.ndarray       # [B, m]
+    at_lower: np.ndarray     # [B, n] bool
+    at_upper: np.ndarray     # [B, n] bool
+    active_rows: np.ndarray  # [B, m - neq] bool
+
+    @property
+    def batch(self):
+        return self.lower.shape[0]
+
+    def problem(self, b: int):
+        return (self.A.copy(), self.rhs[b].copy(), self.lower[b].copy(), self.upper[b].copy(), self.c[b].copy(),
+                self.neq)
+
+
+def known_optimum_batch(batch: int = 8, seed: int = 7, m: int = 120, n: int = 200, n_ineq: int = 30,
+                        nnz_per_col: int = 6) -> KnownOptimumBatch:
+    """Unique, primal- and dual-nondegenerate optima by construction: the
+    active constraints' matrix restricted to the basic columns is square and
+    nonsingular (columns chosen by a pivoted QR), basic columns sit at least
+    20 % of their width inside the box, nonbasic reduced costs are at least
+    0.5 in magnitude and active inequality duals at most -0.5."""
+    from scipy.linalg import qr
+    rng = np.random.default_rng(seed)
+    rows, cols, vals = [], [], []
+    for j in range(n):
+        r = rng.choice(m, size=nnz_per_col, replace=False)
+        rows.append(r); cols.append(np.full(nnz_per_col, j))
+        vals.append(rng.choice([-3.0, -2.0, -1.0, 1.0, 2.0, 3.0], size=nnz_per_col))
+    A = coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(m, n)).tocsr()
+    A.sum_duplicates(); A.eliminate_zeros(); A.sort_indices()
+    Ad = A.toarray()
+    neq = m - n_ineq
+    out = {k: [] for k in ("rhs", "lower", "upper", "c", "x", "y", "lo", "up", "act")}
+    for _ in range(batch):
+        act = rng.random(n_ineq) < 0.5
+        rows_act = np.r_[np.arange(neq), neq + np.flatnonzero(act)]
+        _, R, piv = qr(Ad[rows_act], mode="economic", pivoting=True)
+        k = rows_act.size
+        assert abs(R[k - 1, k - 1]) > 1e-8 * abs(R[0, 0]), "active rows are rank deficient"
+        basic = np.zeros(n, dtype=bool)
+        basic[piv[:k]] = True
+        lower = -rng.uniform(0.0, 10.0, size=n)
+        upper = lower + rng.uniform(2.0, 20.0, size=n)
+        at_up = ~basic & (rng.random(n) < 0.3)
+        at_lo = ~basic & ~at_up
+        x = np.where(at_lo, lower, np.where(at_up, upper, lower + rng.uniform(0.2, 0.8, size=n) * (upper - lower)))
+        ax = Ad @ x
+        rhs = ax.copy()
+        inactive = neq + np.flatnonzero(~act)
+        rhs[inactive] += rng.uniform(1.0, 5.0, size=inactive.size)
+        y = np.zeros(m)
+        y[:neq] = rng.normal(size=neq)
+        y[neq + np.flatnonzero(act)] = -rng.uniform(0.5, 2.0, size=int(act.sum()))
+        r = np.zeros(n)
+        r[at_lo] = rng.uniform(0.5, 2.0, size=int(at_lo.sum()))
+        r[at_up] = -rng.uniform(0.5, 2.0, size=int(at_up.sum()))
+        c = Ad.T @ y + r
+        for key, val in (("rhs", rhs), ("lower", lower), ("upper", upper), ("c", c), ("x", x), ("y", y),
+                         ("lo", at_lo), ("up", at_up), ("act", act)):
+            out[key].append(val)
+    st = {k: np.array(v) for k, v in out.items()}
+    return KnownOptimumBatch(A=A, rhs=st["rhs"], lower=st["lower"], upper=st["upper"], c=st["c"], neq=neq,
+                             x_star=st["x"], y_star=st["y"], at_lower=st["lo"], at_upper=st["up"],
+                             active_rows=st["act"])

@@ -112,6 +112,15 @@ typedef struct lpe_solve_opts {
                                  * (a warm start may be polished sooner); the
                                  * graph chunk is then this length when it
                                  * divides check_interval (0 = check_interval) */
+    int32_t infeasibility_interval; /* restarted / Halpern methods: test the
+                                 * iterate differences since the last restart
+                                 * for a primal or dual infeasibility
+                                 * certificate every this many checks from the
+                                 * 8th check on (0 = 4, < 0 = never); a
+                                 * certified LP ends LPE_STATUS_PRIMAL_INFEASIBLE
+                                 * or LPE_STATUS_DUAL_INFEASIBLE               */
+    double eps_infeasible;    /* certificate tolerance: largest violation of
+                                 * the ray <= eps * |ray objective| (0 = 1e-8) */
 } lpe_solve_opts;
 
 /* Summary of the last lpe_solve. */

@@ -22,6 +22,9 @@ Modules (each function cites the reference file:line it restates):
 * ``stages``      - the cooperative stage chain on HiGHS (src/community_solver.py:80-198, :1207-1844).
 * ``reduction``   - exact homogeneous equality reduction and its lift
   (src/lp_equality_reduction.py, src/gpu_reduced_pdhg.py:81-103).
+* ``infeasibility`` - the dual-ray / primal-ray infeasibility certificate
+  test of restarted PDHG (PDLP; the reference reports HiGHS's infeasible /
+  unbounded statuses, src/cpu_repeated_lp.py:341-345, :694-699).
 
 Parity status of each function is recorded in its docstring ("pinned by").
 """

@@ -130,7 +130,9 @@ def test_backend_infeasible_request_fails_closed(device):
     try:
         r = backend.solve([0.0, 0.0], A_ub=[[1.0, 0.0], [-1.0, 0.0]], b_ub=[0.0, -1.0], bounds=[(None, None)] * 2)
         assert r.success is False and r.x is None and r.fun is None
-        assert "certificate failed" in r.message
+        # x1 <= 0 and x1 >= 1: certified primal infeasible by a dual ray before the budget
+        assert r.diagnostics["status"] == "primal_infeasible" and "primal_infeasible" in r.message
+        assert r.nit < 2000
     finally:
         backend.close()
 
@@ -163,3 +165,18 @@ def test_corrector_validation_before_device_work():
     q = (csr_matrix(np.eye(2)), np.ones(2), np.zeros(2), np.ones(2), np.ones(2), 2)
     with pytest.raises(ValueError):
         GpuPdhgCorrector([p, q])
+
+
+@pytest.mark.gpu
+def test_backend_keeps_a_bounded_number_of_engines(device):
+    """Batches of shrinking size (a microbatched rollout) reuse at most
+    MAX_ENGINES resident engines instead of one per size."""
+    from lpe.backend import GpuLpBackend
+    ub = workloads.uptake_batch(batch=6, seed=9, m=60, n=90)
+    be = GpuLpBackend(precision="fp64")
+    for size in (6, 5, 4, 3, 2, 1, 6):
+        res = be.solve_batch([problem_request(ub.problem(b)) for b in range(size)],
+                             environment_ids=list(range(size)))
+        assert all(r.success for r in res), size
+        assert len(be._engines) <= be.MAX_ENGINES
+    be.close()
