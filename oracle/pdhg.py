@@ -183,6 +183,7 @@ def restarted_pdhg(a, rhs, lower, upper, c, neq, *, iterations, check_interval=6
         omega0 = cnorm / bnorm if (bnorm > 0 and cnorm > 0) else 1.0
         omega0 = min(max(omega0, 1e-8), 1e8)
     omega, kkt_last, kkt_prev, cnt, iters, restarts = float(omega0), np.inf, np.inf, 0, 0, 0
+    xa = ya = None
     status, accepted_at = "running", -1
     history = []
 
@@ -296,7 +297,8 @@ def restarted_pdhg(a, rhs, lower, upper, c, neq, *, iterations, check_interval=6
             cnt += k
     xo = x.astype(np.float64) * dc
     yo = y.astype(np.float64) * dr
-    return dict(x=xo, y=yo, x_scaled=x, y_scaled=y, status=status if status != "running" else "iteration_limit",
+    return dict(x=xo, y=yo, x_scaled=x, y_scaled=y, x_candidate_scaled=xa, y_candidate_scaled=ya,
+                status=status if status != "running" else "iteration_limit",
                 iterations=iters, accepted_iteration=accepted_at, restarts=restarts, omega=omega,
                 history=history, row_scale=dr, col_scale=dc, sigma=sig, tau=tau,
                 certificate=certificate(a, rhs, lower, upper, c, neq, xo, yo))

@@ -61,8 +61,86 @@ def test_restarted_fp64_bit_exact_single_chunk(device, method):
     assert bool((gx == lx).all()) and bool((gy == ly).all())
     r = opdhg.restarted_pdhg(*ub.problem(0), iterations=30, check_interval=30, certify=False,
                              halpern=method == "halpern")
-    np.testing.assert_array_equal(r["x_scaled"], r["x_scaled"])  # oracle determinism
     np.testing.assert_allclose(gx.cpu().numpy()[0], r["x"], rtol=1e-13, atol=1e-10)
+
+
+def _no_accept_options(iterations):
+    """Checks, restarts and primal-weight updates on the device, but no LP
+    accepted (gates that nothing passes) and no polish or infeasibility test:
+    the device's decisions alone, as the oracle makes them."""
+    from lpe.engine import SolveOptions
+    return SolveOptions(max_iterations=iterations, check_interval=64, polish=False, eps_primal=1e-300,
+                        eps_dual=1e-300, eps_gap=1e-300, infeasibility_interval=-1)
+
+
+def test_fp64_iterates_match_oracle_across_restarts(device):
+    """1024 iterations with 16 device checks: every LP restarts 6 times and
+    updates its primal weight at each restart, as the oracle decides; the
+    iterates agree to 1e-11 (relative) and the weights to 1e-9."""
+    ub = _uptake(batch=4)
+    eng = _engine(ub.S, ub.neq, ub.batch, precision="fp64", shared_c=True)
+    _load_uptake(eng, ub)
+    eng.reset()
+    info = eng.solve(_no_accept_options(1024))
+    x, y, status = eng.state()
+    x, y, om = x.cpu().numpy(), y.cpu().numpy(), eng.primal_weight()
+    total = 0
+    for b in range(ub.batch):
+        r = opdhg.restarted_pdhg(*ub.problem(b), iterations=1024, check_interval=64, certify=False)
+        assert r["restarts"] >= 2
+        total += r["restarts"]
+        np.testing.assert_allclose(x[b], r["x"], rtol=1e-11, atol=1e-11)
+        np.testing.assert_allclose(y[b], r["y"], rtol=1e-11, atol=1e-11)
+        np.testing.assert_allclose(om[b], r["omega"], rtol=1e-9)
+    assert info["restarts"] == total
+    assert (status == 2).all()
+
+
+def test_fp32_iterates_bit_exact_before_restarts_and_within_gamma_bound(device):
+    """FP32 iterates: identical to the FP32 oracle (same operations in the same
+    order) before the first restart, and within 32 k u (1 + |x|) of the FP64
+    oracle after k iterations (u = 2^-24, a gamma_k-type forward error bound;
+    measured constant <= 16.4)."""
+    ub = _uptake(batch=3)
+    u = 2.0 ** -24
+    for k in (10, 50, 200):
+        eng = _engine(ub.S, ub.neq, ub.batch, precision="fp32", shared_c=True)
+        _load_uptake(eng, ub)
+        eng.reset()
+        eng.run(k)
+        x, y, _ = eng.state()
+        x, y = x.cpu().numpy(), y.cpu().numpy()
+        for b in range(ub.batch):
+            r32 = opdhg.restarted_pdhg(*ub.problem(b), iterations=k, check_interval=k, precision="fp32",
+                                       certify=False)
+            np.testing.assert_array_equal(x[b], r32["x"])
+            np.testing.assert_array_equal(y[b], r32["y"])
+            r64 = opdhg.restarted_pdhg(*ub.problem(b), iterations=k, check_interval=k, certify=False)
+            sx, sy = r64["x_scaled"], r64["y_scaled"]
+            ex = np.abs(x[b] / r64["col_scale"] - sx)
+            ey = np.abs(y[b] / r64["row_scale"] - sy)
+            assert np.all(ex <= 32 * k * u * (1 + np.abs(sx))), (k, b, float(np.max(ex / (k * u * (1 + np.abs(sx))))))
+            assert np.all(ey <= 32 * k * u * (1 + np.abs(sy))), (k, b)
+        eng.close()
+
+
+def test_fp32_restart_decisions_match_oracle(device):
+    """FP32 iterates over 1024 iterations: the device restarts each LP as often
+    as the FP32 oracle; the primal weights then differ only by the last bits
+    of their FP64 reductions fed back through FP32 (within 2 %)."""
+    ub = _uptake(batch=4)
+    eng = _engine(ub.S, ub.neq, ub.batch, precision="fp32", shared_c=True)
+    _load_uptake(eng, ub)
+    eng.reset()
+    info = eng.solve(_no_accept_options(1024))
+    om = eng.primal_weight()
+    total = 0
+    for b in range(ub.batch):
+        r = opdhg.restarted_pdhg(*ub.problem(b), iterations=1024, check_interval=64, precision="fp32",
+                                 certify=False)
+        total += r["restarts"]
+        np.testing.assert_allclose(om[b], r["omega"], rtol=2e-2)
+    assert info["restarts"] == total
 
 
 @pytest.mark.parametrize("precision,polish,method", [("fp64", False, "restarted"), ("fp64", True, "restarted"),

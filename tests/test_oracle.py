@@ -232,12 +232,23 @@ def test_halpern_steps_known_answers():
 
 
 def test_restarted_pdhg_average_is_running_mean_and_box_invariant():
+    """The average the first check evaluates (and may restart to) is the
+    mean of iterates 1..64, each obtained by a separate run stopped there; a
+    planted off-by-one (iterates 0..63) does not match."""
     ub = workloads.uptake_batch(batch=1, seed=4, m=40, n=60)
     p = ub.problem(0)
-    r = pdhg.restarted_pdhg(*p, iterations=64, check_interval=64, certify=False)
-    assert r["history"][0]["iteration"] == 64
+    K = 64
+    r = pdhg.restarted_pdhg(*p, iterations=K, check_interval=K, certify=False)
+    assert r["history"][0]["iteration"] == K
     ls = p[2] / r["col_scale"]; us = p[3] / r["col_scale"]
     assert np.all(r["x_scaled"] >= ls - 1e-12) and np.all(r["x_scaled"] <= us + 1e-12)
+    xs = [pdhg.restarted_pdhg(*p, iterations=j, check_interval=j, certify=False)["x_scaled"] for j in range(1, K + 1)]
+    ys = [pdhg.restarted_pdhg(*p, iterations=j, check_interval=j, certify=False)["y_scaled"] for j in range(1, K + 1)]
+    x0 = np.clip(np.zeros(p[0].shape[1]), ls, us)
+    np.testing.assert_allclose(r["x_candidate_scaled"], np.mean(xs, axis=0), rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(r["y_candidate_scaled"], np.mean(ys, axis=0), rtol=1e-13, atol=1e-15)
+    assert np.max(np.abs(r["x_candidate_scaled"] - np.mean([x0] + xs[:-1], axis=0))) > 1e-6
+    assert np.all(r["x_candidate_scaled"] >= ls - 1e-12) and np.all(r["x_candidate_scaled"] <= us + 1e-12)
 
 
 def test_ruiz_scaling_equilibrates():
