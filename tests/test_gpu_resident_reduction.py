@@ -1,165 +1,220 @@
-"""The resident reduced service (lpe.reduction.GpuResidentReducedPdhg) on the
-reference's cases (tests/test_gpu_resident_reduced_pdhg.py): graph = loop =
-non-resident corrector, value updates through try_update, rebuild requests
-and invalid input without mutation, owned results, final-certificate
-authority."""
+"""The resident reduced service, lpe.reduction.GpuResidentReducedPdhg
+(src/gpu_resident_reduced_pdhg.py), on the reference's service cases - kept
+as data in tests/golden/resident_reduction_cases.json - and checked against
+this repository's oracle: the service's witnesses and reduced problems are
+oracle.reduction's, and every answer it accepts is HiGHS's optimum of the
+original LP.  Contract points: the graph, loop and non-resident drivers
+agree; a same-pattern update reuses the resident buffers; pattern / batch /
+order changes ask for a rebuild (with the reference's reason strings) and
+invalid input raises, both without mutation; a failed device update
+invalidates the service; results are owned; the final original-LP
+certificate decides acceptance; a certified LP stays frozen."""
+
+import json
+import os
 
 import numpy as np
 import pytest
 import torch
 from scipy.sparse import csr_matrix
 
+from conftest import ROOT
+from oracle import highs, reduction as ored
+from oracle.certificate import relative_objective_error
+
 pytestmark = pytest.mark.gpu
 
-
-def _problem(*, inequality=(1.0, 0.5), rhs=0.8, lower=(0.0, -1.0), upper=(1.0, 2.0), c=(-1.0, -0.3)):
-    return (csr_matrix(np.array([[1.0, -1.0], inequality], dtype=np.float64)), np.array([0.0, rhs]),
-            np.array(lower), np.array(upper), np.array(c), 1)
+with open(os.path.join(ROOT, "tests", "golden", "resident_reduction_cases.json")) as _f:
+    CASES = json.load(_f)
 
 
-def _service(problems=None, **kwargs):
+def lp(d):
+    """LP tuple from a fixture record ("nan" strings become NaN)."""
+    f = lambda v: np.array([np.nan if x == "nan" else x for x in v], dtype=np.float64)  # noqa: E731
+    rec = dict(CASES["base"], **d)
+    return (csr_matrix(np.array(rec["a"], dtype=np.float64)), f(rec["rhs"]), f(rec["lower"]), f(rec["upper"]),
+            f(rec["c"]), int(rec["neq"]))
+
+
+def lps(records):
+    return [lp(r) for r in records]
+
+
+def make_service(problems, **kwargs):
     from lpe.reduction import GpuResidentReducedPdhg
-    problems = problems if problems is not None else [_problem(), _problem(rhs=0.9)]
-    return GpuResidentReducedPdhg(problems, chunk_size=4, **kwargs)
+    kwargs.setdefault("chunk_size", CASES["equivalence"]["chunk_size"])
+    return GpuResidentReducedPdhg(problems, **kwargs)
 
 
-def _np(t):
-    return t.cpu().numpy() if isinstance(t, torch.Tensor) else np.asarray(t)
+def host(t):
+    return t.detach().cpu().numpy().copy() if isinstance(t, torch.Tensor) else np.array(t, copy=True)
 
 
-def _same_result(left, right):
-    for name in ("x", "y", "reduced_x", "reduced_y"):
-        np.testing.assert_allclose(_np(left[name]), _np(right[name]), atol=1e-10, rtol=1e-10)
-    assert left["accepted"].tolist() == right["accepted"].tolist()
-    assert left["iterations_run"] == right["iterations_run"]
-    for lrow, rrow in zip(left["metrics"], right["metrics"]):
-        for name in ("primal_residual", "dual_violation", "relative_kkt_gap", "objective"):
-            np.testing.assert_allclose(lrow[name], rrow[name], atol=1e-10, rtol=1e-10)
-        assert lrow["certificate_passed"] == rrow["certificate_passed"]
+def summary(result):
+    """Arrays that identify a service result."""
+    metr = np.array([[r[k] for k in ("primal_residual", "dual_violation", "relative_kkt_gap", "objective")]
+                     for r in result["metrics"]])
+    return dict(x=host(result["x"]), y=host(result["y"]), rx=host(result["reduced_x"]),
+                ry=host(result["reduced_y"]), accepted=np.array(result["accepted"], dtype=bool),
+                passed=np.array([r["certificate_passed"] for r in result["metrics"]]), metrics=metr,
+                iterations=int(result["iterations_run"]))
 
 
-@pytest.mark.parametrize("iterations", [0, 3, 4, 17])
-def test_graph_loop_and_corrector_equivalence(device, iterations):
+def assert_same(a, b, tol=1e-10):
+    assert a["iterations"] == b["iterations"]
+    np.testing.assert_array_equal(a["accepted"], b["accepted"])
+    np.testing.assert_array_equal(a["passed"], b["passed"])
+    for k in ("x", "y", "rx", "ry", "metrics"):
+        np.testing.assert_allclose(a[k], b[k], rtol=tol, atol=tol, err_msg=k)
+
+
+def assert_accepted_are_optimal(problems, s):
+    for b, p in enumerate(problems):
+        if s["accepted"][b]:
+            ref = highs.solve(*p)
+            assert ref.status == "optimal"
+            assert relative_objective_error(float(p[4] @ s["x"][b]), ref.fun) <= 1e-6, b
+
+
+def oracle_witnesses(problems):
+    pl = ored.plan(problems[0])
+    red = [ored.reduce(pl, p) for p in problems]
+    return np.array([r[1] for r in red]).ravel().tolist(), np.array([r[2] for r in red]).ravel().tolist()
+
+
+@pytest.mark.parametrize("iterations", CASES["equivalence"]["iterations"])
+def test_graph_loop_and_one_shot_drivers_agree(device, iterations):
     from lpe.reduction import GpuReducedPdhgCorrector
-    problems = [_problem(), _problem(rhs=0.9)]
-    graph, loop = _service(problems), _service(problems, use_graph=False)
-    original = GpuReducedPdhgCorrector(problems)
-    args = dict(iterations=iterations, check_interval=17)
-    result = graph.solve(**args)
-    _same_result(result, loop.solve(**args))
-    _same_result(result, original.solve(**args))
-    assert result["cpu_lp_calls"] == 0
-    assert "host input/update/control" in result["scope"]
+    eq = CASES["equivalence"]
+    problems = lps(eq["problems"])
+    args = dict(iterations=iterations, check_interval=eq["check_interval"])
+    runs = [make_service(problems).solve(**args), make_service(problems, use_graph=False).solve(**args),
+            GpuReducedPdhgCorrector(problems).solve(**args)]
+    sums = [summary(r) for r in runs]
+    assert_same(sums[0], sums[1])
+    assert_same(sums[0], sums[2])
+    assert_accepted_are_optimal(problems, sums[0])
+    assert runs[0]["cpu_lp_calls"] == 0 and "host input/update/control" in runs[0]["scope"]
 
 
-def test_value_updates_refresh_witnesses(device):
-    service = _service([_problem(rhs=3.0), _problem(rhs=3.0)])
-    assert service.upper_witness.ravel().tolist() == [0, 0]
-    assert service.lower_witness.ravel().tolist() == [0, 0]
-    changed = [_problem(inequality=(0.25, 1.5), rhs=4.0, lower=(-1.0, 0.0), upper=(2.0, 0.7), c=(-0.4, -0.8)),
-               _problem(inequality=(2.0, 0.25), rhs=5.0, lower=(-1.0, 0.0), upper=(2.0, 0.9), c=(-0.6, -0.4))]
-    assert service.try_update(changed)
-    assert service.upper_witness.ravel().tolist() == [1, 1]
-    assert service.lower_witness.ravel().tolist() == [1, 1]
-    assert service.last_update_timing["reused"]
-    result = service.solve(iterations=12, check_interval=12)
-    rebuilt = _service(changed).solve(iterations=12, check_interval=12)
-    _same_result(result, rebuilt)
-    assert result["all_accepted"]
-    np.testing.assert_allclose(_np(result["x"]), [[0.7, 0.7], [0.9, 0.9]], atol=1e-12)
+def test_value_update_reuses_buffers_and_refreshes_witnesses(device):
+    vu = CASES["value_update"]
+    before, after = lps(vu["before"]), lps(vu["after"])
+    svc = make_service(before)
+    lw, uw = oracle_witnesses(before)
+    assert svc.lower_witness.ravel().tolist() == vu["witnesses_before"]["lower"] == lw
+    assert svc.upper_witness.ravel().tolist() == vu["witnesses_before"]["upper"] == uw
+    assert svc.try_update(after)
+    assert svc.last_update_timing["reused"] and svc.last_update_timing["reason"] == "same_pattern_value_update"
+    lw, uw = oracle_witnesses(after)
+    assert svc.lower_witness.ravel().tolist() == vu["witnesses_after"]["lower"] == lw
+    assert svc.upper_witness.ravel().tolist() == vu["witnesses_after"]["upper"] == uw
+    args = dict(iterations=vu["iterations"], check_interval=vu["iterations"])
+    updated = summary(svc.solve(**args))
+    assert updated["accepted"].all()
+    assert_same(updated, summary(make_service(after).solve(**args)))
+    np.testing.assert_allclose(updated["x"], vu["x_after"], atol=1e-12)
+    for b, p in enumerate(after):
+        np.testing.assert_allclose(updated["x"][b], highs.solve(*p).x, atol=1e-9)
 
 
-def test_returned_arrays_are_owned(device):
-    service = _service([_problem(rhs=3.0)])
-    first = service.solve(iterations=8, check_interval=8)
-    snap = {k: _np(first[k]).copy() for k in ("x", "y", "reduced_x", "reduced_y")}
-    assert service.try_update([_problem(rhs=3.0, upper=(0.3, 2.0))])
-    second = service.solve(iterations=8, check_interval=8)
-    for k, v in snap.items():
-        np.testing.assert_array_equal(_np(first[k]), v)
-        assert first[k].data_ptr() != second[k].data_ptr() or first[k].numel() == 0
+def test_results_are_owned_copies(device):
+    own = CASES["ownership"]
+    svc = make_service(lps(own["first"]))
+    args = dict(iterations=own["iterations"], check_interval=own["iterations"])
+    first = svc.solve(**args)
+    kept = {k: host(first[k]) for k in ("x", "y", "reduced_x", "reduced_y")}
+    for k in kept:                                  # the caller scribbles on its result ...
+        first[k].add_(1.0)
+    again = summary(svc.solve(**args))              # ... which the service never reads back
+    np.testing.assert_array_equal(again["x"], kept["x"])
+    np.testing.assert_array_equal(again["ry"], kept["reduced_y"])
+    assert svc.try_update(lps(own["second"]))
+    svc.solve(**args)                               # a later solve leaves the first result alone
+    for k, v in kept.items():
+        np.testing.assert_array_equal(host(first[k]), v + 1.0)
 
 
-@pytest.mark.parametrize("change,reason", [("order", "batch_or_environment_order_changed"),
-                                           ("batch", "batch_or_environment_order_changed"),
-                                           ("original_pattern", "original_csr_pattern_changed"),
-                                           ("reduced_pattern", "reduced_csr_pattern_changed")])
-def test_rebuild_requests_do_not_mutate(device, change, reason):
-    service = _service(environment_ids=[11, 29])
-    old = service.original_problems
-    problems, ids = [_problem(), _problem()], [11, 29]
-    if change == "order":
-        ids.reverse()
-    elif change == "batch":
-        problems, ids = problems[:1], ids[:1]
-    elif change == "original_pattern":
-        problems[0] = _problem(inequality=(1.0, 0.0))
-    else:
-        problems[0] = _problem(inequality=(1.0, -1.0))
-    assert not service.try_update(problems, environment_ids=ids)
-    assert service.last_update_timing["reason"] == reason
-    assert service.original_problems is old
-    service.solve(iterations=0)
+@pytest.mark.parametrize("case", CASES["rebuild"]["cases"], ids=lambda c: c["name"])
+def test_rebuild_requests_leave_the_service_unchanged(device, case):
+    ids = CASES["rebuild"]["environment_ids"]
+    svc = make_service(lps([{}, {}]), environment_ids=ids)
+    problems_before = svc.original_problems
+    assert not svc.try_update(lps(case["problems"]), environment_ids=case["ids"])
+    assert svc.last_update_timing["reason"] == case["reason"]
+    assert svc.original_problems is problems_before
+    assert svc.environment_ids == tuple(ids)
+    svc.solve(iterations=0)
 
 
-@pytest.mark.parametrize("bad", ["equality", "equality_rhs", "nan", "bounds"])
-def test_invalid_problem_rejected_before_mutation(device, bad):
-    service = _service()
-    old = service.original_problems
-    problem = list(_problem())
-    if bad == "equality":
-        problem[0] = csr_matrix([[1.0, -2.0], [1.0, 0.5]])
-    elif bad == "equality_rhs":
-        problem[1] = np.array([1.0, 0.8])
-    elif bad == "nan":
-        problem[4] = np.array([np.nan, -0.3])
-    else:
-        problem[2], problem[3] = np.array([1.0, 0.0]), np.array([2.0, 0.5])
+@pytest.mark.parametrize("case", CASES["invalid"], ids=lambda c: c["name"])
+def test_invalid_input_raises_before_any_change(device, case):
+    svc = make_service(lps([{}, {}]))
+    problems_before = svc.original_problems
+    bad = {k: v for k, v in case.items() if k != "name"}
     with pytest.raises(ValueError):
-        service.try_update([tuple(problem), _problem()])
-    assert service.original_problems is old and service._valid
+        svc.try_update(lps([bad, {}]))
+    assert svc.original_problems is problems_before and svc._valid
+    svc.solve(iterations=0)
 
 
-def test_interrupted_update_invalidates(device, monkeypatch):
-    service = _service()
+class _EngineWithFailingUpdate:
+    """The service's engine with a device update that fails part-way."""
 
-    def fail(*args, **kwargs):
-        raise RuntimeError("simulated device update failure")
+    def __init__(self, engine):
+        self._engine = engine
 
-    monkeypatch.setattr(service.engine, "update", fail)
-    with pytest.raises(RuntimeError, match="simulated"):
-        service.try_update([_problem(rhs=2.0), _problem(rhs=2.0)])
+    def update(self, *args, **kwargs):
+        raise OSError("device update failed")
+
+    def __getattr__(self, name):
+        return getattr(self._engine, name)
+
+
+def test_failed_device_update_invalidates_the_service(device):
+    svc = make_service(lps([{}, {}]))
+    svc.engine = _EngineWithFailingUpdate(svc.engine)
+    with pytest.raises(OSError):
+        svc.try_update(lps([{"rhs": [0.0, 2.0]}, {"rhs": [0.0, 2.0]}]))
     with pytest.raises(RuntimeError, match="invalidated"):
-        service.solve(iterations=1)
+        svc.solve(iterations=1)
 
 
-def test_final_certificate_is_the_authority(device, monkeypatch):
-    service = _service([_problem(rhs=3.0, c=(1.0, 0.3)), _problem()])
-    original = service._certificate
-    calls = 0
+class _ScheduledCertificate:
+    """The service's certificate with failures forced on a schedule:
+    LP 1 always, LP 0 from the second call on."""
 
-    def certificate(x, y):
-        nonlocal calls
-        rows = original(x, y)
-        if calls:
+    def __init__(self, certificate):
+        self._certificate = certificate
+        self.calls = 0
+
+    def __call__(self, x, y):
+        rows = self._certificate(x, y)
+        rows[1]["certificate_passed"] = False
+        if self.calls > 0:
             rows[0]["certificate_passed"] = False
             rows[0]["relative_kkt_gap"] = 1.0
-        rows[1]["certificate_passed"] = False
-        calls += 1
+        self.calls += 1
         return rows
 
-    monkeypatch.setattr(service, "_certificate", certificate)
-    result = service.solve(iterations=4, check_interval=4)
-    assert result["checkpoints"][0]["accepted_count"] == 1
-    assert not result["all_accepted"] and result["accepted"].tolist() == [False, False]
+
+def test_final_certificate_decides_acceptance(device):
+    auth = CASES["authority"]
+    svc = make_service(lps(auth["problems"]))
+    svc._certificate = _ScheduledCertificate(svc._certificate)
+    result = svc.solve(iterations=auth["iterations"], check_interval=auth["iterations"])
+    assert result["checkpoints"][0]["accepted_count"] == auth["accepted_at_first_checkpoint"]
+    assert list(result["accepted"]) == auth["accepted_final"] and not result["all_accepted"]
     assert result["metrics"][0]["accepted_iteration"] == -1 and not result["metrics"][0]["success"]
-    assert not service.history[-1]["accepted"][0]
+    assert not svc.history[-1]["accepted"][0]
 
 
-def test_certified_environment_stays_frozen(device):
-    service = _service([_problem(rhs=3.0, c=(1.0, 0.3)), _problem(rhs=3.0)])
-    result = service.solve(iterations=8, check_interval=8)
-    assert result["all_accepted"]
-    assert result["metrics"][0]["accepted_iteration"] == 0
-    np.testing.assert_array_equal(_np(result["x"])[0], [0.0, 0.0])
-    assert result["cpu_lp_calls"] == 0
+def test_certified_lp_stays_frozen(device):
+    fr = CASES["frozen"]
+    problems = lps(fr["problems"])
+    result = make_service(problems).solve(iterations=fr["iterations"], check_interval=fr["iterations"])
+    s = summary(result)
+    assert result["all_accepted"] and result["cpu_lp_calls"] == 0
+    assert result["metrics"][0]["accepted_iteration"] == fr["lp0_accepted_iteration"]
+    np.testing.assert_array_equal(s["x"][0], fr["lp0_x"])
+    assert_accepted_are_optimal(problems, s)
