@@ -382,6 +382,61 @@ def cooperative_stages(model, B, dev, snapshot, args):
             "stage_lps_per_s": round(sum(v["certified"] for v in info.values()) / dt, 2), "stages": info}
 
 
+def consortium_batch_leg(args, dev, B=64):
+    """Config-2 regime: one batch of 64 consortium LPs (per-environment
+    concentrations and biomass), cold start, autotuned launch shape, hipGraph
+    chunks; certified LPs/s, the time per PDHG iteration against its HBM
+    bytes, and the share of an iteration the kernels are busy (the rest is
+    launch and graph overhead)."""
+    import numpy as np
+    import torch
+    from lpe import workloads
+    from lpe.autotune import autotune
+    from lpe.engine import BatchedLpEngine, SolveOptions
+    cb = workloads.consortium_batch(batch=B, seed=args.seed + 3, conc_range=(1e-2, 10.0))
+    md = cb.model
+    m, n = md.shape
+    eng = BatchedLpEngine(md.A, md.neq, B, var_positions=md.var_positions, precision="fp32", shared_c=True,
+                          device=dev)
+    eng.update(md.c, cb.rhs(), cb.lower(), cb.upper(), var_values=cb.coupling_values())
+    tuning = autotune(eng, cache_path=None)
+    opts = SolveOptions(max_iterations=args.fp32_iterations, polish=True, check_interval=int(tuning["chunk"]))
+    eng.reset()
+    eng.solve(opts)                                # code objects and graphs
+    eng.reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    info, x, _, status = eng.solve_with_fallback(opts)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ok = int((status == 1).sum())
+    eng.reset()
+    ms_iter = eng.time_iterations(256, chunk=int(tuning["chunk"]))
+    ms_kern = eng.time_kernel("primal", reps=50) + eng.time_kernel("dual", reps=50)
+    lpl = int(tuning["lps_per_lane"])
+    it_bytes = _primal_bytes(B, m, n, md.A.nnz, md.var_positions.size, 4) + \
+        _dual_bytes(B, m, n, md.A.nnz, md.var_positions.size, 4)
+    eng.close()
+    from oracle import highs, certificate as cert
+    xs = x.cpu().numpy()
+    errs = []
+    for b in (0, B // 2, B - 1):
+        ref = highs.solve(*cb.problem(b))
+        if status[b] == 1 and ref.status == "optimal":
+            errs.append(cert.relative_objective_error(float(md.c @ xs[b]), ref.fun))
+    print(f"# consortium batch {B}: {ok}/{B} certified in {dt:.3f} s, {ms_iter / 256 * 1e3:.1f} us per iteration",
+          file=sys.stderr, flush=True)
+    return {"environments": B, "lp": f"{m}x{n}", "certified": ok, "seconds": round(dt, 4),
+            "certified_lps_per_s": round(ok / dt, 2), "iterations": info["iterations"],
+            "polish_attempts": info["polish_attempts"], "autotune": {k: v for k, v in tuning.items() if k != "timings"},
+            "autotune_timings_ms_per_iteration": tuning.get("timings"),
+            "us_per_iteration": round(ms_iter / 256 * 1e3, 2),
+            "us_per_iteration_kernels_back_to_back": round(ms_kern * 1e3, 2),
+            "iteration_algorithmic_bytes": it_bytes,
+            "iteration_hbm_frac": round(it_bytes / (ms_iter / 256 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "highs_check": {"checked": len(errs), "max_rel_obj_err": max(errs) if errs else None}}
+
+
 def gem_reduced_leg(args, dev):
     """The reference's three member GEMs (parsed SBML) as the joint max-min LP
     of a batch of environments: exact equality reduction, restarted Halpern
@@ -472,6 +527,7 @@ def main():
     ap.add_argument("--plain-steps", default="first,last",
                     help="timed steps re-solved by the plain form (FP64 cold PDHG): 'first,last', indices, or ''")
     ap.add_argument("--no-gem", action="store_true", help="skip the real-GEM (reduced, Halpern FP64) leg")
+    ap.add_argument("--no-batch64", action="store_true", help="skip the 64-LP consortium batch (config 2) leg")
     ap.add_argument("--gem-envs", type=int, default=1024)
     ap.add_argument("--gem-iterations", type=int, default=100000)
     ap.add_argument("--no-stages", action="store_true",
@@ -711,6 +767,8 @@ def main():
     else:
         result["cpu_baseline"] = None
     result["cooperative_stages"] = stages_report
+    result["consortium_batch_64"] = None if args.no_batch64 or args.workload == "gem" else \
+        consortium_batch_leg(args, dev)
     result["real_gem"] = None if args.no_gem or args.workload == "gem" else gem_reduced_leg(args, dev)
     print(json.dumps(result))
 

@@ -71,6 +71,9 @@ Shape make_shape_waves(int rows, int ld, int batch, int rpw, int lpl, int waves,
     return s;
 }
 
+// Only the LP tiles that hold real LPs are launched: a batch below the padded
+// width (64 LPs in a 256-wide layout) runs one tile, not four (the padding
+// LPs are frozen and their partial sums are never read).
 Shape make_shape(int rows, int ld, int batch, int rpw, int lpl, int* grid) {
     Shape s;
     s.rows = rows;
@@ -78,7 +81,7 @@ Shape make_shape(int rows, int ld, int batch, int rpw, int lpl, int* grid) {
     s.batch = batch;
     s.rpw = rpw;
     const int W = WAVE * lpl;
-    s.n_tiles = ceil_div(ld, W);
+    s.n_tiles = ceil_div(std::min(std::max(batch, 1), ld), W);
     s.n_rb = std::max(1, ceil_div(rows, WAVES_PER_BLOCK * rpw));
     s.n_real = s.n_tiles * s.n_rb;
     *grid = ((s.n_real + 7) / 8) * 8;
