@@ -10,9 +10,11 @@
   holding the certificate (``certificate_passed``, ``primal_residual``,
   ``dual_violation``, ``relative_kkt_gap``, ``stage``, ...).
 
-Requests with the same sparsity pattern, equality count and stage are solved
-as one shared-pattern batch; per-LP coefficient differences become per-LP
-matrix entries.  Every answer is the engine's FP64 certificate; an LP that
+Requests of one stage are solved as one shared-pattern batch: LPs of other
+sizes are padded to the group's largest row / column counts (empty rows,
+columns fixed at zero) and the union of their patterns is the batch's
+pattern, with per-LP coefficients where values differ (the reference's
+corrector takes same-shape LPs only, src/gpu_pdhg_corrector.py:120-123).  Every answer is the engine's FP64 certificate; an LP that
 does not certify within the budget is reported ``success=False`` (no CPU
 fallback).  With ``warm_start`` each environment's previous primal/dual of
 the same stage seeds its next solve (src/gpu_resident_reduced_pdhg.py:104-118).
@@ -26,6 +28,8 @@ from types import SimpleNamespace
 import numpy as np
 
 from . import _lib
+from scipy.sparse import csr_matrix
+
 from .corrector import shared_pattern
 from .engine import BatchedLpEngine, SolveOptions
 from .requests import normalize, pattern_key
@@ -36,7 +40,7 @@ class GpuLpBackend:
 
     def __init__(self, *, precision: str = "fp32", polish: bool = True, max_iterations: int = 20000,
                  check_interval: int = 64, warm_start: bool = True, launch: tuple | None = None,
-                 pdhg: str = "restarted"):
+                 pdhg: str = "restarted", mixed_shapes: bool = True):
         if pdhg not in ("restarted", "halpern"):
             raise ValueError("pdhg must be 'restarted' or 'halpern'")
         self.precision = precision
@@ -47,6 +51,7 @@ class GpuLpBackend:
         if launch:
             self.options.rows_per_wave, self.options.lps_per_lane = launch
         self.warm_start = bool(warm_start)
+        self.mixed_shapes = bool(mixed_shapes)
         self._engines: dict = {}
         self._warm: dict = {}
         self.history: list[dict] = []
@@ -74,7 +79,32 @@ class GpuLpBackend:
             self._engines[cache_key] = eng
         return eng
 
+    @staticmethod
+    def _pad_to_common(problems):
+        """LPs of different sizes as one shape: equality rows, inequality
+        rows and columns each padded to the group's largest count (empty
+        rows with zero right-hand side, columns fixed at 0 with zero cost),
+        which leaves every LP's optimum, duals and certificate unchanged.
+        Returns the padded problems and each LP's (neq, n_ineq, n)."""
+        dims = [(p[5], p[0].shape[0] - p[5], p[0].shape[1]) for p in problems]
+        E, U, N = (max(d[k] for d in dims) for k in range(3))
+        out = []
+        for (a, rhs, lo, hi, c, neq), (e, u, n) in zip(problems, dims):
+            a = csr_matrix(a)
+            rows = np.r_[np.arange(e), E + np.arange(u)]
+            coo = a.tocoo()
+            pa = csr_matrix((coo.data, (rows[coo.row], coo.col)), shape=(E + U, N))
+            prhs = np.zeros(E + U)
+            prhs[rows] = rhs
+            plo, phi, pc = np.zeros(N), np.zeros(N), np.zeros(N)
+            plo[:n], phi[:n], pc[:n] = lo, hi, c
+            out.append((pa, prhs, plo, phi, pc, E))
+        return out, dims
+
     def _solve_group(self, problems, env_ids, stage):
+        dims = None
+        if len({(p[0].shape, p[5]) for p in problems}) > 1:
+            problems, dims = self._pad_to_common(problems)
         a0, _, _, _, c0, neq = problems[0]
         template, varying, values = shared_pattern([p[0] for p in problems])
         B = len(problems)
@@ -98,6 +128,12 @@ class GpuLpBackend:
             x, y, status = eng.state()
         metrics = eng.certify(x, y).cpu().numpy()
         x, y = x.cpu().numpy(), y.cpu().numpy()
+        xw, yw = x, y                     # the engine's layout, kept for warm starts
+        if dims is not None:
+            # back to each LP's own rows and columns
+            E = neq
+            x = [x[b, :d[2]] for b, d in enumerate(dims)]
+            y = [np.r_[y[b, :d[0]], y[b, E:E + d[1]]] for b, d in enumerate(dims)]
         out = []
         for b, (p, e) in enumerate(zip(problems, env_ids)):
             ok = bool(status[b] == _lib.STATUS_OPTIMAL and metrics[b, 4] == 1.0)
@@ -107,7 +143,7 @@ class GpuLpBackend:
                         stage=stage, iterations=int(info["iterations"]), warm_started=x0 is not None,
                         cpu_solver_runs=0, numerical_retry_count=0)
             if ok and self.warm_start:
-                self._warm[(e, stage, key)] = (x[b].copy(), y[b].copy())
+                self._warm[(e, stage, key)] = (xw[b].copy(), yw[b].copy())
             if ok:
                 message = "optimal"
             elif int(status[b]) in (_lib.STATUS_PRIMAL_INFEASIBLE, _lib.STATUS_DUAL_INFEASIBLE):
@@ -134,10 +170,12 @@ class GpuLpBackend:
         stages = [kw.get("_stage") for _, kw in requests]
         groups: dict = {}
         for i, (p, st) in enumerate(zip(problems, stages)):
-            groups.setdefault((p[0].shape, p[5], st), []).append(i)
+            # LPs of one stage share a launch; sizes are padded to a common shape
+            groups.setdefault(st if self.mixed_shapes else (p[0].shape, p[5], st), []).append(i)
         results = [None] * len(requests)
         infos = []
-        for (_, _, stage), idx in groups.items():
+        for key, idx in groups.items():
+            stage = key if self.mixed_shapes else key[2]
             res, info = self._solve_group([problems[i] for i in idx], [environment_ids[i] for i in idx], stage)
             infos.append(info)
             for i, r in zip(idx, res):

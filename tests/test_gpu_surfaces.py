@@ -110,7 +110,13 @@ def test_backend_mixed_shapes_and_stages_grouped(device, pdhg):
     sp = workloads.species_lp(seed=6, m=80, n=120)
     c_b, kw_b = problem_request(sp)
     kw_b2 = dict(kw_b, _stage="aggregate")
-    backend = GpuLpBackend(precision="fp64", polish=True, pdhg=pdhg)
+    for mixed in (True, False):
+        _grouping_run(pdhg, mixed, reqs_a, c_b, kw_b, kw_b2, sp)
+
+
+def _grouping_run(pdhg, mixed, reqs_a, c_b, kw_b, kw_b2, sp):
+    from lpe.backend import GpuLpBackend
+    backend = GpuLpBackend(precision="fp64", polish=True, pdhg=pdhg, mixed_shapes=mixed)
     try:
         res = backend.solve_batch(reqs_a + [(c_b, kw_b), (c_b, kw_b2)], environment_ids=["a", "b", "c", "d"])
         assert [r.success for r in res] == [True] * 4
@@ -118,7 +124,9 @@ def test_backend_mixed_shapes_and_stages_grouped(device, pdhg):
         ref = highs.solve(*sp)
         for r in res[2:]:
             assert relative_objective_error(r.fun, ref.fun) <= 1e-6
-        assert backend.history[-1]["groups"] == 3
+        # one launch per stage (the two shapes of the default stage share one),
+        # or per shape and stage without padding
+        assert backend.history[-1]["groups"] == (2 if mixed else 3)
     finally:
         backend.close()
 
@@ -180,3 +188,32 @@ def test_backend_keeps_a_bounded_number_of_engines(device):
         assert all(r.success for r in res), size
         assert len(be._engines) <= be.MAX_ENGINES
     be.close()
+
+
+@pytest.mark.gpu
+def test_backend_solves_lps_of_different_sizes_in_one_batch(device):
+    """Requests of three sizes (and inequality rows in some) share one engine
+    launch: each is padded to the common shape and answered in its own rows
+    and columns, certified and within 1e-6 of HiGHS; a warm-started repeat
+    of the batch reuses the padded layout."""
+    from lpe.backend import GpuLpBackend
+    probs = []
+    for seed, (m, n) in enumerate(((60, 90), (40, 70), (80, 100))):
+        ub = workloads.uptake_batch(batch=2, seed=seed + 20, m=m, n=n)
+        for b in range(2):
+            a, rhs, lo, hi, c, neq = ub.problem(b)
+            if seed == 1:          # last 5 equality rows as <= rows
+                neq = m - 5
+            probs.append((a, rhs, lo, hi, c, neq))
+    be = GpuLpBackend(precision="fp32")
+    try:
+        for _ in range(2):
+            res = be.solve_batch([problem_request(p) for p in probs], environment_ids=list(range(len(probs))))
+            assert be.history[-1]["groups"] == 1
+            for p, r in zip(probs, res):
+                ref = highs.solve(*p)
+                assert r.success, r.message
+                assert r.x.shape == (p[0].shape[1],)
+                assert relative_objective_error(r.fun, ref.fun) <= 1e-6
+    finally:
+        be.close()
