@@ -489,10 +489,86 @@ def gem_reduced_leg(args, dev):
     return report
 
 
+class _Range:
+    """roctx range around a region of the run (rocprofv3 --marker-trace
+    records it; a no-op without the roctx library)."""
+
+    _lib = None
+
+    def __init__(self, name):
+        self.name = name.encode()
+        if _Range._lib is None:
+            import ctypes
+            _Range._lib = False
+            for lib in ("librocprofiler-sdk-roctx.so.1", "/opt/rocm/lib/librocprofiler-sdk-roctx.so.1"):
+                try:
+                    _Range._lib = ctypes.CDLL(lib)
+                    _Range._lib.roctxRangePushA.argtypes = [ctypes.c_char_p]
+                    break
+                except OSError:
+                    continue
+
+    def __enter__(self):
+        if _Range._lib:
+            _Range._lib.roctxRangePushA(self.name)
+        return self
+
+    def __exit__(self, *exc):
+        if _Range._lib:
+            _Range._lib.roctxRangePop()
+        return False
+
+
+def trace_summary(directory, region="timed-steps", top=30):
+    """Per-kernel time inside one marked region of a rocprofv3 CSV run
+    (--kernel-trace --marker-trace --output-format csv): dispatches whose
+    start lies inside the region, summed per kernel name."""
+    import csv
+    import glob
+
+    def rows(pattern):
+        files = sorted(glob.glob(os.path.join(directory, "**", pattern), recursive=True))
+        if not files:
+            raise SystemExit(f"no {pattern} under {directory}")
+        with open(files[0]) as f:
+            return list(csv.DictReader(f))
+
+    def col(row, *names):
+        for n in names:
+            if n in row:
+                return row[n]
+        raise KeyError(names)
+
+    marks = [r for r in rows("*marker_api_trace.csv")
+             if region in col(r, "Function", "Name", "Message", "Marker_Name")]
+    if not marks:
+        raise SystemExit(f"region {region!r} not found")
+    t0 = int(col(marks[0], "Start_Timestamp"))
+    t1 = int(col(marks[0], "End_Timestamp"))
+    per = {}
+    for r in rows("*kernel_trace.csv"):
+        s0, s1 = int(col(r, "Start_Timestamp")), int(col(r, "End_Timestamp"))
+        if t0 <= s0 <= t1:
+            k = col(r, "Kernel_Name")
+            c, tot, mn, mx = per.get(k, (0, 0, None, 0))
+            d = s1 - s0
+            per[k] = (c + 1, tot + d, d if mn is None else min(mn, d), max(mx, d))
+    span = (t1 - t0) / 1e6
+    busy = sum(v[1] for v in per.values()) / 1e6
+    print(f"region {region}: {span:.2f} ms wall, kernels busy {busy:.2f} ms ({100 * busy / span:.1f} %), "
+          f"{sum(v[0] for v in per.values())} dispatches")
+    print("| kernel | calls | total ms | avg us | min us | max us | % of busy |")
+    print("|---|---:|---:|---:|---:|---:|---:|")
+    for k, (c, tot, mn, mx) in sorted(per.items(), key=lambda kv: -kv[1][1])[:top]:
+        name = k if len(k) <= 80 else k[:77] + "..."
+        print(f"| `{name}` | {c} | {tot / 1e6:.2f} | {tot / c / 1e3:.2f} | {mn / 1e3:.2f} | {mx / 1e3:.2f} | "
+              f"{100 * tot / 1e6 / busy:.1f} |")
+
+
 def _step_record(out):
     keys = ("certified", "iterations", "restarts", "polish_attempts", "cg_steps", "cg_lp_steps", "pdhg_ms",
             "check_ms", "polish_ms", "wall_seconds", "fallback_lps", "fallback_certified", "fallback_seconds",
-            "start_mlp")
+            "start_mlp", "active_changed_columns", "active_changed_rows", "active_changed_share", "active_columns")
     return {k: out.get(k) for k in keys}
 
 
@@ -534,7 +610,12 @@ def main():
                     help="skip the cooperative stage-driver measurement (max-min + parsimonious exchange)")
     ap.add_argument("--workload", choices=("synthetic", "gem"), default="synthetic",
                     help="synthetic: seeded 3-species consortium; gem: the reference's member GEMs (SBML)")
+    ap.add_argument("--trace-summary", metavar="DIR",
+                    help="print the per-kernel table of the timed steps from a rocprofv3 CSV run in DIR and exit")
     args = ap.parse_args()
+    if args.trace_summary:
+        trace_summary(args.trace_summary)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the LPs are independent and would shard "
                                    "by environment with no per-iteration communication; run --gpus 1"}))
@@ -589,16 +670,17 @@ def main():
     snapshots, certified, step_logs, sample_x, sample_ok = [], 0, [], [], []
     sample_t = torch.as_tensor(sample, device=dev)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(K):
-        snapshots.append(target.snapshot())
-        out = target.step()
-        certified += int(out["certified"])
-        sample_x.append(out["x"][sample_t])
-        sample_ok.append(np.asarray(out["certified_mask"])[sample])
-        step_logs.append(_step_record(out))
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    with _Range("timed-steps"):
+        t0 = time.perf_counter()
+        for k in range(K):
+            snapshots.append(target.snapshot())
+            out = target.step()
+            certified += int(out["certified"])
+            sample_x.append(out["x"][sample_t])
+            sample_ok.append(np.asarray(out["certified_mask"])[sample])
+            step_logs.append(_step_record(out))
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
     value = certified / elapsed
     print(f"# target: {certified}/{B * K} certified in {elapsed:.2f} s", file=sys.stderr, flush=True)
     sample_x = [t.cpu().numpy() for t in sample_x]
@@ -610,6 +692,7 @@ def main():
     if not args.no_compare and args.warm_start == "mlp":
         prev_ro = ConsortiumRollout(model, B, seed=args.seed, precision="fp32", options=target_options(),
                                     warm_start="previous", device=dev, engine=eng, tuning=tuning)
+        prev_ro.track_active = True                 # active-set changes per step (this leg is not the value)
         p_cert, p_logs = 0, []
         torch.cuda.synchronize()
         t1 = time.perf_counter()
@@ -624,7 +707,17 @@ def main():
                                            "certified": p_cert, "seconds": round(p_el, 3),
                                            "iterations": [l["iterations"] for l in p_logs],
                                            "polish_attempts": [l["polish_attempts"] for l in p_logs],
-                                           "wall_ms": [round(1e3 * l["wall_seconds"], 2) for l in p_logs]},
+                                           "wall_ms": [round(1e3 * l["wall_seconds"], 2) for l in p_logs],
+                                           "active_set_changes": {
+                                               "note": "per step against the previous step, environments certified "
+                                                       "at both: mean columns that left or reached a bound, mean "
+                                                       "inequality rows whose dual became or stopped being "
+                                                       "nonzero, share of environments with any change; mean "
+                                                       "columns at a bound",
+                                               "columns": [l.get("active_changed_columns") for l in p_logs],
+                                               "rows": [l.get("active_changed_rows") for l in p_logs],
+                                               "share": [l.get("active_changed_share") for l in p_logs],
+                                               "columns_at_bound": [l.get("active_columns") for l in p_logs]}},
                    "learned_start": {"value": round(value, 3), "certified": certified, "seconds": round(elapsed, 3),
                                      "iterations": [l["iterations"] for l in step_logs],
                                      "polish_attempts": [l["polish_attempts"] for l in step_logs],

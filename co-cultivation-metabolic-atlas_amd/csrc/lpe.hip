@@ -701,6 +701,17 @@ int launch_infeasibility(lpe_handle* h, double eps) {
     return launch_infeasibility_t<double>(h, eps);
 }
 
+// Per-LP norms of a batch-minor vector; the chunk partials use the block
+// partial buffer (sized for many more values per LP than rows / NORM_CHUNK).
+void launch_norms(lpe_handle* h, const double* v, int rows, int shared, double* out) {
+    const int chunks = ceil_div(rows, NORM_CHUNK);
+    if (chunks > 0)
+        hipLaunchKernelGGL(k_norms_part, dim3(ceil_div(h->ld, 256), chunks), dim3(256), 0, h->s, v, rows, h->ld,
+                           shared, h->d_partial);
+    hipLaunchKernelGGL(k_norms_final, dim3(ceil_div(h->ld, 256)), dim3(256), 0, h->s, (const double*)h->d_partial,
+                       chunks, h->ld, out);
+}
+
 int launch_check(lpe_handle* h, int K, int certify, double ep, double ed, double eg) {
     if (h->method != LPE_METHOD_FIXED && h->precision == LPE_PREC_FP32)
         return launch_check_t<float>(h, K, certify, ep, ed, eg);
@@ -1732,8 +1743,8 @@ int lpe_update(lpe_handle* h, const lpe_vectors* v, void* stream) {
         shared_metric(h);
         CHK(upload_metric(h));
     }
-    hipLaunchKernelGGL(k_norms, dim3(ceil_div(ld, 256)), tb, 0, h->s, (const double*)h->d_rhs_s, (int)m, (int)ld, 0, h->d_bnorm);
-    hipLaunchKernelGGL(k_norms, dim3(ceil_div(ld, 256)), tb, 0, h->s, (const double*)h->d_c_s, (int)n, (int)ld, h->shared_c, h->d_cnorm);
+    launch_norms(h, (const double*)h->d_rhs_s, (int)m, 0, h->d_bnorm);
+    launch_norms(h, (const double*)h->d_c_s, (int)n, h->shared_c, h->d_cnorm);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->s));
     h->have_vectors = true;

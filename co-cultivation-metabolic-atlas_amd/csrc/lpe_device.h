@@ -1832,14 +1832,28 @@ __global__ void __launch_bounds__(BLOCK) k_ref_metric(RefMetricArgs a) {
 
 // Per-LP 2-norm of an internal [rows][ld] FP64 vector (or of a shared [rows]
 // vector when shared != 0).
-__global__ void k_norms(const double* __restrict__ v, int rows, int ld, int shared, double* __restrict__ out) {
+// Per-LP Euclidean norms of a [rows][ld] (or shared [rows]) vector in two
+// launches: squared sums of NORM_CHUNK-row chunks (grid: LP tiles x chunks),
+// then the chunks summed in order per LP (deterministic).
+constexpr int NORM_CHUNK = 64;
+
+__global__ void k_norms_part(const double* __restrict__ v, int rows, int ld, int shared, double* __restrict__ part) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= ld) return;
+    const int r0 = blockIdx.y * NORM_CHUNK, r1 = min(rows, r0 + NORM_CHUNK);
     double s = 0.0;
-    for (int r = 0; r < rows; ++r) {
+    for (int r = r0; r < r1; ++r) {
         const double x = shared ? v[r] : v[(long)r * ld + b];
         s += x * x;
     }
+    part[(long)blockIdx.y * ld + b] = s;
+}
+
+__global__ void k_norms_final(const double* __restrict__ part, int chunks, int ld, double* __restrict__ out) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= ld) return;
+    double s = 0.0;
+    for (int k = 0; k < chunks; ++k) s += part[(long)k * ld + b];
     out[b] = sqrt(s);
 }
 

@@ -224,6 +224,44 @@ class WarmStartNet:
         reg = o[:, :k] * self.out_std + self.out_mean
         return reg[:, :self.nx], reg[:, self.nx:k], o[:, k:k + self.na]
 
+    # ------------------------------------------------------ checkpoints
+    def save(self, path: str) -> None:
+        """Weights, biases, Adam moments and step, and the input / output
+        standardisation in one safetensors file (nothing executable)."""
+        from safetensors.torch import save_file
+        if self.in_mean is None:
+            raise RuntimeError("the network has no normalisation statistics yet (fit or fit_normalization first)")
+        t = {}
+        for i, (p, m, v) in enumerate(zip(self.params, self.m, self.v)):
+            t[f"param.{i}"], t[f"adam_m.{i}"], t[f"adam_v.{i}"] = p, m, v
+        t.update(in_mean=self.in_mean, in_std=self.in_std, out_mean=self.out_mean, out_std=self.out_std)
+        meta = {"dims": ",".join(str(d) for d in self.dims), "nx": str(self.nx), "ny": str(self.ny),
+                "na": str(self.na), "n_in": str(self.n_in), "step_count": str(self.step_count),
+                "trained": str(int(self.trained))}
+        save_file({k: v.detach().contiguous().cpu() for k, v in t.items()}, path, metadata=meta)
+
+    def load(self, path: str) -> None:
+        """Restore a checkpoint written by ``save`` for a network of the same
+        widths (the BF16 copies are rebuilt from the FP32 master weights)."""
+        from safetensors import safe_open
+        with safe_open(path, framework="pt", device="cpu") as f:
+            meta = f.metadata()
+            if meta.get("dims") != ",".join(str(d) for d in self.dims) or int(meta["nx"]) != self.nx \
+                    or int(meta["ny"]) != self.ny or int(meta["na"]) != self.na or int(meta["n_in"]) != self.n_in:
+                raise ValueError("the checkpoint's network widths differ from this network's")
+            for i in range(len(self.params)):
+                self.params[i].copy_(f.get_tensor(f"param.{i}"))
+                self.m[i].copy_(f.get_tensor(f"adam_m.{i}"))
+                self.v[i].copy_(f.get_tensor(f"adam_v.{i}"))
+            self.in_mean, self.in_std, self.out_mean, self.out_std = (
+                f.get_tensor(k).to(self.device) for k in ("in_mean", "in_std", "out_mean", "out_std"))
+            self.step_count = int(meta["step_count"])
+            self.trained = bool(int(meta["trained"]))
+        for l in range(len(self.w)):
+            to_bf16(self.w[l], self.wb[l])
+            transpose(self.wb[l], self.wbt[l])
+        self._inf = None
+
     # ------------------------------------------------- fused inference
     def _inference_layout(self):
         """Output layer for ``infer_start``: rows of the [out][in] BF16 weights

@@ -192,3 +192,34 @@ def test_fused_inference_equals_the_unfused_forward(ws, device):
     on_bound = (x0 == lower) | (x0 == upper)
     assert bool(on_bound[:, 5:][at[:, 5:]].all())
     assert bool((x0[:, :5] == xr[:, :5].to(torch.float64)).all())
+
+
+def test_checkpoint_round_trip_restores_predictions_and_adam(ws, device, tmp_path):
+    """save / load: the restored network predicts the same starts bit for
+    bit, and one more Adam step from the restored state equals the step the
+    original takes (moments and step count restored)."""
+    rng = np.random.default_rng(11)
+    feats = torch.tensor(rng.normal(size=(512, 20)), dtype=torch.float32, device=device)
+    targ = torch.tensor(np.concatenate([rng.normal(size=(512, 14)), (rng.random((512, 8)) > 0.5)], axis=1),
+                        dtype=torch.float32, device=device)
+    a = ws.WarmStartNet(20, 8, 6, 8, hidden=64, layers=2, device=device, seed=1)
+    a.fit(feats, targ, epochs=3, batch=128, lr=1e-3)
+    path = str(tmp_path / "net.safetensors")
+    a.save(path)
+    b = ws.WarmStartNet(20, 8, 6, 8, hidden=64, layers=2, device=device, seed=99)
+    b.load(path)
+    lo = torch.full((512, 8), -5.0, dtype=torch.float64, device=device)
+    hi = torch.full((512, 8), 5.0, dtype=torch.float64, device=device)
+    xa, ya = a.infer_start(feats, lo, hi, 3)
+    xb, yb = b.infer_start(feats, lo, hi, 3)
+    assert torch.equal(xa, xb) and torch.equal(ya, yb)
+    xs, ts = a._inputs(feats[:128]), a._targets(targ[:128])
+    _, ga = a.loss_and_grad(xs, ts)
+    a.adam_step(ga, 1e-3)
+    _, gb = b.loss_and_grad(xs, ts)
+    b.adam_step(gb, 1e-3)
+    for pa, pb in zip(a.params, b.params):
+        assert torch.equal(pa, pb)
+    c = ws.WarmStartNet(20, 8, 6, 9, hidden=64, layers=2, device=device)
+    with pytest.raises(ValueError):
+        c.load(path)
