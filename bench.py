@@ -49,59 +49,49 @@ def _primal_bytes(B, m, n, nnz, nvar, w):
     return B * per_lp + shared
 
 
-def _cg_update_bytes(B, m, n_rb, weighted=False):
-    """One polish CG update, both systems, every LP active: w, p, r, s of two
-    systems read, w and r written, the Jacobi diagonal read (13 m FP64 words
-    per LP), alpha / activity / selection per LP, the block partials of three
-    dot products written; the row scales once when the stop test is weighted."""
-    return B * (13 * m * 8 + 2 * 8 + 2 * 4 + 4 + 3 * n_rb * 8) + (m * 8 if weighted else 0)
-
-
-def _update_row_blocks(m):
-    """Row blocks of the CG update launch (16 waves per block, one row per wave)."""
-    return max(1, -(-m // 16))
-
-
-def _polish_row_blocks(m, ld):
-    """Row blocks of the polish launches (4 waves per block, rows per wave by batch)."""
+def _polish_row_blocks(rows, ld):
+    """Row blocks of a polish CG launch over `rows` rows (4 waves per block,
+    rows per wave by batch width)."""
     rpw = 4 if ld >= 1024 else (2 if ld >= 256 else 1)
-    return max(1, -(-m // (4 * rpw)))
+    return max(1, -(-rows // (4 * rpw)))
 
 
-def _polish_cols_bytes(B, m, n, nnz, nvar):
-    """One polish columns pass q = D A^T p, both CG systems, every LP active:
-    p1, p2 gathered (2 m FP64 words per LP), q1, q2 written (2 n), the free
-    mask (n bytes), per-LP coefficients, activity / selection; the shared
+def _cg_cols_bytes(B, m, n, nnz, nvar, n_rb):
+    """One CG columns pass q = D A^T p with the |q|^2 partials, both systems,
+    every LP active: p (interleaved p1, p2) gathered once (2 m FP64 words per
+    LP), q written (2 n), the free mask (n bytes), per-LP coefficients,
+    activity / selection, two partials per column block; the shared
     transposed structure and scaled values once."""
-    per_lp = (2 * m + 2 * n) * 8 + n + nvar * 8 + 12
+    per_lp = (2 * m + 2 * n) * 8 + n + nvar * 8 + 12 + 2 * n_rb * 8
     return B * per_lp + nnz * 16 + (n + 1) * 4
 
 
-def _polish_rows_bytes(B, m, n, nnz, nvar, n_rb):
-    """One polish rows pass s = R A q with the p.s partials, both systems:
-    q1, q2 gathered (2 n words), p1, p2 read and s1, s2 written (4 m), the row
-    mask, per-LP coefficients, two partials per row block; shared CSR once."""
-    per_lp = (2 * n + 4 * m) * 8 + m + nvar * 8 + 2 * n_rb * 8 + 12
-    return B * per_lp + nnz * 16 + (m + 1) * 4
+def _cg_rows_bytes(B, m, n, nnz, nvar, n_rb):
+    """One CG rows pass with the update, both systems, every LP active: q
+    gathered once (2 n words), p read (2 m), w and r of both systems read and
+    written (8 m), the Jacobi diagonal (m) and row mask (m bytes), per-LP
+    coefficients, alpha / activity / selection, three partials per row block;
+    the shared CSR structure, values and row scales once."""
+    per_lp = (2 * n + 11 * m) * 8 + m + nvar * 8 + 16 + 12 + 3 * n_rb * 8
+    return B * per_lp + nnz * 16 + (m + 1) * 4 + m * 8
 
 
 def _cg_direction_bytes(B, m):
     """One CG direction update p = D^-1 r + beta p, both systems, every LP
-    active: r1, p1, r2, p2 and the Jacobi diagonal read, p1, p2 written
-    (7 m FP64 words per LP), beta / activity / selection per LP."""
+    active: r1, r2 and the Jacobi diagonal read, p (interleaved) read and
+    written (7 m FP64 words per LP), beta / activity / selection per LP."""
     return B * (7 * m * 8 + 2 * 8 + 12)
 
 
-def _cg_step_bytes(m, n, nnz, nvar, n_rb, n_rb_update):
+def _cg_step_bytes(m, n, nnz, nvar, ld):
     """Algorithmic bytes of one polish CG step (both systems): (per active LP,
-    shared per step) - the columns and rows passes, the CG update (13 m words)
-    and the direction p = D^-1 r + beta p (7 m words)."""
-    cols = _polish_cols_bytes(1, m, n, 0, nvar) - (n + 1) * 4
-    rows = _polish_rows_bytes(1, m, n, 0, nvar, n_rb) - (m + 1) * 4
-    update = 13 * m * 8 + 3 * n_rb_update * 8
-    direction = 7 * m * 8
-    shared = 2 * (nnz * 16) + (n + 1) * 4 + (m + 1) * 4
-    return cols + rows + update + direction + 24, shared
+    shared per step) - the columns pass, the rows pass with the update and
+    the direction."""
+    cols = _cg_cols_bytes(1, m, n, 0, nvar, _polish_row_blocks(n, ld)) - (n + 1) * 4
+    rows = _cg_rows_bytes(1, m, n, 0, nvar, _polish_row_blocks(m, ld)) - (m + 1) * 4 - m * 8
+    direction = _cg_direction_bytes(1, m)
+    shared = 2 * (nnz * 16) + (n + 1) * 4 + (m + 1) * 4 + m * 8
+    return cols + rows + direction, shared
 
 
 def _dual_bytes(B, m, n, nnz, nvar, w):
@@ -235,9 +225,8 @@ def probe_kernels(args):
     eng.run(64, chunk=64, use_graph=False)
     ms_p = eng.time_kernel("primal", reps=args.probe_reps)
     ms_d = eng.time_kernel("dual", reps=args.probe_reps)
-    ms_cg = eng.time_kernel("cg_update", reps=args.probe_reps)
-    ms_pc = eng.time_kernel("polish_cols", reps=args.probe_reps)
-    ms_pr = eng.time_kernel("polish_rows", reps=args.probe_reps)
+    ms_cc = eng.time_kernel("cg_cols", reps=args.probe_reps)
+    ms_cr = eng.time_kernel("cg_rows", reps=args.probe_reps)
     ms_cd = eng.time_kernel("cg_direction", reps=args.probe_reps)
     n_copy = 256 << 20                                   # 256 Mi float32 = 1 GiB read + 1 GiB written
     src = torch.ones(n_copy, dtype=torch.float32, device=dev)
@@ -248,15 +237,15 @@ def probe_kernels(args):
     m, n = model.shape
     print(json.dumps({"probe": "step kernels", "envs": args.envs, "rows_per_wave": args.rows_per_wave,
                       "lps_per_lane": args.lps_per_lane, "reps": args.probe_reps,
-                      "primal_ms": ms_p, "dual_ms": ms_d, "cg_update_ms": ms_cg,
-                      "polish_cols_ms": ms_pc, "polish_rows_ms": ms_pr, "cg_direction_ms": ms_cd,
+                      "primal_ms": ms_p, "dual_ms": ms_d,
+                      "cg_cols_ms": ms_cc, "cg_rows_ms": ms_cr, "cg_direction_ms": ms_cd,
+                      "cg_cols_algorithmic_bytes": _cg_cols_bytes(args.envs, m, n, model.A.nnz,
+                                                                  model.var_positions.size,
+                                                                  _polish_row_blocks(n, args.envs)),
+                      "cg_rows_algorithmic_bytes": _cg_rows_bytes(args.envs, m, n, model.A.nnz,
+                                                                  model.var_positions.size,
+                                                                  _polish_row_blocks(m, args.envs)),
                       "cg_direction_algorithmic_bytes": _cg_direction_bytes(args.envs, m),
-                      "cg_update_algorithmic_bytes": _cg_update_bytes(args.envs, m, _update_row_blocks(m)),
-                      "polish_cols_algorithmic_bytes": _polish_cols_bytes(args.envs, m, n, model.A.nnz,
-                                                                          model.var_positions.size),
-                      "polish_rows_algorithmic_bytes": _polish_rows_bytes(args.envs, m, n, model.A.nnz,
-                                                                          model.var_positions.size,
-                                                                          _polish_row_blocks(m, args.envs)),
                       "copy_bytes_read": 4 * n_copy,
                       "copy_bytes_written": 4 * n_copy,
                       "primal_algorithmic_bytes": _primal_bytes(args.envs, m, n, model.A.nnz,
@@ -267,8 +256,8 @@ def probe_kernels(args):
 
 def measured_traffic(path, envs, lps_per_lane, key="primal_step_hbm_bytes"):
     """HBM bytes per launch from a committed counter summary: the line
-    '<key> envs=E lpl=L fetch=F write=W' (key primal_step_hbm_bytes or
-    cg_update_hbm_bytes)."""
+    '<key> envs=E lpl=L fetch=F write=W' (key primal_step_hbm_bytes,
+    cg_cols_hbm_bytes, cg_rows_hbm_bytes or cg_direction_hbm_bytes)."""
     try:
         with open(path) as f:
             for line in f:
@@ -519,10 +508,12 @@ class _Range:
         return False
 
 
-def trace_summary(directory, region="timed-steps", top=30):
-    """Per-kernel time inside one marked region of a rocprofv3 CSV run
+def trace_summary(directory, regions=("timed-steps", "kernel-timing"), top=30):
+    """Per-kernel time inside marked regions of a rocprofv3 CSV run
     (--kernel-trace --marker-trace --output-format csv): dispatches whose
-    start lies inside the region, summed per kernel name."""
+    start lies inside each region, summed per kernel name.  "timed-steps"
+    is the timed rollout; "kernel-timing" holds the all-LPs-active launches
+    whose hipEvent averages the roofline reports."""
     import csv
     import glob
 
@@ -539,14 +530,21 @@ def trace_summary(directory, region="timed-steps", top=30):
                 return row[n]
         raise KeyError(names)
 
-    marks = [r for r in rows("*marker_api_trace.csv")
-             if region in col(r, "Function", "Name", "Message", "Marker_Name")]
-    if not marks:
-        raise SystemExit(f"region {region!r} not found")
-    t0 = int(col(marks[0], "Start_Timestamp"))
-    t1 = int(col(marks[0], "End_Timestamp"))
+    markers = rows("*marker_api_trace.csv")
+    kernels = rows("*kernel_trace.csv")
+    for region in regions:
+        marks = [r for r in markers if region in col(r, "Function", "Name", "Message", "Marker_Name")]
+        if not marks:
+            print(f"region {region}: not recorded\n")
+            continue
+        _region_table(region, marks[0], kernels, col, top)
+
+
+def _region_table(region, mark, kernels, col, top):
+    t0 = int(col(mark, "Start_Timestamp"))
+    t1 = int(col(mark, "End_Timestamp"))
     per = {}
-    for r in rows("*kernel_trace.csv"):
+    for r in kernels:
         s0, s1 = int(col(r, "Start_Timestamp")), int(col(r, "End_Timestamp"))
         if t0 <= s0 <= t1:
             k = col(r, "Kernel_Name")
@@ -563,6 +561,7 @@ def trace_summary(directory, region="timed-steps", top=30):
         name = k if len(k) <= 80 else k[:77] + "..."
         print(f"| `{name}` | {c} | {tot / 1e6:.2f} | {tot / c / 1e3:.2f} | {mn / 1e3:.2f} | {mx / 1e3:.2f} | "
               f"{100 * tot / 1e6 / busy:.1f} |")
+    print()
 
 
 def _step_record(out):
@@ -611,7 +610,8 @@ def main():
     ap.add_argument("--workload", choices=("synthetic", "gem"), default="synthetic",
                     help="synthetic: seeded 3-species consortium; gem: the reference's member GEMs (SBML)")
     ap.add_argument("--trace-summary", metavar="DIR",
-                    help="print the per-kernel table of the timed steps from a rocprofv3 CSV run in DIR and exit")
+                    help="print the per-kernel tables of the timed steps and of the kernel-timing launches from a "
+                         "rocprofv3 CSV run in DIR and exit")
     args = ap.parse_args()
     if args.trace_summary:
         trace_summary(args.trace_summary)
@@ -731,16 +731,15 @@ def main():
     # LPs active in them): PDHG launches run every LP, CG launches only the
     # LPs whose CG systems are still active (cg_lp_steps / B full launches)
     eng.reset(*final_state)
-    ms_primal = eng.time_kernel("primal", reps=20)
-    ms_dual = eng.time_kernel("dual", reps=20)
-    ms_cg = eng.time_kernel("cg_update", reps=20)
-    ms_pc = eng.time_kernel("polish_cols", reps=20)
-    ms_pr = eng.time_kernel("polish_rows", reps=20)
-    ms_cd = eng.time_kernel("cg_direction", reps=20)
+    with _Range("kernel-timing"):
+        ms_primal = eng.time_kernel("primal", reps=20)
+        ms_dual = eng.time_kernel("dual", reps=20)
+        ms_cc = eng.time_kernel("cg_cols", reps=20)
+        ms_cr = eng.time_kernel("cg_rows", reps=20)
+        ms_cd = eng.time_kernel("cg_direction", reps=20)
     lpl = int((tuning or {}).get("lps_per_lane", 1))
     pb = _primal_bytes(B, m, n, nnz, nvar, 4)
     db = _dual_bytes(B, m, n, nnz, nvar, 4)
-    cgb = _cg_update_bytes(B, m, _update_row_blocks(m))
     mean_iters = float(np.mean([s["iterations"] for s in step_logs]))
     full_cg = float(np.mean([(s.get("cg_lp_steps") or 0) for s in step_logs])) / B
     kernels = {
@@ -753,22 +752,18 @@ def main():
                                                 achieved=round(db / (ms_dual * 1e-3) / 1e9, 1),
                                                 full_launches_per_step=round(mean_iters, 2),
                                                 ms_per_step=round(mean_iters * ms_dual, 3)),
-        "k_cg_update": dict(algorithmic_bytes_per_launch=cgb, avg_launch_ms=round(ms_cg, 5),
-                            achieved=round(cgb / (ms_cg * 1e-3) / 1e9, 1), full_launches_per_step=round(full_cg, 2),
-                            ms_per_step=round(full_cg * ms_cg, 3),
-                            traffic=measured_traffic(args.traffic_file, B, lpl, key="cg_update_hbm_bytes")),
     }
-    pcb = _polish_cols_bytes(B, m, n, nnz, nvar)
-    prb = _polish_rows_bytes(B, m, n, nnz, nvar, _polish_row_blocks(m, B))
-    for name, nbytes, ms, key in (("k_polish_cols<2>", pcb, ms_pc, "polish_cols_hbm_bytes"),
-                                  ("k_polish_rows<2>", prb, ms_pr, "polish_rows_hbm_bytes"),
+    ccb = _cg_cols_bytes(B, m, n, nnz, nvar, _polish_row_blocks(n, B))
+    crb = _cg_rows_bytes(B, m, n, nnz, nvar, _polish_row_blocks(m, B))
+    for name, nbytes, ms, key in (("k_cg_cols<1,4,4>", ccb, ms_cc, "cg_cols_hbm_bytes"),
+                                  ("k_cg_rows<1,4,4>", crb, ms_cr, "cg_rows_hbm_bytes"),
                                   ("k_cg_direction", _cg_direction_bytes(B, m), ms_cd, "cg_direction_hbm_bytes")):
         kernels[name] = dict(algorithmic_bytes_per_launch=nbytes, avg_launch_ms=round(ms, 5),
                              achieved=round(nbytes / (ms * 1e-3) / 1e9, 1), full_launches_per_step=round(full_cg, 2),
                              ms_per_step=round(full_cg * ms, 3),
                              traffic=measured_traffic(args.traffic_file, B, lpl, key=key))
     dominant = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
-    per_lp_cg, shared_cg = _cg_step_bytes(m, n, nnz, nvar, _polish_row_blocks(m, B), _update_row_blocks(m))
+    per_lp_cg, shared_cg = _cg_step_bytes(m, n, nnz, nvar, B)
     mean_cg = float(np.mean([s.get("cg_steps") or 0 for s in step_logs]))
     pdhg_step_bytes = mean_iters * (pb + db)
     cg_step_bytes = full_cg * B * per_lp_cg + mean_cg * shared_cg

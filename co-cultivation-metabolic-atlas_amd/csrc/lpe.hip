@@ -64,7 +64,7 @@ Shape make_shape_waves(int rows, int ld, int batch, int rpw, int lpl, int waves,
     s.ld = ld;
     s.batch = batch;
     s.rpw = rpw;
-    s.n_tiles = ceil_div(ld, WAVE * lpl);
+    s.n_tiles = ceil_div(std::min(std::max(batch, 1), ld), WAVE * lpl);
     s.n_rb = std::max(1, ceil_div(rows, waves * rpw));
     s.n_real = s.n_tiles * s.n_rb;
     *grid = ((s.n_real + 7) / 8) * 8;
@@ -153,9 +153,11 @@ struct lpe_handle {
     int *d_ops_pinf = nullptr, *d_ops_pinit = nullptr, *d_ops_ir = nullptr, *d_ops_ic = nullptr;
 
     // polish / results
-    double *d_xs = nullptr, *d_rp_ = nullptr, *d_pp = nullptr, *d_wp = nullptr;
-    double *d_rd = nullptr, *d_pd = nullptr, *d_wd = nullptr;
-    double *d_q1 = nullptr, *d_q2 = nullptr, *d_s1 = nullptr, *d_s2 = nullptr;
+    double *d_xs = nullptr, *d_rp_ = nullptr, *d_wp = nullptr;
+    double *d_rd = nullptr, *d_wd = nullptr;
+    double* d_q1 = nullptr;                     // dual set-up product (and lpe_spmv scratch)
+    double2* d_pcg = nullptr;                   // CG directions, interleaved (p1, p2) [m][ld]
+    double2* d_qcg = nullptr;                   // CG products, interleaved (q1, q2) [n][ld]
     uint8_t *d_fmask = nullptr, *d_rmask = nullptr;
     double *d_rho = nullptr, *d_alpha = nullptr, *d_beta = nullptr, *d_invdiag = nullptr, *d_rho0 = nullptr;
     int* d_cgact = nullptr;
@@ -485,16 +487,6 @@ int polish_rpw_h(const lpe_handle* h, int ld) {
     const Shape s = make_shape(std::max(h->m, 1), ld, ld, k, 1, &g);
     return (size_t)s.n_rb * 3 * ld <= h->partial_elems ? k : polish_rpw_default(ld);
 }
-// Rows per wave of the CG update: one (measured at 4096 LPs: 0.29 ms against
-// 0.34 ms at four rows, for 4x the block partials); LPE_CG_UPDATE_RPW
-// overrides it for sweeps, within the partial buffers sized at create.
-int cg_update_rpw(const lpe_handle* h, int ld) {
-    const int k = env_choice("LPE_CG_UPDATE_RPW", 1, 16);
-    const int r = k ? k : 1;
-    int g;
-    const Shape s = make_shape(std::max(h->m, 1), ld, ld, r, 1, &g);
-    return (size_t)s.n_rb * 3 * ld <= h->partial_elems ? r : polish_rpw_default(ld);
-}
 // LPs per lane of the polish gather kernels (wide batches: two FP64 values per lane)
 int polish_lpl(int ld) {
     const int k = env_choice("LPE_POLISH_LPL", 1, 4);
@@ -502,46 +494,114 @@ int polish_lpl(int ld) {
     return ld >= 512 ? 2 : 1;
 }
 
-// LPE_POLISH_SPLIT=1: the CG gather passes run one launch per CG system
-// (half the gathered vector per LP tile in flight), bit-identical results.
-int polish_split() { return env_choice("LPE_POLISH_SPLIT", 1, 1); }
-
-void launch_polish_cols_one(lpe_handle* h, const PolishColsArgs& b, int grid, int lpl) {
-    if (lpl == 4) hipLaunchKernelGGL(k_polish_cols<4>, dim3(grid), dim3(BLOCK), 0, h->s, b);
-    else if (lpl == 2) hipLaunchKernelGGL(k_polish_cols<2>, dim3(grid), dim3(BLOCK), 0, h->s, b);
-    else hipLaunchKernelGGL(k_polish_cols<1>, dim3(grid), dim3(BLOCK), 0, h->s, b);
+// LPs per lane of the CG columns and rows passes.  Their vectors hold both
+// systems interleaved, so one LP per lane already moves 16 bytes per entry
+// and an LP tile is 64 LPs: the gathered vector of one tile's species block
+// (about 1200 rows x 1 KB) stays within an XCD's 4 MB L2.  Measured at 4096
+// LPs: rows pass 0.393 ms at one LP per lane against 0.559 ms at two (and
+// 0.478 ms for the earlier layout, one vector per system at two LPs per
+// lane).  LPE_CG_LPL=2 for sweeps.
+int cg_lpl(int ld) {
+    const int k = env_choice("LPE_CG_LPL", 1, 2);
+    return (k == 2 && ld >= 128) ? 2 : 1;
 }
+
+// Gather depth of the CG passes (sparse entries whose gathers are in flight
+// together per wave): 4 (54-63 VGPRs at one LP per lane, 8 waves per SIMD;
+// measured at 4096 LPs: columns pass 0.288 against 0.362 ms at depth 8, rows
+// pass 0.393 against 0.401 ms); LPE_CG_GATHER=8 for sweeps.  The result
+// does not depend on it.
+int cg_gather_depth() { return env_choice("LPE_CG_GATHER", 4, 8) == 8 ? 8 : 4; }
 
 void launch_polish_cols(lpe_handle* h, const PolishColsArgs& a, int ldv) {
     int grid;
     PolishColsArgs b = a;
     const int lpl = polish_lpl(ldv);
     b.sh = make_shape(a.sh.rows, ldv, ldv, a.sh.rpw, lpl, &grid);
-    if (polish_split() && !a.c && a.p1 && a.p2) {
-        PolishColsArgs b1 = b, b2 = b;
-        b1.p2 = nullptr; b1.q2 = nullptr;
-        b2.p1 = nullptr; b2.q1 = nullptr;
-        launch_polish_cols_one(h, b1, grid, lpl);
-        launch_polish_cols_one(h, b2, grid, lpl);
-        return;
+    if (lpl == 4) hipLaunchKernelGGL(k_polish_cols<4>, dim3(grid), dim3(BLOCK), 0, h->s, b);
+    else if (lpl == 2) hipLaunchKernelGGL(k_polish_cols<2>, dim3(grid), dim3(BLOCK), 0, h->s, b);
+    else hipLaunchKernelGGL(k_polish_cols<1>, dim3(grid), dim3(BLOCK), 0, h->s, b);
+}
+
+// Returns the number of row blocks (partials per LP and value) of the launch.
+int launch_polish_rows(lpe_handle* h, const PolishRowsArgs& a, int ldv) {
+    int grid;
+    PolishRowsArgs b = a;
+    const int lpl = polish_lpl(ldv);
+    b.sh = make_shape(a.sh.rows, ldv, ldv, a.sh.rpw, lpl, &grid);
+    if (lpl == 4) hipLaunchKernelGGL(k_polish_rows<4>, dim3(grid), dim3(BLOCK), 0, h->s, b);
+    else if (lpl == 2) hipLaunchKernelGGL(k_polish_rows<2>, dim3(grid), dim3(BLOCK), 0, h->s, b);
+    else hipLaunchKernelGGL(k_polish_rows<1>, dim3(grid), dim3(BLOCK), 0, h->s, b);
+    return b.sh.n_rb;
+}
+
+// Waves per block and rows per wave of the CG columns and rows passes.  The
+// rows (columns) an XCD walks at once are its resident waves x rows per
+// wave: with 4-wave blocks at 4 rows per wave that is about 4096 of one LP
+// tile's rows - every species block, more than the 4 MB L2 holds of the
+// gathered vector - while 16-wave blocks at one row per wave walk about 1024
+// (one species block) with the same number of block partials.
+// LPE_CG_WAVES (4 or 16) and LPE_CG_RPW (1-16) override them for sweeps.
+int cg_waves() { return env_choice("LPE_CG_WAVES", 4, 16) == 16 ? 16 : 4; }
+int cg_rpw(int ld) {
+    const int k = env_choice("LPE_CG_RPW", 1, 16);
+    if (k) return k;
+    return cg_waves() == 16 ? 1 : polish_rpw_default(ld);
+}
+
+template <int LPL, int D>
+void launch_cg_cols_w(lpe_handle* h, const CgColsArgs& b, int grid, int waves) {
+    if (waves == 16) hipLaunchKernelGGL((k_cg_cols<LPL, D, 16>), dim3(grid), dim3(16 * WAVE), 0, h->s, b);
+    else hipLaunchKernelGGL((k_cg_cols<LPL, D, 4>), dim3(grid), dim3(4 * WAVE), 0, h->s, b);
+}
+
+template <int LPL, int D>
+void launch_cg_rows_w(lpe_handle* h, const CgRowsArgs& b, int grid, int waves) {
+    if (waves == 16) hipLaunchKernelGGL((k_cg_rows<LPL, D, 16>), dim3(grid), dim3(16 * WAVE), 0, h->s, b);
+    else hipLaunchKernelGGL((k_cg_rows<LPL, D, 4>), dim3(grid), dim3(4 * WAVE), 0, h->s, b);
+}
+
+// CG columns pass (q = D A^T p and the |q|^2 partials); returns its row-block count.
+int launch_cg_cols(lpe_handle* h, const CgColsArgs& a, int ldv) {
+    int grid;
+    CgColsArgs b = a;
+    const int lpl = cg_lpl(ldv), waves = cg_waves(), d = cg_gather_depth();
+    b.sh = make_shape_waves(a.sh.rows, ldv, ldv, cg_rpw(ldv), lpl, waves, &grid);
+    if (lpl == 2) {
+        if (d == 8) launch_cg_cols_w<2, 8>(h, b, grid, waves); else launch_cg_cols_w<2, 4>(h, b, grid, waves);
+    } else {
+        if (d == 8) launch_cg_cols_w<1, 8>(h, b, grid, waves); else launch_cg_cols_w<1, 4>(h, b, grid, waves);
     }
-    launch_polish_cols_one(h, b, grid, lpl);
+    return b.sh.n_rb;
+}
+
+// CG rows pass with the update (s = R A q, w += alpha p, r -= alpha s and
+// the partials of rho and the stop measure); returns its row-block count.
+int launch_cg_rows(lpe_handle* h, const CgRowsArgs& a, int ldv) {
+    int grid;
+    CgRowsArgs b = a;
+    const int lpl = cg_lpl(ldv), waves = cg_waves(), d = cg_gather_depth();
+    b.sh = make_shape_waves(a.sh.rows, ldv, ldv, cg_rpw(ldv), lpl, waves, &grid);
+    if (lpl == 2) {
+        if (d == 8) launch_cg_rows_w<2, 8>(h, b, grid, waves); else launch_cg_rows_w<2, 4>(h, b, grid, waves);
+    } else {
+        if (d == 8) launch_cg_rows_w<1, 8>(h, b, grid, waves); else launch_cg_rows_w<1, 4>(h, b, grid, waves);
+    }
+    return b.sh.n_rb;
 }
 
 // The CG rows pass walks the LP tiles in the reverse of the columns pass's
 // order, so it starts on the tiles of q the columns pass wrote last (still in
-// the memory-side cache); the CG update then walks forward again.  Measured at
-// 4096 LPs: 75.0 against 76.4 ms per polish, identical results (the per-LP
-// arithmetic is unchanged).  LPE_POLISH_ROWS_REVERSE=0 turns it off.
+// the memory-side cache).  LPE_POLISH_ROWS_REVERSE=0 turns it off (identical
+// results: the per-LP arithmetic is unchanged).
 int rows_reverse() {
     const char* v = std::getenv("LPE_POLISH_ROWS_REVERSE");
     return (v && v[0] == '0') ? 0 : 1;
 }
 
-// The CG direction update walks the LP tiles last to first, after the CG
-// update's forward walk, so it starts on the residual tiles written last; the
-// columns pass then walks forward again.  Measured at 4096 LPs: 73.2 against
-// 73.9 ms per polish, identical results.  LPE_CG_DIR_REVERSE=0 turns it off.
+// The CG direction update walks the LP tiles last to first (as the rows pass
+// does); LPE_CG_DIR_REVERSE=0 walks them forward.  Measured at 4096 LPs: the
+// two orders within 0.5 % of each other, identical results.
 int dir_reverse() {
     const char* v = std::getenv("LPE_CG_DIR_REVERSE");
     return (v && v[0] == '0') ? 0 : 1;
@@ -555,31 +615,6 @@ int dir_reverse() {
 int dual_reverse() {
     const char* v = std::getenv("LPE_DUAL_REVERSE");
     return (v && v[0] == '0') ? 0 : 1;
-}
-
-// Returns the number of row blocks (partials per LP and value) of the launch.
-void launch_polish_rows_one(lpe_handle* h, const PolishRowsArgs& b, int grid, int lpl) {
-    if (lpl == 4) hipLaunchKernelGGL(k_polish_rows<4>, dim3(grid), dim3(BLOCK), 0, h->s, b);
-    else if (lpl == 2) hipLaunchKernelGGL(k_polish_rows<2>, dim3(grid), dim3(BLOCK), 0, h->s, b);
-    else hipLaunchKernelGGL(k_polish_rows<1>, dim3(grid), dim3(BLOCK), 0, h->s, b);
-}
-
-int launch_polish_rows(lpe_handle* h, const PolishRowsArgs& a, int ldv) {
-    int grid;
-    PolishRowsArgs b = a;
-    const int lpl = polish_lpl(ldv);
-    b.sh = make_shape(a.sh.rows, ldv, ldv, a.sh.rpw, lpl, &grid);
-    if (polish_split() && !a.setup_r && a.q1 && a.q2) {
-        // each launch writes its own partial slot; one reduction follows
-        PolishRowsArgs b1 = b, b2 = b;
-        b1.q2 = nullptr; b1.only = 0;
-        b2.q1 = nullptr; b2.only = 1;
-        launch_polish_rows_one(h, b1, grid, lpl);
-        launch_polish_rows_one(h, b2, grid, lpl);
-        return b.sh.n_rb;
-    }
-    launch_polish_rows_one(h, b, grid, lpl);
-    return b.sh.n_rb;
 }
 
 // Device check: KKT / certificate values of (current, average), decisions, actions.
@@ -824,6 +859,14 @@ int build_results(lpe_handle* h) {
 #ifndef LPE_LD_PAD
 #define LPE_LD_PAD 0
 #endif
+// LPE_LD_PAD in the environment (a multiple of 64, at most 1024) overrides the
+// build's padding at create, for sweeps.
+int ld_pad() {
+    const char* v = std::getenv("LPE_LD_PAD");
+    if (!v) return LPE_LD_PAD;
+    const int k = std::atoi(v);
+    return (k >= 0 && k <= 1024 && k % 64 == 0) ? k : LPE_LD_PAD;
+}
 
 #ifndef LPE_CG_BLOCK
 #define LPE_CG_BLOCK 10
@@ -872,44 +915,47 @@ double cg_ptol2_h(const lpe_handle* h) {
     return t * t;
 }
 
-void cg_steps(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, int grid_r, int grid_c, int steps) {
+// Arguments of the three CG passes over a view (the buffers are fixed per
+// handle, so captured graphs replay them).
+void cg_args(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, CgColsArgs& c1, CgRowsArgs& c2,
+             CgDirArgs& dd, int& grid_d) {
+    const bool pinf = cg_primal_inf();
+    c1.sh = sc; c1.AT = op_scaled_v(h, true, v); c1.p = h->d_pcg; c1.q = h->d_qcg;
+    c1.fmask = h->d_fmask; c1.sel = v.sel; c1.act = h->d_cgact; c1.partial = h->d_partial;
+    c2.sh = sr; c2.A = op_scaled_v(h, false, v); c2.q = h->d_qcg; c2.p = h->d_pcg;
+    c2.w1 = h->d_wp; c2.r1 = h->d_rp_; c2.w2 = h->d_wd; c2.r2 = h->d_rd;
+    c2.alpha = h->d_alpha; c2.invdiag = h->d_invdiag; c2.rmask = h->d_rmask;
+    c2.dr = (pinf || LPE_CG_PRIMAL_WEIGHTED) ? h->d_dr : nullptr; c2.sel = v.sel; c2.act = h->d_cgact;
+    c2.partial = h->d_partial; c2.inf_norm = pinf ? 1 : 0; c2.reverse = rows_reverse();
+    dd.sh = make_shape(std::max(h->m, 1), v.ld, v.ld, 1, 1, &grid_d);
+    dd.p = h->d_pcg; dd.r1 = h->d_rp_;
+    dd.r2 = h->d_rd; dd.beta = h->d_beta; dd.invdiag = h->d_invdiag; dd.sel = v.sel;
+    dd.act = h->d_cgact;
+    dd.reverse = dir_reverse();
+}
+
+// `steps` CG steps for both systems: columns pass, alpha, rows pass with the
+// update, beta and the stop tests, direction.
+void cg_steps(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, int steps) {
     CgScalarArgs cs;
     cs.rho = h->d_rho; cs.rho0 = h->d_rho0; cs.alpha = h->d_alpha; cs.beta = h->d_beta;
     cs.act = h->d_cgact; cs.floor2 = h->d_cgfloor; cs.sel = v.sel; cs.rel = CG_REL; cs.batch = v.ld; cs.ld = v.ld;
     cs.ptol2 = cg_ptol2_h(h);
     const bool pinf = cg_primal_inf();
+    CgColsArgs c1;
+    CgRowsArgs c2;
+    CgDirArgs dd;
+    int grid_d = 0;
+    cg_args(h, v, sr, sc, c1, c2, dd, grid_d);
     for (int k = 0; k < steps; ++k) {
-        PolishColsArgs c1;
-        c1.sh = sc; c1.AT = op_scaled_v(h, true, v); c1.p1 = h->d_pp; c1.p2 = h->d_pd; c1.q1 = h->d_q1;
-        c1.q2 = h->d_q2; c1.c = nullptr; c1.c_shared = 0; c1.fmask = h->d_fmask; c1.sel = v.sel; c1.act = h->d_cgact;
-        launch_polish_cols(h, c1, v.ld);
-        PolishRowsArgs c2;
-        c2.sh = sr; c2.A = op_scaled_v(h, false, v); c2.q1 = h->d_q1; c2.q2 = h->d_q2; c2.s1 = h->d_s1; c2.s2 = h->d_s2;
-        c2.p1 = h->d_pp; c2.p2 = h->d_pd; c2.setup_r = nullptr; c2.setup_p = nullptr; c2.invdiag = h->d_invdiag;
-        c2.rmask = h->d_rmask; c2.sel = v.sel; c2.act = h->d_cgact; c2.partial = h->d_partial;
-        c2.reverse = rows_reverse();
-        const int nrb2 = launch_polish_rows(h, c2, v.ld);
-        reduce_ld(h, h->d_partial, h->d_red_a, h->d_ops_sum, 2, nrb2, v.ld);
+        const int nrb1 = launch_cg_cols(h, c1, v.ld);
+        reduce_ld(h, h->d_partial, h->d_red_a, h->d_ops_sum, 2, nrb1, v.ld);
         cs.mode = 1; cs.d0 = h->d_red_a; cs.d1 = h->d_red_a + v.ld; cs.plain = nullptr;
         hipLaunchKernelGGL(k_cg_scalars, dim3(ceil_div(v.ld, 256)), dim3(256), 0, h->s, cs);
-        CgUpdateArgs u;
-        int grid_u = 0;
-        u.sh = make_shape_waves(std::max(h->m, 1), v.ld, v.ld, cg_update_rpw(h, v.ld), 1, CG_UPDATE_WAVES, &grid_u);
-        u.w1 = h->d_wp; u.r1 = h->d_rp_; u.p1 = h->d_pp; u.s1 = h->d_s1;
-        u.w2 = h->d_wd; u.r2 = h->d_rd; u.p2 = h->d_pd; u.s2 = h->d_s2; u.alpha = h->d_alpha;
-        u.invdiag = h->d_invdiag; u.dr = (pinf || LPE_CG_PRIMAL_WEIGHTED) ? h->d_dr : nullptr; u.sel = v.sel;
-        u.act = h->d_cgact; u.partial = h->d_partial; u.inf_norm = pinf ? 1 : 0;
-        hipLaunchKernelGGL(k_cg_update, dim3(grid_u), dim3(CG_UPDATE_WAVES * WAVE), 0, h->s, u);
-        reduce_ld(h, h->d_partial, h->d_red_a, pinf ? h->d_ops_pinf : h->d_ops_sum, 3, u.sh.n_rb, v.ld);
+        const int nrb2 = launch_cg_rows(h, c2, v.ld);
+        reduce_ld(h, h->d_partial, h->d_red_a, pinf ? h->d_ops_pinf : h->d_ops_sum, 3, nrb2, v.ld);
         cs.mode = 2; cs.d0 = h->d_red_a; cs.d1 = h->d_red_a + v.ld; cs.plain = h->d_red_a + 2 * (long)v.ld;
         hipLaunchKernelGGL(k_cg_scalars, dim3(ceil_div(v.ld, 256)), dim3(256), 0, h->s, cs);
-        CgDirArgs dd;
-        int grid_d = 0;
-        dd.sh = make_shape(std::max(h->m, 1), v.ld, v.ld, 1, 1, &grid_d);
-        dd.p1 = h->d_pp; dd.r1 = h->d_rp_;
-        dd.p2 = h->d_pd; dd.r2 = h->d_rd; dd.beta = h->d_beta; dd.invdiag = h->d_invdiag; dd.sel = v.sel;
-        dd.act = h->d_cgact;
-        dd.reverse = dir_reverse();
         hipLaunchKernelGGL(k_cg_direction, dim3(grid_d), dim3(BLOCK), 0, h->s, dd);
     }
 }
@@ -919,16 +965,17 @@ void cg_steps(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, in
 // from fixed buffers, so the graph is reused across attempts).
 int cg_block(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, int grid_r, int grid_c) {
     const int steps = cg_block_steps();
-    const long long key = (long long)v.ld | ((long long)sr.rpw << 32) | ((long long)polish_lpl(v.ld) << 40) |
+    const long long key = (long long)v.ld | ((long long)sr.rpw << 32) | ((long long)cg_lpl(v.ld) << 40) |
                           ((long long)steps << 44) | (v.sel == h->d_sel ? 0LL : 1LL << 52) |
-                          ((long long)cg_update_rpw(h, v.ld) << 54) | ((long long)rows_reverse() << 60) |
-                          ((long long)dir_reverse() << 61) | ((long long)polish_split() << 62) |
+                          ((long long)(cg_gather_depth() == 8 ? 1 : 0) << 53) |
+                          ((long long)(cg_waves() == 16 ? 1 : 0) << 54) | ((long long)(cg_rpw(v.ld) - 1) << 55) |
+                          ((long long)rows_reverse() << 60) | ((long long)dir_reverse() << 61) |
                           ((long long)(cg_primal_inf() ? 1 : 0) << 51);
     hipGraphExec_t& g_exec = h->cg_graphs[key];
     if (!g_exec) {
         hipGraph_t g;
         HIPCHK(hipStreamBeginCapture(h->s, hipStreamCaptureModeThreadLocal));
-        cg_steps(h, v, sr, sc, grid_r, grid_c, steps);
+        cg_steps(h, v, sr, sc, steps);
         HIPCHK(hipStreamEndCapture(h->s, &g));
         hipError_t e = hipGraphInstantiate(&g_exec, g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
@@ -965,7 +1012,7 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
         PolishRowsInitArgs<T> a;
         a.sh = sr; a.A = op_orig_v(h, false, v); a.As = op_scaled_v(h, false, v); a.xs = h->d_xs; a.y = yin;
         a.rhs = v.rhs; a.dr = h->d_dr; a.orig = orig; a.fmask = h->d_fmask; a.invdiag = h->d_invdiag;
-        a.sel = v.sel; a.rmask = h->d_rmask; a.rp = h->d_rp_; a.pp = h->d_pp; a.wp = h->d_wp; a.wd = h->d_wd;
+        a.sel = v.sel; a.rmask = h->d_rmask; a.rp = h->d_rp_; a.p = h->d_pcg; a.wp = h->d_wp; a.wd = h->d_wd;
         const bool pinf = cg_primal_inf();
         a.neq = h->neq; a.tol_y = 1e-9; a.tol_s = 1e-7; a.plain_orig = pinf ? 2 : LPE_CG_PRIMAL_WEIGHTED;
         a.partial = h->d_partial;
@@ -974,13 +1021,13 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
         reduce_ld(h, h->d_partial, h->d_red_b, pinf ? h->d_ops_pinit : h->d_ops_sum, 2, sr.n_rb, v.ld);
         // dual setup: q1 = D (c^ - A^T wd); r_d = R A q1, p_d = r_d / diag
         PolishColsArgs pc;
-        pc.sh = sc; pc.AT = op_scaled_v(h, true, v); pc.p1 = h->d_wd; pc.p2 = nullptr; pc.q1 = h->d_q1; pc.q2 = nullptr;
-        pc.c = v.c_s; pc.c_shared = h->shared_c; pc.fmask = h->d_fmask; pc.sel = v.sel; pc.act = nullptr;
+        pc.sh = sc; pc.AT = op_scaled_v(h, true, v); pc.p1 = h->d_wd; pc.q1 = h->d_q1;
+        pc.c = v.c_s; pc.c_shared = h->shared_c; pc.fmask = h->d_fmask; pc.sel = v.sel;
         launch_polish_cols(h, pc, v.ld);
         PolishRowsArgs pr;
-        pr.sh = sr; pr.A = op_scaled_v(h, false, v); pr.q1 = h->d_q1; pr.q2 = nullptr; pr.s1 = nullptr; pr.s2 = nullptr;
-        pr.p1 = nullptr; pr.p2 = nullptr; pr.setup_r = h->d_rd; pr.setup_p = h->d_pd; pr.invdiag = h->d_invdiag;
-        pr.rmask = h->d_rmask; pr.sel = v.sel; pr.act = nullptr; pr.partial = h->d_partial; pr.reverse = 0;
+        pr.sh = sr; pr.A = op_scaled_v(h, false, v); pr.q1 = h->d_q1;
+        pr.setup_r = h->d_rd; pr.setup_p = h->d_pcg; pr.invdiag = h->d_invdiag;
+        pr.rmask = h->d_rmask; pr.sel = v.sel; pr.partial = h->d_partial;
         const int nrbs = launch_polish_rows(h, pr, v.ld);
         // the rows kernel writes two partials per row block (the second is 0 in
         // setup mode): reduce both so the stride matches, rho_d is the first
@@ -1352,8 +1399,8 @@ int alloc_all(lpe_handle* h) {
     Shape sc = make_shape(h->n, h->ld, h->B, CHECK_RPW, 1, &g);
     const size_t pr = (size_t)sr.n_rb * 2 * CR_N, pc = (size_t)sc.n_rb * 2 * CC_N;
     h->partial_elems = std::max(pr, pc) * ld;
-    // polish rows kernels write 2 partials per row block; narrow views use 1 row per wave
-    // (the CG update runs one row per wave: the largest row-block count)
+    // polish rows kernels write up to 3 partials per row block (the CG rows
+    // pass); narrow views use 1 row per wave (the largest row-block count)
     for (int w = 64; w <= (int)ld; w *= 2) {
         Shape sp = make_shape(std::max(h->m, 1), w, w, 1, 1, &g);
         h->partial_elems = std::max(h->partial_elems, (size_t)sp.n_rb * 3 * w);
@@ -1361,6 +1408,15 @@ int alloc_all(lpe_handle* h) {
     {
         Shape sp = make_shape(std::max(h->m, 1), ld, ld, 1, 1, &g);
         h->partial_elems = std::max(h->partial_elems, (size_t)sp.n_rb * 3 * ld);
+    }
+    // the CG columns pass writes 2 partials (|q1|^2, |q2|^2) per column block
+    for (int w = 64; w <= (int)ld; w *= 2) {
+        Shape sp = make_shape(h->n, w, w, 1, 1, &g);
+        h->partial_elems = std::max(h->partial_elems, (size_t)sp.n_rb * 2 * w);
+    }
+    {
+        Shape sp = make_shape(h->n, ld, ld, 1, 1, &g);
+        h->partial_elems = std::max(h->partial_elems, (size_t)sp.n_rb * 2 * ld);
     }
     CHK(dalloc(h, &h->d_partial, h->partial_elems));
     CHK(dalloc(h, &h->d_red_a, (size_t)2 * CC_N * ld)); CHK(dalloc(h, &h->d_red_b, (size_t)2 * CC_N * ld));
@@ -1377,7 +1433,7 @@ int alloc_all(lpe_handle* h) {
     CHK(dalloc(h, &h->d_ops_xr, XR_N)); CHK(upload(h->d_ops_xr, ops_xr, XR_N));
     CHK(dalloc(h, &h->d_ops_xc, XC_N)); CHK(upload(h->d_ops_xc, ops_xc, XC_N));
     CHK(dalloc(h, &h->d_ops_sum, 3)); CHK(upload(h->d_ops_sum, ops_sum, 3));
-    const int ops_pinf[3] = {RSUM, RSUM, RMAX};      // CG update: two rho, the primal residual maximum
+    const int ops_pinf[3] = {RSUM, RSUM, RMAX};      // CG rows pass: two rho, the primal residual maximum
     const int ops_pinit[2] = {RSUM, RMAX};           // CG setup: rho_p, the primal residual maximum
     CHK(dalloc(h, &h->d_ops_pinf, 3)); CHK(upload(h->d_ops_pinf, ops_pinf, 3));
     CHK(dalloc(h, &h->d_ops_pinit, 2)); CHK(upload(h->d_ops_pinit, ops_pinit, 2));
@@ -1387,10 +1443,10 @@ int alloc_all(lpe_handle* h) {
     CHK(dalloc(h, &h->d_ops_ic, IC_N)); CHK(upload(h->d_ops_ic, ops_ic, IC_N));
     // polish and results
     CHK(dalloc(h, &h->d_xs, n * ld)); CHK(dalloc(h, &h->d_fmask, n * ld)); CHK(dalloc(h, &h->d_rmask, m * ld));
-    CHK(dalloc(h, &h->d_rp_, m * ld)); CHK(dalloc(h, &h->d_pp, m * ld)); CHK(dalloc(h, &h->d_wp, m * ld));
-    CHK(dalloc(h, &h->d_rd, m * ld)); CHK(dalloc(h, &h->d_pd, m * ld)); CHK(dalloc(h, &h->d_wd, m * ld));
-    CHK(dalloc(h, &h->d_q1, n * ld)); CHK(dalloc(h, &h->d_q2, n * ld));
-    CHK(dalloc(h, &h->d_s1, m * ld)); CHK(dalloc(h, &h->d_s2, m * ld));
+    CHK(dalloc(h, &h->d_rp_, m * ld)); CHK(dalloc(h, &h->d_wp, m * ld));
+    CHK(dalloc(h, &h->d_rd, m * ld)); CHK(dalloc(h, &h->d_wd, m * ld));
+    CHK(dalloc(h, &h->d_pcg, m * ld)); CHK(dalloc(h, &h->d_qcg, n * ld));
+    CHK(dalloc(h, &h->d_q1, n * ld));
     CHK(dalloc(h, &h->d_invdiag, m * ld));
     CHK(dalloc(h, &h->d_rho, 2 * ld)); CHK(dalloc(h, &h->d_rho0, 2 * ld)); CHK(dalloc(h, &h->d_cgact, 2 * ld));
     CHK(dalloc(h, &h->d_cgfloor, 2 * ld)); CHK(dalloc(h, &h->d_alpha, 2 * ld)); CHK(dalloc(h, &h->d_beta, 2 * ld));
@@ -1537,7 +1593,7 @@ int lpe_create(const lpe_desc* d, lpe_handle** out) {
 
     lpe_handle* h = new lpe_handle();
     h->m = d->m; h->n = d->n; h->neq = d->neq; h->B = d->batch; h->nnz = nnz; h->nvar = d->n_var;
-    h->ld = ((d->batch + 255) / 256) * 256 + LPE_LD_PAD;
+    h->ld = ((d->batch + 255) / 256) * 256 + ld_pad();
     // Halpern shares the restarted method's scaling, weights, checks and polish
     h->precision = d->precision; h->method = d->method == LPE_METHOD_HALPERN ? LPE_METHOD_RESTARTED : d->method;
     h->halpern = d->method == LPE_METHOD_HALPERN ? 1 : 0; h->eta = d->step_safety; h->theta = d->theta;
@@ -2089,7 +2145,7 @@ int lpe_spmv(lpe_handle* h, int32_t transpose, int32_t precision, const double* 
                                (int)lin, (int)B, (int)ld, (const double*)nullptr, 0, 0.0);
         hipLaunchKernelGGL((k_convert<double, float>), dim3(ceil_div(h->nnz, 256)), tb, 0, h->s,
                            (const double*)(transpose ? h->d_tval_o : h->d_val_o), vals, (long)h->nnz);
-        float* vvar = reinterpret_cast<float*>(h->d_q2);
+        float* vvar = reinterpret_cast<float*>(h->d_qcg);
         if (h->nvar > 0)
             hipLaunchKernelGGL((k_convert<double, float>), dim3(ceil_div((long)h->nvar * ld, 256)), tb, 0, h->s,
                                (const double*)h->d_var_o, vvar, (long)h->nvar * ld);
@@ -2207,46 +2263,31 @@ int lpe_time_iterations(lpe_handle* h, int64_t iterations, int32_t chunk, int32_
 int lpe_time_kernel(lpe_handle* h, int32_t kind, int32_t reps, double* ms_out, void* stream) {
     if (!h || !ms_out) return fail(LPE_EINVAL, "handle and ms are required");
     if (kind < 0 || kind > 5)
-        return fail(LPE_EINVAL, "kind must be 0 (dual step), 1 (primal step), 2 (CG update), 3 (polish columns "
-                                "pass), 4 (polish rows pass) or 5 (CG direction)");
+        return fail(LPE_EINVAL, "kind must be 0 (dual step), 1 (primal step), 3 (CG columns pass), 4 (CG rows "
+                                "pass with the update) or 5 (CG direction)");
+    if (kind == 2) return fail(LPE_EINVAL, "the CG update runs inside the CG rows pass (kind 4)");
     if (reps < 1) return fail(LPE_EINVAL, "reps must be positive");
     if (!h->ready) return fail(LPE_ESTATE, "reset is required after a problem update");
     if (kind != 1 && h->m == 0) return fail(LPE_EINVAL, "no rows");
     CHK(enter(h, stream));
-    CgUpdateArgs u;
-    PolishColsArgs c1;
-    PolishRowsArgs c2;
+    CgColsArgs c1;
+    CgRowsArgs c2;
     CgDirArgs dd;
-    int grid_r = 0, grid_c = 0, grid_u = 0, grid_d = 0;
+    int grid_r = 0, grid_c = 0, grid_d = 0;
     const View v = full_view(h);
-    if (kind >= 2) {
+    if (kind >= 3) {
         // every LP selected, both CG systems active, on the full view
         hipLaunchKernelGGL(k_activate_all, dim3(ceil_div(h->ld, 256)), dim3(256), 0, h->s, h->d_sel, h->d_cgact,
                            h->B, h->ld);
         const int rpw = polish_rpw_h(h, v.ld);
-        c1.sh = make_shape(h->n, v.ld, v.ld, rpw, 1, &grid_c);
-        c1.AT = op_scaled_v(h, true, v); c1.p1 = h->d_pp; c1.p2 = h->d_pd; c1.q1 = h->d_q1; c1.q2 = h->d_q2;
-        c1.c = nullptr; c1.c_shared = 0; c1.fmask = h->d_fmask; c1.sel = v.sel; c1.act = h->d_cgact;
-        c2.sh = make_shape(h->m, v.ld, v.ld, rpw, 1, &grid_r);
-        c2.A = op_scaled_v(h, false, v); c2.q1 = h->d_q1; c2.q2 = h->d_q2; c2.s1 = h->d_s1; c2.s2 = h->d_s2;
-        c2.p1 = h->d_pp; c2.p2 = h->d_pd; c2.setup_r = nullptr; c2.setup_p = nullptr; c2.invdiag = h->d_invdiag;
-        c2.rmask = h->d_rmask; c2.sel = v.sel; c2.act = h->d_cgact; c2.partial = h->d_partial; c2.reverse = rows_reverse();
-        u.sh = make_shape_waves(h->m, v.ld, v.ld, cg_update_rpw(h, v.ld), 1, CG_UPDATE_WAVES, &grid_u);
-        u.w1 = h->d_wp; u.r1 = h->d_rp_; u.p1 = h->d_pp; u.s1 = h->d_s1;
-        u.w2 = h->d_wd; u.r2 = h->d_rd; u.p2 = h->d_pd; u.s2 = h->d_s2; u.alpha = h->d_alpha;
-        u.invdiag = h->d_invdiag; u.sel = v.sel;
-        u.inf_norm = cg_primal_inf() ? 1 : 0;
-        u.dr = (u.inf_norm || LPE_CG_PRIMAL_WEIGHTED) ? h->d_dr : nullptr;
-        u.act = h->d_cgact; u.partial = h->d_partial;
-        dd.sh = make_shape(h->m, v.ld, v.ld, 1, 1, &grid_d);
-        dd.p1 = h->d_pp; dd.r1 = h->d_rp_; dd.p2 = h->d_pd; dd.r2 = h->d_rd; dd.beta = h->d_beta;
-        dd.invdiag = h->d_invdiag; dd.sel = v.sel; dd.act = h->d_cgact; dd.reverse = dir_reverse();
+        const Shape sc = make_shape(h->n, v.ld, v.ld, rpw, 1, &grid_c);
+        const Shape sr = make_shape(h->m, v.ld, v.ld, rpw, 1, &grid_r);
+        cg_args(h, v, sr, sc, c1, c2, dd, grid_d);
     }
     HIPCHK(hipEventRecord(h->ev_a, h->s));
     for (int i = 0; i < reps; ++i) {
-        if (kind == 2) hipLaunchKernelGGL(k_cg_update, dim3(grid_u), dim3(CG_UPDATE_WAVES * WAVE), 0, h->s, u);
-        else if (kind == 3) launch_polish_cols(h, c1, v.ld);
-        else if (kind == 4) launch_polish_rows(h, c2, v.ld);
+        if (kind == 3) launch_cg_cols(h, c1, v.ld);
+        else if (kind == 4) launch_cg_rows(h, c2, v.ld);
         else if (kind == 5) hipLaunchKernelGGL(k_cg_direction, dim3(grid_d), dim3(BLOCK), 0, h->s, dd);
         else launch_step(h, kind == 0 ? 1 : 2);
     }

@@ -81,7 +81,7 @@ struct Shape {
 
 // Gather-accumulate one sparse row for LPL consecutive LPs.
 // acc[t] = sum_k a_k * x[idx_k][b0 + t], sequential in storage order.
-template <typename TV, typename TX, typename TA, int LPL>
+template <typename TV, typename TX, typename TA, int LPL, int D = 8>
 __device__ __forceinline__ void row_dot(const int* __restrict__ ptr, const int* __restrict__ idx,
                                         const int* __restrict__ slot, const TV* __restrict__ val,
                                         const TV* __restrict__ var, const TX* __restrict__ x,
@@ -89,11 +89,12 @@ __device__ __forceinline__ void row_dot(const int* __restrict__ ptr, const int* 
 #pragma unroll
     for (int t = 0; t < LPL; ++t) acc[t] = TA(0);
     const int s = ptr[row], e = ptr[row + 1];
-    // Chunks of GATHER_DEPTH entries: all (index, slot, value) scalars of the
-    // chunk are loaded first, then all gathers are issued, then the products
-    // are accumulated in storage order - the gathers of one chunk are in
-    // flight together instead of one dependent load at a time.
-    constexpr int D = 8;
+    // Chunks of D entries: all (index, slot, value) scalars of the chunk are
+    // loaded first, then all gathers are issued, then the products are
+    // accumulated in storage order - the gathers of one chunk are in flight
+    // together instead of one dependent load at a time.  The depth trades
+    // loads in flight per wave against registers (waves per SIMD); it does
+    // not change the result.
     for (int k0 = s; k0 < e; k0 += D) {
         const int cnt = e - k0 < D ? e - k0 : D;
         int col[D], sl[D];
@@ -121,6 +122,52 @@ __device__ __forceinline__ void row_dot(const int* __restrict__ ptr, const int* 
                 } else {
 #pragma unroll
                     for (int t = 0; t < LPL; ++t) acc[t] = acc[t] + (TA)vv[u][t] * (TA)xv[u][t];
+                }
+            }
+        }
+    }
+}
+
+// Gather-accumulate one sparse row for the two polish CG systems of LPL
+// consecutive LPs whose vectors are interleaved, x[k][b] = (x1, x2): one
+// 16-byte load per LP and entry serves both systems.  acc1 and acc2 are
+// summed in storage order exactly as row_dot sums each system alone.
+template <int LPL, int D>
+__device__ __forceinline__ void row_dot2(const int* __restrict__ ptr, const int* __restrict__ idx,
+                                         const int* __restrict__ slot, const double* __restrict__ val,
+                                         const double* __restrict__ var, const double2* __restrict__ x, long ld,
+                                         int b0, int row, double (&acc1)[LPL], double (&acc2)[LPL]) {
+#pragma unroll
+    for (int t = 0; t < LPL; ++t) { acc1[t] = 0.0; acc2[t] = 0.0; }
+    const int s = ptr[row], e = ptr[row + 1];
+    for (int k0 = s; k0 < e; k0 += D) {
+        const int cnt = e - k0 < D ? e - k0 : D;
+        int col[D], sl[D];
+        double a[D];
+#pragma unroll
+        for (int u = 0; u < D; ++u) {
+            if (u < cnt) { col[u] = idx[k0 + u]; sl[u] = slot[k0 + u]; a[u] = val[k0 + u]; }
+            else { col[u] = 0; sl[u] = -1; a[u] = 0.0; }
+        }
+        double2 xv[D][LPL];
+#pragma unroll
+        for (int u = 0; u < D; ++u)
+            if (u < cnt) {
+#pragma unroll
+                for (int t = 0; t < LPL; ++t) xv[u][t] = x[(long)col[u] * ld + b0 + t];
+            }
+        double vv[D][LPL];
+#pragma unroll
+        for (int u = 0; u < D; ++u)
+            if (u < cnt && sl[u] >= 0) load_vec<double, LPL>(var + (long)sl[u] * ld + b0, vv[u]);
+#pragma unroll
+        for (int u = 0; u < D; ++u) {
+            if (u < cnt) {
+#pragma unroll
+                for (int t = 0; t < LPL; ++t) {
+                    const double av = sl[u] < 0 ? a[u] : vv[u][t];
+                    acc1[t] = acc1[t] + av * xv[u][t].x;
+                    acc2[t] = acc2[t] + av * xv[u][t].y;
                 }
             }
         }
@@ -1231,7 +1278,7 @@ struct PolishRowsInitArgs {
     const int* sel;
     const uint8_t* fmask;
     uint8_t* rmask;
-    double* rp; double* pp; double* wp;
+    double* rp; double2* p; double* wp;   // p: the CG directions, interleaved (p1, p2); writes p1
     double* wd;
     double* invdiag;         // [m][ld] 1 / (R A^ D A^T R)_ii, 0 on inactive rows
     int neq;
@@ -1277,7 +1324,7 @@ __global__ void __launch_bounds__(BLOCK) k_polish_rows_init(PolishRowsInitArgs<T
             const double inv = (on && dg > 0.0) ? 1.0 / dg : 0.0;
             a.invdiag[i] = inv;
             const double rv = on ? a.dr[row] * res : 0.0;
-            a.rp[i] = rv; a.pp[i] = rv * inv; a.wp[i] = 0.0;
+            a.rp[i] = rv; a.p[i].x = rv * inv; a.wp[i] = 0.0;
             a.wd[i] = on ? yv : 0.0;
             v[0][0] += rv * rv * inv;
             const double ro = a.plain_orig ? (on ? res : 0.0) : rv;
@@ -1289,18 +1336,17 @@ __global__ void __launch_bounds__(BLOCK) k_polish_rows_init(PolishRowsInitArgs<T
     block_partials<2, 1>(v, ops, a.partial, rb, ld, b0, lds, valid);
 }
 
-// q_j = fmask * (base_j - sum_i a^_ij p_i), with base = c^ (dual setup) or 0 and
-// sign flipped (CG products: q = D A^T p).  Two systems per launch.
+// q_j = fmask * (c^_j - sum_i a^_ij p_i): the dual system's set-up product
+// (q = D (c^ - A^T u)), one system per launch.
 struct PolishColsArgs {
     Shape sh;
     Op<double> AT;           // scaled FP64
-    const double* p1; const double* p2;   // [m][ld]
-    double* q1; double* q2;               // [n][ld]
-    const double* c;         // scaled c^ for the dual setup, or nullptr
+    const double* p1;        // [m][ld]
+    double* q1;              // [n][ld]
+    const double* c;         // scaled c^
     int c_shared;
     const uint8_t* fmask;
     const int* sel;
-    const int* act;          // [2][ld] CG activity (nullptr: both active)
 };
 
 template <int LPL>
@@ -1311,68 +1357,42 @@ __global__ void __launch_bounds__(BLOCK) k_polish_cols(PolishColsArgs a) {
     const int b0 = tile * WAVE * LPL + lane * LPL;
     if (b0 >= a.sh.ld) return;
     const long ld = a.sh.ld;
-    bool sel[LPL], on1[LPL], on2[LPL];
-    bool any1 = false, any2 = false, anys = false;
+    bool sel[LPL];
+    bool anys = false;
 #pragma unroll
     for (int t = 0; t < LPL; ++t) {
         sel[t] = a.sel[b0 + t] != 0;
-        on1[t] = sel[t] && (!a.act || a.act[b0 + t]);
-        on2[t] = sel[t] && (!a.act || a.act[ld + b0 + t]);
-        any1 = any1 || on1[t]; any2 = any2 || on2[t]; anys = anys || sel[t];
+        anys = anys || sel[t];
     }
-    if (a.c ? !anys : (!(a.p1 && any1) && !(a.p2 && any2))) return;
+    if (!anys) return;
     const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
     for (int r = 0; r < a.sh.rpw; ++r) {
         const int col = uniform(r0 + r);
         if (col >= a.sh.rows) break;
         const long i = col * ld + b0;
-        double f[LPL];
+        double s1[LPL];
+        row_dot<double, double, double, LPL>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.p1, ld, b0, col, s1);
 #pragma unroll
-        for (int t = 0; t < LPL; ++t) f[t] = a.fmask[i + t] ? 1.0 : 0.0;
-        if (a.c) {
-            double s1[LPL];
-            row_dot<double, double, double, LPL>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.p1, ld, b0, col, s1);
-#pragma unroll
-            for (int t = 0; t < LPL; ++t) {
-                const double cc = a.c_shared ? a.c[col] : a.c[i + t];
-                if (sel[t]) a.q1[i + t] = f[t] * (cc - s1[t]);
-            }
-        } else {
-            if (a.p1 && any1) {
-                double s1[LPL];
-                row_dot<double, double, double, LPL>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.p1, ld, b0, col, s1);
-#pragma unroll
-                for (int t = 0; t < LPL; ++t)
-                    if (on1[t]) a.q1[i + t] = f[t] * s1[t];
-            }
-            if (a.p2 && any2) {
-                double s2[LPL];
-                row_dot<double, double, double, LPL>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.p2, ld, b0, col, s2);
-#pragma unroll
-                for (int t = 0; t < LPL; ++t)
-                    if (on2[t]) a.q2[i + t] = f[t] * s2[t];
-            }
+        for (int t = 0; t < LPL; ++t) {
+            const double cc = a.c_shared ? a.c[col] : a.c[i + t];
+            if (sel[t]) a.q1[i + t] = (a.fmask[i + t] ? 1.0 : 0.0) * (cc - s1[t]);
         }
     }
 }
 
-// s_i = rmask * sum_j a^_ij q_j for two systems; partial dots d1 = p1.s1,
-// d2 = p2.s2 (or |s1|^2 when p1 is nullptr, for the dual setup, which also
-// stores s1 into r1/pp1).
+// Dual set-up, rows: r_d = R A^ q, p_d = r_d * invdiag and the partial of
+// r_d.p_d (the second partial of each row block is 0, so the two-value
+// reduction of the primal set-up applies).
 struct PolishRowsArgs {
     Shape sh;
     Op<double> A;            // scaled FP64
-    const double* q1; const double* q2;
-    double* s1; double* s2;
-    const double* p1; const double* p2;
-    double* setup_r; double* setup_p;     // dual setup: r = s1, p = s1 * invdiag
+    const double* q1;
+    double* setup_r;
+    double2* setup_p;        // interleaved CG directions: writes p2
     const double* invdiag;
     const uint8_t* rmask;
     const int* sel;
-    const int* act;          // [2][ld] CG activity (nullptr: both active)
     double* partial;         // [n_rb][2][ld]
-    int reverse;             // walk the LP tiles last to first
-    int only = -1;           // one-system launch: the partial slot it writes (0 or 1)
 };
 
 template <int LPL>
@@ -1380,66 +1400,186 @@ __global__ void __launch_bounds__(BLOCK) k_polish_rows(PolishRowsArgs a) {
     __shared__ double lds[WAVES_PER_BLOCK * 2 * WAVE * LPL];
     int tile, rb;
     if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
-    if (a.reverse) tile = a.sh.n_tiles - 1 - tile;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int b0 = tile * WAVE * LPL + lane * LPL;
     const bool valid = b0 < a.sh.ld;
     const long ld = a.sh.ld;
     double v[2][LPL];
-    bool on1[LPL], on2[LPL], sel[LPL];
-    bool any1 = false, any2 = false, anys = false;
+    bool sel[LPL];
+    bool anys = false;
 #pragma unroll
     for (int t = 0; t < LPL; ++t) {
         v[0][t] = 0.0; v[1][t] = 0.0;
         sel[t] = valid && a.sel[b0 + t] != 0;
-        on1[t] = sel[t] && (!a.act || a.act[b0 + t]);
-        on2[t] = sel[t] && (!a.act || a.act[ld + b0 + t]);
-        any1 = any1 || on1[t]; any2 = any2 || on2[t]; anys = anys || sel[t];
+        anys = anys || sel[t];
     }
-    const bool setup = a.setup_r != nullptr;
-    if (setup ? anys : ((a.q1 && any1) || (a.q2 && any2))) {
+    if (anys) {
         const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
         for (int r = 0; r < a.sh.rpw; ++r) {
             const int row = uniform(r0 + r);
             if (row >= a.sh.rows) break;
             const long i = row * ld + b0;
-            double on[LPL];
+            double t1[LPL];
+            row_dot<double, double, double, LPL>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.q1, ld, b0, row, t1);
 #pragma unroll
-            for (int t = 0; t < LPL; ++t) on[t] = a.rmask[i + t] ? 1.0 : 0.0;
-            if (setup || (a.q1 && any1)) {
-                double t1[LPL];
-                row_dot<double, double, double, LPL>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.q1, ld, b0, row, t1);
-#pragma unroll
-                for (int t = 0; t < LPL; ++t) {
-                    const double sv1 = on[t] * t1[t];
-                    if (setup) {
-                        if (sel[t]) {
-                            const double z = sv1 * a.invdiag[i + t];
-                            a.setup_r[i + t] = sv1; a.setup_p[i + t] = z;
-                            v[0][t] += sv1 * z;
-                        }
-                    } else if (on1[t]) {
-                        a.s1[i + t] = sv1;
-                        v[0][t] += a.p1[i + t] * sv1;
-                    }
-                }
-            }
-            if (!setup && a.q2 && any2) {
-                double t2[LPL];
-                row_dot<double, double, double, LPL>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.q2, ld, b0, row, t2);
-#pragma unroll
-                for (int t = 0; t < LPL; ++t) {
-                    if (on2[t]) {
-                        const double sv2 = on[t] * t2[t];
-                        a.s2[i + t] = sv2;
-                        v[1][t] += a.p2[i + t] * sv2;
-                    }
+            for (int t = 0; t < LPL; ++t) {
+                if (sel[t]) {
+                    const double sv1 = (a.rmask[i + t] ? 1.0 : 0.0) * t1[t];
+                    const double z = sv1 * a.invdiag[i + t];
+                    a.setup_r[i + t] = sv1; a.setup_p[i + t].y = z;
+                    v[0][t] += sv1 * z;
                 }
             }
         }
     }
     const int ops[2] = {RSUM, RSUM};
-    block_partials<2, LPL>(v, ops, a.partial, rb, ld, b0, lds, valid, a.only);
+    block_partials<2, LPL>(v, ops, a.partial, rb, ld, b0, lds, valid);
+}
+
+// ---------------------------------------------------------------------------
+// One Jacobi-preconditioned CG step for both systems (0: primal correction,
+// 1: dual least squares) is three passes over the batch:
+//   columns  q = D A^T p, with the partials of |q|^2 - p is zero off the
+//            active rows and D is a 0/1 mask, so p.Mp = p.R A^ D A^T R p =
+//            |q|^2 and alpha = rho / |q|^2 is known before the rows pass;
+//   rows     s = R A^ q, applied at once: w += alpha p, r -= alpha s (s is
+//            never stored), with the partials of r.D^-1 r and of the primal
+//            stop measure;
+//   direction p = D^-1 r + beta p.
+// ---------------------------------------------------------------------------
+struct CgColsArgs {
+    Shape sh;
+    Op<double> AT;           // scaled FP64
+    const double2* p;        // [m][ld] interleaved (p1, p2)
+    double2* q;              // [n][ld] interleaved (q1, q2)
+    const uint8_t* fmask;
+    const int* sel;
+    const int* act;          // [2][ld] CG activity
+    double* partial;         // [n_rb][2][ld]: |q1|^2, |q2|^2
+};
+
+template <int LPL, int D, int WPB>
+__global__ void __launch_bounds__(WPB * WAVE) k_cg_cols(CgColsArgs a) {
+    __shared__ double lds[WPB * 2 * WAVE * LPL];
+    int tile, rb;
+    if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int b0 = tile * WAVE * LPL + lane * LPL;
+    const bool valid = b0 < a.sh.ld;
+    const long ld = a.sh.ld;
+    double v[2][LPL];
+    bool on1[LPL], on2[LPL];
+    bool any = false;
+#pragma unroll
+    for (int t = 0; t < LPL; ++t) {
+        v[0][t] = 0.0; v[1][t] = 0.0;
+        const bool sel = valid && a.sel[b0 + t] != 0;
+        on1[t] = sel && a.act[b0 + t];
+        on2[t] = sel && a.act[ld + b0 + t];
+        any = any || on1[t] || on2[t];
+    }
+    if (any) {
+        const int r0 = (rb * WPB + wave) * a.sh.rpw;
+        for (int r = 0; r < a.sh.rpw; ++r) {
+            const int col = uniform(r0 + r);
+            if (col >= a.sh.rows) break;
+            const long i = col * ld + b0;
+            double s1[LPL], s2[LPL];
+            row_dot2<LPL, D>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.p, ld, b0, col, s1, s2);
+#pragma unroll
+            for (int t = 0; t < LPL; ++t) {
+                if (!(on1[t] || on2[t])) continue;
+                // both components are written; the rows pass reads only those
+                // of active systems
+                const double f = a.fmask[i + t] ? 1.0 : 0.0;
+                const double2 q = make_double2(f * s1[t], f * s2[t]);
+                if (on1[t]) v[0][t] += q.x * q.x;
+                if (on2[t]) v[1][t] += q.y * q.y;
+                a.q[i + t] = q;
+            }
+        }
+    }
+    const int ops[2] = {RSUM, RSUM};
+    block_partials<2, LPL, WPB>(v, ops, a.partial, rb, ld, b0, lds, valid);
+}
+
+struct CgRowsArgs {
+    Shape sh;
+    Op<double> A;            // scaled FP64
+    const double2* q;        // [n][ld] interleaved (q1, q2)
+    const double2* p;        // [m][ld] interleaved (p1, p2)
+    double* w1; double* r1;  // [m][ld], primal system
+    double* w2; double* r2;  // dual system
+    const double* alpha;     // [2][ld]
+    const double* invdiag;   // Jacobi preconditioner, 0 off the active rows
+    const uint8_t* rmask;
+    const double* dr;        // row scale: the primal stop measure in original units (r / dr), or nullptr (scaled)
+    const int* sel;
+    const int* act;          // [2][ld]
+    double* partial;         // [n_rb][3][ld]: r_p.D^-1 r_p, r_d.D^-1 r_d, primal stop measure
+    int inf_norm;            // 1: stop measure max_i (r_i / dr_i)^2, else sum_i (r_i [/ dr_i])^2
+    int reverse;             // walk the LP tiles last to first
+};
+
+template <int LPL, int D, int WPB>
+__global__ void __launch_bounds__(WPB * WAVE) k_cg_rows(CgRowsArgs a) {
+    __shared__ double lds[WPB * 3 * WAVE * LPL];
+    int tile, rb;
+    if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
+    if (a.reverse) tile = a.sh.n_tiles - 1 - tile;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int b0 = tile * WAVE * LPL + lane * LPL;
+    const bool valid = b0 < a.sh.ld;
+    const long ld = a.sh.ld;
+    double v[3][LPL];
+    double al1[LPL], al2[LPL];
+    bool on1[LPL], on2[LPL];
+    bool any = false;
+#pragma unroll
+    for (int t = 0; t < LPL; ++t) {
+        v[0][t] = 0.0; v[1][t] = 0.0; v[2][t] = 0.0;
+        const bool sel = valid && a.sel[b0 + t] != 0;
+        on1[t] = sel && a.act[b0 + t];
+        on2[t] = sel && a.act[ld + b0 + t];
+        any = any || on1[t] || on2[t];
+        al1[t] = on1[t] ? a.alpha[b0 + t] : 0.0;
+        al2[t] = on2[t] ? a.alpha[ld + b0 + t] : 0.0;
+    }
+    if (any) {
+        const int r0 = (rb * WPB + wave) * a.sh.rpw;
+        for (int r = 0; r < a.sh.rpw; ++r) {
+            const int row = uniform(r0 + r);
+            if (row >= a.sh.rows) break;
+            const long i = row * ld + b0;
+            const double drow = a.dr ? a.dr[row] : 1.0;
+            double t1[LPL], t2[LPL];
+            row_dot2<LPL, D>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.q, ld, b0, row, t1, t2);
+#pragma unroll
+            for (int t = 0; t < LPL; ++t) {
+                if (!(on1[t] || on2[t])) continue;
+                const double on = a.rmask[i + t] ? 1.0 : 0.0;
+                const double iv = a.invdiag[i + t];
+                const double2 pv = a.p[i + t];
+                if (on1[t]) {
+                    a.w1[i + t] = a.w1[i + t] + al1[t] * pv.x;
+                    const double n1 = a.r1[i + t] - al1[t] * (on * t1[t]);
+                    a.r1[i + t] = n1;
+                    v[0][t] += n1 * n1 * iv;
+                    const double u = a.dr ? n1 / drow : n1;
+                    if (a.inf_norm) v[2][t] = fmax(v[2][t], u * u);
+                    else v[2][t] += u * u;
+                }
+                if (on2[t]) {
+                    a.w2[i + t] = a.w2[i + t] + al2[t] * pv.y;
+                    const double n2 = a.r2[i + t] - al2[t] * (on * t2[t]);
+                    a.r2[i + t] = n2;
+                    v[1][t] += n2 * n2 * iv;
+                }
+            }
+        }
+    }
+    const int ops[3] = {RSUM, RSUM, a.inf_norm ? RMAX : RSUM};
+    block_partials<3, LPL, WPB>(v, ops, a.partial, rb, ld, b0, lds, valid);
 }
 
 // Per-LP CG scalars (Jacobi-preconditioned CG, rho = r . D^-1 r), two systems
@@ -1494,67 +1634,11 @@ __global__ void k_cg_scalars(CgScalarArgs a) {
     }
 }
 
-// w += alpha p; r -= alpha s (two systems); partial |r|^2.
-struct CgUpdateArgs {
-    Shape sh;
-    double* w1; double* r1; const double* p1; const double* s1;
-    double* w2; double* r2; const double* p2; const double* s2;
-    const double* alpha;    // [2][ld]
-    const double* invdiag;  // Jacobi preconditioner
-    const double* dr;       // row scale: the primal stop measure in original units (r / dr), or nullptr (scaled)
-    const int* sel;
-    const int* act;         // [2][ld]
-    double* partial;        // [n_rb][3][ld]: r_p.D^-1 r_p, r_d.D^-1 r_d, primal stop measure
-    int inf_norm;           // 1: stop measure max_i (r_i / dr_i)^2, else sum_i (r_i [/ dr_i])^2
-};
-
-// 16 waves per block, one row per wave: each block covers 16 rows, so the
-// block partials of the three dots are a quarter of a 4-wave block's.
-constexpr int CG_UPDATE_WAVES = 16;
-
-__global__ void __launch_bounds__(CG_UPDATE_WAVES * WAVE) k_cg_update(CgUpdateArgs a) {
-    __shared__ double lds[CG_UPDATE_WAVES * 3 * WAVE];
-    int tile, rb;
-    if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
-    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    const int b0 = tile * WAVE + lane;
-    const bool valid = b0 < a.sh.ld;
-    const long ld = a.sh.ld;
-    double v[3][1] = {{0.0}, {0.0}, {0.0}};
-    const bool on1 = valid && a.act[b0], on2 = valid && a.act[ld + b0];
-    if (valid && a.sel[b0] && (on1 || on2)) {
-        const double al1 = a.alpha[b0], al2 = a.alpha[ld + b0];
-        const int r0 = (rb * CG_UPDATE_WAVES + wave) * a.sh.rpw;
-        for (int r = 0; r < a.sh.rpw; ++r) {
-            const int row = r0 + r;
-            if (row >= a.sh.rows) break;
-            const long i = row * ld + b0;
-            const double iv = a.invdiag[i];
-            if (on1) {
-                a.w1[i] = a.w1[i] + al1 * a.p1[i];
-                const double n1 = a.r1[i] - al1 * a.s1[i];
-                a.r1[i] = n1;
-                v[0][0] += n1 * n1 * iv;
-                const double u = a.dr ? n1 / a.dr[row] : n1;
-                if (a.inf_norm) v[2][0] = fmax(v[2][0], u * u);
-                else v[2][0] += u * u;
-            }
-            if (on2) {
-                a.w2[i] = a.w2[i] + al2 * a.p2[i];
-                const double n2 = a.r2[i] - al2 * a.s2[i];
-                a.r2[i] = n2;
-                v[1][0] += n2 * n2 * iv;
-            }
-        }
-    }
-    const int ops[3] = {RSUM, RSUM, a.inf_norm ? RMAX : RSUM};
-    block_partials<3, 1, CG_UPDATE_WAVES>(v, ops, a.partial, rb, ld, b0, lds, valid);
-}
-
 // p = r + beta p (two systems).
 struct CgDirArgs {
-    Shape sh;               // rows = m; one row per wave, 64 LPs per wave (the CG update's layout)
-    double* p1; const double* r1; double* p2; const double* r2;
+    Shape sh;               // rows = m; one row per wave, 64 LPs per wave
+    double2* p;             // interleaved (p1, p2)
+    const double* r1; const double* r2;
     const double* beta;
     const double* invdiag;
     const int* sel;
@@ -1562,8 +1646,8 @@ struct CgDirArgs {
     int reverse;            // walk the LP tiles last to first
 };
 
-// Launched with the CG update's block -> (LP tile, row block) mapping, so each
-// wave streams whole 512-byte row segments of the five vectors it touches.
+// One row per wave and 64 LPs per wave, so each wave streams whole row
+// segments (1 KB of p, 512 bytes of the others).
 __global__ void __launch_bounds__(BLOCK) k_cg_direction(CgDirArgs a) {
     int tile, rb;
     if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
@@ -1581,8 +1665,10 @@ __global__ void __launch_bounds__(BLOCK) k_cg_direction(CgDirArgs a) {
         if (row >= a.sh.rows) break;
         const long i = row * ld + b;
         const double iv = a.invdiag[i];
-        if (on1) a.p1[i] = a.r1[i] * iv + be1 * a.p1[i];
-        if (on2) a.p2[i] = a.r2[i] * iv + be2 * a.p2[i];
+        double2 pv = a.p[i];
+        if (on1) pv.x = a.r1[i] * iv + be1 * pv.x;
+        if (on2) pv.y = a.r2[i] * iv + be2 * pv.y;
+        a.p[i] = pv;
     }
 }
 

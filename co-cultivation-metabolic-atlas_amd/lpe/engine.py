@@ -330,9 +330,10 @@ class BatchedLpEngine:
 
     def time_kernel(self, kind: str, reps: int = 20) -> float:
         """Average ms of one launch of the 'dual' or 'primal' step kernel, or
-        of the polish's 'cg_update', 'polish_cols', 'polish_rows' or
-        'cg_direction' pass with every LP active."""
-        kinds = {"dual": 0, "primal": 1, "cg_update": 2, "polish_cols": 3, "polish_rows": 4, "cg_direction": 5}
+        of the polish's CG passes with every LP active: 'cg_cols' (q = D A^T p
+        and |q|^2), 'cg_rows' (s = R A q with the CG update) or
+        'cg_direction'."""
+        kinds = {"dual": 0, "primal": 1, "cg_cols": 3, "cg_rows": 4, "cg_direction": 5}
         ms = C.c_double(0.0)
         with torch.cuda.device(self.device):
             check(self._lib.lpe_time_kernel(self._h, kinds[kind], int(reps), C.byref(ms), _stream()))

@@ -272,11 +272,12 @@ int lpe_time_iterations(lpe_handle *h, int64_t iterations, int32_t chunk, int32_
 /* Average duration (ms) of `reps` back-to-back launches of one kernel on
  * the engine's stream, with hipEvents: kind 0 = dual step (the fused A x
  * SpMV + dual update), 1 = primal step (fused A^T y SpMV + primal update +
- * extrapolation + averaging), 2 = the polish's preconditioned-CG update,
- * 3 = the polish's columns pass q = D A^T p, 4 = its rows pass s = R A q with
- * the p.s partials, 5 = the CG direction p = D^-1 r + beta p (2-5: both CG
- * systems, every LP of the batch active; the
- * polish scratch vectors are overwritten).  The iterate is advanced; reset
+ * extrapolation + averaging), 3 = the polish's CG columns pass q = D A^T p
+ * with the |q|^2 partials, 4 = its CG rows pass s = R A q with the CG update
+ * (w += alpha p, r -= alpha s) and the residual partials, 5 = the CG direction
+ * p = D^-1 r + beta p (3-5: both CG systems, every LP of the batch active; the
+ * polish scratch vectors are overwritten); kind 2 (a separate CG update
+ * pass) no longer exists and is rejected.  The iterate is advanced; reset
  * before solving again.  No
  * reference counterpart (the reference times whole graph chunks,
  * scripts/benchmark_pdhg_graph_chunk.py:217-234). */
