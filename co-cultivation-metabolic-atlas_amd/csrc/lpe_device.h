@@ -1484,6 +1484,11 @@ __global__ void __launch_bounds__(WPB * WAVE) k_cg_cols(CgColsArgs a) {
             const int col = uniform(r0 + r);
             if (col >= a.sh.rows) break;
             const long i = col * ld + b0;
+            // the column's mask is loaded before the gathers so that the two
+            // memory round trips overlap
+            double f[LPL];
+#pragma unroll
+            for (int t = 0; t < LPL; ++t) f[t] = (on1[t] || on2[t]) && a.fmask[i + t] ? 1.0 : 0.0;
             double s1[LPL], s2[LPL];
             row_dot2<LPL, D>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.p, ld, b0, col, s1, s2);
 #pragma unroll
@@ -1491,8 +1496,7 @@ __global__ void __launch_bounds__(WPB * WAVE) k_cg_cols(CgColsArgs a) {
                 if (!(on1[t] || on2[t])) continue;
                 // both components are written; the rows pass reads only those
                 // of active systems
-                const double f = a.fmask[i + t] ? 1.0 : 0.0;
-                const double2 q = make_double2(f * s1[t], f * s2[t]);
+                const double2 q = make_double2(f[t] * s1[t], f[t] * s2[t]);
                 if (on1[t]) v[0][t] += q.x * q.x;
                 if (on2[t]) v[1][t] += q.y * q.y;
                 a.q[i + t] = q;
@@ -1552,28 +1556,40 @@ __global__ void __launch_bounds__(WPB * WAVE) k_cg_rows(CgRowsArgs a) {
             if (row >= a.sh.rows) break;
             const long i = row * ld + b0;
             const double drow = a.dr ? a.dr[row] : 1.0;
+            // the row's streamed operands are loaded before the gathers so
+            // that their memory round trip overlaps the gathers'
+            double on[LPL], iv[LPL], w1v[LPL], r1v[LPL], w2v[LPL], r2v[LPL];
+            double2 pv[LPL];
+#pragma unroll
+            for (int t = 0; t < LPL; ++t) {
+                on[t] = 0.0; iv[t] = 0.0; w1v[t] = 0.0; r1v[t] = 0.0; w2v[t] = 0.0; r2v[t] = 0.0;
+                pv[t] = make_double2(0.0, 0.0);
+                if (!(on1[t] || on2[t])) continue;
+                on[t] = a.rmask[i + t] ? 1.0 : 0.0;
+                iv[t] = a.invdiag[i + t];
+                pv[t] = a.p[i + t];
+                if (on1[t]) { w1v[t] = a.w1[i + t]; r1v[t] = a.r1[i + t]; }
+                if (on2[t]) { w2v[t] = a.w2[i + t]; r2v[t] = a.r2[i + t]; }
+            }
             double t1[LPL], t2[LPL];
             row_dot2<LPL, D>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.q, ld, b0, row, t1, t2);
 #pragma unroll
             for (int t = 0; t < LPL; ++t) {
                 if (!(on1[t] || on2[t])) continue;
-                const double on = a.rmask[i + t] ? 1.0 : 0.0;
-                const double iv = a.invdiag[i + t];
-                const double2 pv = a.p[i + t];
                 if (on1[t]) {
-                    a.w1[i + t] = a.w1[i + t] + al1[t] * pv.x;
-                    const double n1 = a.r1[i + t] - al1[t] * (on * t1[t]);
+                    a.w1[i + t] = w1v[t] + al1[t] * pv[t].x;
+                    const double n1 = r1v[t] - al1[t] * (on[t] * t1[t]);
                     a.r1[i + t] = n1;
-                    v[0][t] += n1 * n1 * iv;
+                    v[0][t] += n1 * n1 * iv[t];
                     const double u = a.dr ? n1 / drow : n1;
                     if (a.inf_norm) v[2][t] = fmax(v[2][t], u * u);
                     else v[2][t] += u * u;
                 }
                 if (on2[t]) {
-                    a.w2[i + t] = a.w2[i + t] + al2[t] * pv.y;
-                    const double n2 = a.r2[i + t] - al2[t] * (on * t2[t]);
+                    a.w2[i + t] = w2v[t] + al2[t] * pv[t].y;
+                    const double n2 = r2v[t] - al2[t] * (on[t] * t2[t]);
                     a.r2[i + t] = n2;
-                    v[1][t] += n2 * n2 * iv;
+                    v[1][t] += n2 * n2 * iv[t];
                 }
             }
         }

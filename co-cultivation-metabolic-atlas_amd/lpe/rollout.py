@@ -182,6 +182,14 @@ class ConsortiumRollout:
         if x0 is not None and not opts.first_check_interval:
             # a warm start is checked (and possibly polished) after half a chunk
             opts = dataclasses.replace(opts, first_check_interval=max(1, opts.check_interval // 2))
+        if x0 is not None and not opts.polish_kkt_start:
+            # ... and polished at once when it is already as close as an
+            # iterate the polish would be tried from: between steps the active
+            # set rarely changes, so one polish from the previous solution
+            # certifies where 32 PDHG iterations and up to two polishes were
+            # spent otherwise (4096 consortium LPs, steps 2-8: 277 instead of
+            # 419 ms)
+            opts = dataclasses.replace(opts, polish_kkt_start=opts.polish_kkt)
         if self.fallback and eng.precision == "fp32":
             info, x, y, status = eng.solve_with_fallback(opts)
         else:
