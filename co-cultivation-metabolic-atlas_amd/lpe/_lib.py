@@ -11,7 +11,9 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liblpe.so")
+# LPE_LIBRARY selects another build of the same library (the host-sanitizer
+# build of csrc/sanitize.sh); the in-tree build is the default.
+LIB_PATH = os.environ.get("LPE_LIBRARY") or os.path.join(_HERE, "liblpe.so")
 
 LPE_OK, LPE_EINVAL, LPE_EDEVICE, LPE_ESTATE = 0, -1, -2, -3
 PREC_FP64, PREC_FP32 = 0, 1
