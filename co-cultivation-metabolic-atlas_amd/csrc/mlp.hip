@@ -126,6 +126,70 @@ struct Epilogue {
     int nx, ny, neq, snap;
 };
 
+// Stores a wave's 64 x 64 accumulator tile (2 x 2 MFMA tiles of 32 x 32)
+// whose first row / column are r0 / c0, through the epilogue EPI.
+template <int EPI>
+__device__ __forceinline__ void store_tile(const float16_t (&acc)[2][2], int r0, int c0, int lane, int M, int N,
+                                           float* __restrict__ C, int ldc, int accumulate, const Epilogue& ep) {
+    // per-column parameters of the lane's two columns, loaded once
+    float bias[2] = {0.0f, 0.0f}, scale[2] = {0.0f, 0.0f}, shift[2] = {0.0f, 0.0f};
+    if constexpr (EPI != 0) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int col = c0 + j * 32 + lane % 32;
+            if (col < N) {
+                bias[j] = ep.bias[col];
+                if constexpr (EPI == 2) { scale[j] = ep.scale[col]; shift[j] = ep.shift[col]; }
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = r0 + i * 32 + (r / 4) * 8 + (lane / 32) * 4 + (r % 4);
+                const int col = c0 + j * 32 + lane % 32;
+                if constexpr (EPI == 0) {
+                    if (row < M && col < N) {
+                        const long o = (long)row * ldc + col;
+                        C[o] = accumulate ? C[o] + acc[i][j][r] : acc[i][j][r];
+                    }
+                } else if constexpr (EPI == 1) {
+                    if (row < M && col < N) ep.H[(long)row * ep.ldh + col] = f2bf(gelu(acc[i][j][r] + bias[j]));
+                } else {
+                    // lanes l and l ^ 1 hold columns col and col ^ 1 of the same row:
+                    // the even lane of a (x_j, logit_j) pair reads the logit from its
+                    // neighbour (every lane takes part in the exchange)
+                    const float v = col < N ? acc[i][j][r] + bias[j] : 0.0f;
+                    const float other = __shfl_xor(v, 1);
+                    if (row < M && col < N) {
+                        if (col < 2 * ep.nx) {
+                            if ((col & 1) == 0) {
+                                const int xj = col >> 1;
+                                const long o = (long)row * ep.nx + xj;
+                                const double lo = ep.lower[o], hi = ep.upper[o];
+                                double xv = (double)(v * scale[j] + shift[j]);
+                                xv = fmin(fmax(xv, lo), hi);
+                                if (ep.snap && other > 0.0f) {
+                                    const bool flo = isfinite(lo), fhi = isfinite(hi);
+                                    if (flo && (!fhi || xv - lo <= hi - xv)) xv = lo;
+                                    else if (fhi) xv = hi;
+                                }
+                                ep.x0[o] = xv;
+                            }
+                        } else if (col < 2 * ep.nx + ep.ny) {
+                            const int yi = col - 2 * ep.nx;
+                            double yv = (double)(v * scale[j] + shift[j]);
+                            if (yi >= ep.neq) yv = fmin(yv, 0.0);
+                            ep.y0[(long)row * ep.ny + yi] = yv;
+                        }
+                    }
+                }
+            }
+}
+
 template <int TA, int TB, int EPI = 0>
 __global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, const uint16_t* __restrict__ A, int lda,
                                               const uint16_t* __restrict__ B, int ldb, float* __restrict__ C,
@@ -174,51 +238,125 @@ __global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, const uint16_
         }
         __syncthreads();
     }
+    store_tile<EPI>(acc, m0 + wm, n0 + wn, lane, M, N, C, ldc, accumulate, ep);
+}
+
+// ---------------------------------------------------------------------------
+// GEMM with both operands k-contiguous (A [M][K], B [N][K], 16-byte aligned
+// rows): the form of every product the network runs (inference, the forward
+// pass and both backward products, whose operands are transposed first).
+// 128 x 128 block tile, four waves of 64 x 64, K advances by 64 through two
+// LDS buffers: the next K block is loaded from global memory into registers
+// while the MFMAs run on the current buffer, then stored into the other one,
+// so each K block costs one barrier (the 32-wide kernel above costs two per
+// 32).  16 MFMAs per wave per barrier.
+constexpr int KB = 32, KPAD = 8, KCH = KB / 8, KQ = BM * KCH / 256;
+
+// thread tid owns 16-byte chunks tid + 256 q (q < 4) of a 128 x 64 tile:
+// row c / 8, k offset (c % 8) * 8; entries past K are zeroed
+__device__ __forceinline__ void kk_load(uint4 (&v)[KQ], const uint16_t* __restrict__ X, int ldx, int r0, int R,
+                                        int k0, int K, int tid) {
+#pragma unroll
+    for (int q = 0; q < KQ; ++q) {
+        const int c = tid + q * 256;
+        const int gr = r0 + c / KCH, gk = k0 + (c % KCH) * 8;
+        v[q] = make_uint4(0, 0, 0, 0);
+        if (gr < R && gk < K) {
+            v[q] = *reinterpret_cast<const uint4*>(X + (long)gr * ldx + gk);
+            if (gk + 8 > K) {
+                uint16_t* e = reinterpret_cast<uint16_t*>(&v[q]);
+#pragma unroll
+                for (int t = 0; t < 8; ++t)
+                    if (gk + t >= K) e[t] = 0;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void kk_store(uint16_t (*S)[KB + KPAD], const uint4 (&v)[KQ], int tid) {
+#pragma unroll
+    for (int q = 0; q < KQ; ++q) {
+        const int c = tid + q * 256;
+        *reinterpret_cast<uint4*>(&S[c / KCH][(c % KCH) * 8]) = v[q];
+    }
+}
+
+// Split K (gridDim.z > 1, EPI 0 only): block z multiplies the k range
+// [z kchunk, (z + 1) kchunk) into its own slice C + z zstride.
+template <int EPI = 0>
+__global__ void __launch_bounds__(256) k_gemm_kk(int M, int N, int K, const uint16_t* __restrict__ A, int lda,
+                                                 const uint16_t* __restrict__ B, int ldb, float* __restrict__ C,
+                                                 int ldc, int accumulate, Epilogue ep, int kchunk = 0,
+                                                 long zstride = 0) {
+    __shared__ __align__(16) uint16_t As[2][BM][KB + KPAD];
+    __shared__ __align__(16) uint16_t Bs[2][BN][KB + KPAD];
+    if (gridDim.z > 1) {
+        const int kb = blockIdx.z * kchunk;
+        A += kb;
+        B += kb;
+        C += blockIdx.z * zstride;
+        K = min(K - kb, kchunk);
+    }
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int wm = (wave / 2) * 64, wn = (wave % 2) * 64;
+    float16_t acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int row = m0 + wm + i * 32 + (r / 4) * 8 + (lane / 32) * 4 + (r % 4);
-                const int col = n0 + wn + j * 32 + lane % 32;
-                if constexpr (EPI == 0) {
-                    if (row < M && col < N) {
-                        const long o = (long)row * ldc + col;
-                        C[o] = accumulate ? C[o] + acc[i][j][r] : acc[i][j][r];
-                    }
-                } else if constexpr (EPI == 1) {
-                    if (row < M && col < N) ep.H[(long)row * ep.ldh + col] = f2bf(gelu(acc[i][j][r] + ep.bias[col]));
-                } else {
-                    // lanes l and l ^ 1 hold columns col and col ^ 1 of the same row:
-                    // the even lane of a (x_j, logit_j) pair reads the logit from its
-                    // neighbour (every lane takes part in the exchange)
-                    const float v = col < N ? acc[i][j][r] + ep.bias[col] : 0.0f;
-                    const float other = __shfl_xor(v, 1);
-                    if (row < M && col < N) {
-                        if (col < 2 * ep.nx) {
-                            if ((col & 1) == 0) {
-                                const int xj = col >> 1;
-                                const long o = (long)row * ep.nx + xj;
-                                const double lo = ep.lower[o], hi = ep.upper[o];
-                                double xv = (double)(v * ep.scale[col] + ep.shift[col]);
-                                xv = fmin(fmax(xv, lo), hi);
-                                if (ep.snap && other > 0.0f) {
-                                    const bool flo = isfinite(lo), fhi = isfinite(hi);
-                                    if (flo && (!fhi || xv - lo <= hi - xv)) xv = lo;
-                                    else if (fhi) xv = hi;
-                                }
-                                ep.x0[o] = xv;
-                            }
-                        } else if (col < 2 * ep.nx + ep.ny) {
-                            const int yi = col - 2 * ep.nx;
-                            double yv = (double)(v * ep.scale[col] + ep.shift[col]);
-                            if (yi >= ep.neq) yv = fmin(yv, 0.0);
-                            ep.y0[(long)row * ep.ny + yi] = yv;
-                        }
-                    }
-                }
-            }
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+    const int nk = (K + KB - 1) / KB;
+    uint4 ra[KQ], rb[KQ];
+    kk_load(ra, A, lda, m0, M, 0, K, tid);
+    kk_load(rb, B, ldb, n0, N, 0, K, tid);
+    kk_store(As[0], ra, tid);
+    kk_store(Bs[0], rb, tid);
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        const bool more = kt + 1 < nk;
+        if (more) {
+            kk_load(ra, A, lda, m0, M, (kt + 1) * KB, K, tid);
+            kk_load(rb, B, ldb, n0, N, (kt + 1) * KB, K, tid);
+        }
+#pragma unroll
+        for (int ks = 0; ks < KB / 16; ++ks) {
+            bf16x8_t a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                a[i] = *reinterpret_cast<const bf16x8_t*>(&As[cur][wm + i * 32 + lane % 32][ks * 16 + (lane / 32) * 8]);
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                b[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[cur][wn + j * 32 + lane % 32][ks * 16 + (lane / 32) * 8]);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        if (more) {
+            // the other buffer was last read in iteration kt - 1, which every
+            // wave finished before the barrier that ended it
+            kk_store(As[cur ^ 1], ra, tid);
+            kk_store(Bs[cur ^ 1], rb, tid);
+        }
+        __syncthreads();
+    }
+    store_tile<EPI>(acc, m0 + wm, n0 + wn, lane, M, N, C, ldc, accumulate, ep);
+}
+
+// C[r][c] = (accumulate ? C[r][c] : 0) + sum over s = 0 .. S-1 of W[s][r][c], in that order.
+__global__ void k_splitk_reduce(const float* __restrict__ W, float* __restrict__ C, int M, int N, int ldc, int S,
+                                int accumulate) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long total = (long)M * N;
+    if (i >= total) return;
+    const long r = i / N, c = i - r * N;
+    float v = accumulate ? C[r * ldc + c] : 0.0f;
+    for (int s = 0; s < S; ++s) v += W[s * total + i];
+    C[r * ldc + c] = v;
 }
 
 __device__ __forceinline__ float gelu_grad(float z) {
@@ -356,11 +494,36 @@ int ws_gemm_bf16(int M, int N, int K, const uint16_t* A, int lda, int trans_a, c
     if (!trans_a && !trans_b)
         hipLaunchKernelGGL((k_gemm<0, 0>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate, va, vb, Epilogue{});
     else if (!trans_a && trans_b)
-        hipLaunchKernelGGL((k_gemm<0, 1>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate, va, vb, Epilogue{});
+        if (va && vb)
+            hipLaunchKernelGGL((k_gemm_kk<0>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate, Epilogue{}, 0, 0L);
+        else
+            hipLaunchKernelGGL((k_gemm<0, 1>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate, va, vb, Epilogue{});
     else if (trans_a && !trans_b)
         hipLaunchKernelGGL((k_gemm<1, 0>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate, va, vb, Epilogue{});
     else
         hipLaunchKernelGGL((k_gemm<1, 1>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate, va, vb, Epilogue{});
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int ws_gemm_bf16_splitk(int M, int N, int K, const uint16_t* A, int lda, const uint16_t* B, int ldb, float* C,
+                        int ldc, int accumulate, float* work, int splits, void* stream) {
+    if (M < 0 || N < 0 || K < 0 || !A || !B || !C || splits < 1 || (splits > 1 && !work) || ldc < N) return -1;
+    if (lda % 8 || ldb % 8 || ((uintptr_t)A & 15) || ((uintptr_t)B & 15)) return -1;
+    if (M == 0 || N == 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    // k chunks are whole K blocks, so every slice's operand rows stay 16-byte aligned
+    const int kchunk = ((K + splits - 1) / splits + KB - 1) / KB * KB;
+    const int S = K == 0 ? 1 : (K + kchunk - 1) / kchunk;
+    if (S <= 1) {
+        hipLaunchKernelGGL((k_gemm_kk<0>), dim3(blocks(N, BN), blocks(M, BM)), dim3(256), 0, s, M, N, K, A, lda, B,
+                           ldb, C, ldc, accumulate, Epilogue{}, 0, 0L);
+        return hipGetLastError() == hipSuccess ? 0 : -2;
+    }
+    const long zstride = (long)M * N;
+    hipLaunchKernelGGL((k_gemm_kk<0>), dim3(blocks(N, BN), blocks(M, BM), S), dim3(256), 0, s, M, N, K, A, lda, B,
+                       ldb, work, N, 0, Epilogue{}, kchunk, zstride);
+    hipLaunchKernelGGL(k_splitk_reduce, dim3(blocks(zstride, 256)), dim3(256), 0, s, (const float*)work, C, M, N,
+                       ldc, S, accumulate);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
@@ -373,8 +536,12 @@ int ws_gemm_bf16_act(int M, int N, int K, const uint16_t* A, int lda, const uint
     const int vb = (ldb % 8 == 0 && ((uintptr_t)B & 15) == 0) ? 1 : 0;
     Epilogue ep{};
     ep.bias = bias; ep.H = H; ep.ldh = ldh;
-    hipLaunchKernelGGL((k_gemm<0, 1, 1>), grid, dim3(256), 0, (hipStream_t)stream, M, N, K, A, lda, B, ldb,
-                       (float*)nullptr, 0, 0, va, vb, ep);
+    if (va && vb)
+        hipLaunchKernelGGL((k_gemm_kk<1>), grid, dim3(256), 0, (hipStream_t)stream, M, N, K, A, lda, B, ldb,
+                           (float*)nullptr, 0, 0, ep, 0, 0L);
+    else
+        hipLaunchKernelGGL((k_gemm<0, 1, 1>), grid, dim3(256), 0, (hipStream_t)stream, M, N, K, A, lda, B, ldb,
+                           (float*)nullptr, 0, 0, va, vb, ep);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
@@ -392,8 +559,12 @@ int ws_gemm_bf16_start(int M, int nx, int ny, int K, const uint16_t* A, int lda,
     Epilogue ep{};
     ep.bias = bias; ep.scale = scale; ep.shift = shift; ep.lower = lower; ep.upper = upper;
     ep.x0 = x0; ep.y0 = y0; ep.nx = nx; ep.ny = ny; ep.neq = neq; ep.snap = snap;
-    hipLaunchKernelGGL((k_gemm<0, 1, 2>), grid, dim3(256), 0, (hipStream_t)stream, M, N, K, A, lda, B, ldb,
-                       (float*)nullptr, 0, 0, va, vb, ep);
+    if (va && vb)
+        hipLaunchKernelGGL((k_gemm_kk<2>), grid, dim3(256), 0, (hipStream_t)stream, M, N, K, A, lda, B, ldb,
+                           (float*)nullptr, 0, 0, ep, 0, 0L);
+    else
+        hipLaunchKernelGGL((k_gemm<0, 1, 2>), grid, dim3(256), 0, (hipStream_t)stream, M, N, K, A, lda, B, ldb,
+                           (float*)nullptr, 0, 0, va, vb, ep);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -27,7 +27,7 @@ import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libwarmstart.so")
-EXPORTS = ("ws_gemm_bf16", "ws_bias_act", "ws_act_backward", "ws_loss", "ws_adam", "ws_transpose_bf16",
+EXPORTS = ("ws_gemm_bf16", "ws_gemm_bf16_splitk", "ws_bias_act", "ws_act_backward", "ws_loss", "ws_adam", "ws_transpose_bf16",
            "ws_to_bf16", "ws_gemm_bf16_act", "ws_gemm_bf16_start", "ws_standardize", "ws_version")
 
 _lib = None
@@ -43,6 +43,7 @@ def load():
     lib = C.CDLL(LIB_PATH)
     P, I, F = C.c_void_p, C.c_int, C.c_float
     lib.ws_gemm_bf16.argtypes = [I, I, I, P, I, I, P, I, I, P, I, I, P]
+    lib.ws_gemm_bf16_splitk.argtypes = [I, I, I, P, I, P, I, P, I, I, P, I, P]
     lib.ws_bias_act.argtypes = [P, P, P, I, I, I, P]
     lib.ws_act_backward.argtypes = [P, P, P, P, I, I, I, P]
     lib.ws_loss.argtypes = [P, P, P, P, I, I, I, I, F, F, F, P]
@@ -88,9 +89,41 @@ def gemm(a: torch.Tensor, b: torch.Tensor, *, trans_a: bool = False, trans_b: bo
         out = torch.empty((m, n), dtype=torch.float32, device=a.device)
     elif out.shape != (m, n) or out.dtype != torch.float32 or not out.is_contiguous():
         raise ValueError("out must be a contiguous float32 [M, N] tensor")
+    splits = _splits(m, n, k) if (not trans_a and trans_b) else 1
+    if splits > 1:
+        # few output tiles and a long k: the k range is split over more blocks
+        # (partial products summed in a fixed order)
+        work = _workspace(splits * m * n, a.device)
+        _ok(lib.ws_gemm_bf16_splitk(m, n, k, _ptr(a), a.shape[1], _ptr(b), b.shape[1], _ptr(out), n,
+                                    int(accumulate), _ptr(work), splits, _stream()), "ws_gemm_bf16_splitk")
+        return out
     _ok(lib.ws_gemm_bf16(m, n, k, _ptr(a), a.shape[1], int(trans_a), _ptr(b), b.shape[1], int(trans_b),
                          _ptr(out), n, int(accumulate), _stream()), "ws_gemm_bf16")
     return out
+
+
+TILE = 128          # output tile of the GEMM kernels
+TARGET_BLOCKS = 256  # one block per compute unit
+
+
+def _splits(m: int, n: int, k: int) -> int:
+    """K splits for a product whose output tiles alone cannot fill the chip."""
+    tiles = (-(-m // TILE)) * (-(-n // TILE))
+    if tiles >= TARGET_BLOCKS or k < 2048:
+        return 1
+    return int(max(1, min(8, TARGET_BLOCKS // tiles, k // 1024)))
+
+
+_work = {}
+
+
+def _workspace(numel: int, device) -> torch.Tensor:
+    """FP32 scratch of the split-K products, kept per device and grown on demand."""
+    w = _work.get(device)
+    if w is None or w.numel() < numel:
+        w = torch.empty(numel, dtype=torch.float32, device=device)
+        _work[device] = w
+    return w
 
 
 def transpose(x: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tensor:

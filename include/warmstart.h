@@ -26,6 +26,16 @@ extern "C" {
 int ws_gemm_bf16(int M, int N, int K, const uint16_t *A, int lda, int trans_a, const uint16_t *B, int ldb,
                  int trans_b, float *C, int ldc, int accumulate, void *stream);
 
+/* C[M][N] = A B^T (+ C if accumulate) for k-contiguous A[M][K] and B[N][K]
+ * (leading dimensions multiples of 8, 16-byte aligned) with K split into
+ * `splits` ranges multiplied by separate blocks into work[s][M][N] (FP32,
+ * splits x M x N floats; unused when splits = 1) and summed into C in range
+ * order (deterministic).  For products whose output has few tiles but a long
+ * K (the backward product through the output layer).  Returns 0, or -1 on
+ * invalid arguments. */
+int ws_gemm_bf16_splitk(int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb, float *C,
+                        int ldc, int accumulate, float *work, int splits, void *stream);
+
 /* Inference products with fused epilogues (op(A) = A[M][K], op(B) = the
  * transpose of B[N][K], both k-contiguous BF16):
  * ws_gemm_bf16_act: H[M][N] (leading dimension ldh) = bf16(gelu(A B^T + bias)).

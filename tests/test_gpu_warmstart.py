@@ -35,6 +35,32 @@ def test_gemm_exact_on_integers(ws, device, trans_a, trans_b, m, n, k):
     np.testing.assert_array_equal(got.cpu().numpy(), want.astype(np.float32))
 
 
+@pytest.mark.parametrize("splits", [1, 2, 3, 8])
+@pytest.mark.parametrize("m,n,k", [(96, 200, 4136), (130, 257, 1000), (1, 9, 72)])
+def test_gemm_split_k_exact_on_integers(ws, device, splits, m, n, k):
+    """ws_gemm_bf16_splitk: k ranges multiplied by separate blocks and summed
+    in range order; exact on small integers, with and without accumulate."""
+    import ctypes
+    rng = np.random.default_rng(splits * 7 + m + n + k)
+    a = _int_bf16(rng, (m, k))
+    b = _int_bf16(rng, (n, k))
+    want = a.astype(np.float64) @ b.astype(np.float64).T
+    ta = torch.tensor(a, device=device).to(torch.bfloat16).contiguous()
+    tb = torch.tensor(b, device=device).to(torch.bfloat16).contiguous()
+    out = torch.full((m, n), 2.0, device=device)
+    work = torch.empty(splits * m * n, dtype=torch.float32, device=device)
+    lib = ws.load()
+    p = lambda t: ctypes.c_void_p(t.data_ptr())   # noqa: E731
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    assert lib.ws_gemm_bf16_splitk(m, n, k, p(ta), k, p(tb), k, p(out), n, 1, p(work), splits, stream) == 0
+    np.testing.assert_array_equal(out.cpu().numpy(), (want + 2.0).astype(np.float32))
+    # the wrapper picks the split itself for few output tiles and a long k
+    got = ws.gemm(ta, tb, trans_b=True)
+    np.testing.assert_array_equal(got.cpu().numpy(), want.astype(np.float32))
+    # misaligned rows are rejected
+    assert lib.ws_gemm_bf16_splitk(m, n, k, p(ta), k + 1, p(tb), k, p(out), n, 0, p(work), splits, stream) == -1
+
+
 def test_gemm_accumulate_and_lane_layout(ws, device):
     """Distinct entries per (row, col) expose any lane/row/column mix-up."""
     m, n, k = 64, 96, 8
