@@ -239,3 +239,69 @@ class ConsortiumRollout:
 
     def restore(self, snap):
         self.conc, self.biomass = snap[0].clone(), snap[1].clone()
+
+
+class CooperativeRollout:
+    """dFBA rollout whose step is the reference's default cooperative step:
+    max-min growth, then parsimonious exchange (src/community_solver.py:
+    1712-1800, parsimonious_exchange=True by default, :554), with the medium
+    and biomass advanced from the chosen stage-3 fluxes (the same Euler and
+    growth updates as ``ConsortiumRollout``, src/dfba_simulator.py:1317-1321,
+    :1352, :1425-1427).  Stage 1 starts from the previous step's stage-1
+    solution and stage 3 from the previous step's stage-3 solution (projected
+    onto the new bounds), each with its primal weights; ``warm=False`` starts
+    every step cold.  An environment whose step does not certify keeps its
+    state (the reference aborts that environment's step)."""
+
+    def __init__(self, model: Optional[Consortium] = None, n_env: int = 4096, *, seed: int = 2,
+                 precision: str = "fp32", options: Optional[SolveOptions] = None, stages=None,
+                 dt: float = 1.0, conc_range=(1e-2, 10.0), device: Optional[torch.device] = None,
+                 fallback: bool = True, warm: bool = True):
+        from .stages import CooperativeStages
+        model = model if model is not None else consortium(seed)
+        self.options = options or SolveOptions()
+        self.driver = CooperativeStages(model, n_env, precision=precision, options=self.options, stages=stages,
+                                        device=device, fallback=fallback)
+        # the state and its dynamics; the stage driver's joint engine is shared
+        self.state = ConsortiumRollout(model, n_env, seed=seed, precision=precision, options=self.options,
+                                       warm_start="none", dt=dt, conc_range=conc_range, autotune=False,
+                                       fallback=fallback, device=device, engine=self.driver.joint)
+        self.model = model
+        self.n_env = int(n_env)
+        self.warm = bool(warm)
+        self._prev = None
+        self.step_index = 0
+        self.log: list[dict] = []
+
+    def lp_inputs(self, conc=None, biomass=None):
+        return self.state.lp_inputs(conc, biomass)
+
+    def snapshot(self):
+        return self.state.snapshot()
+
+    def step(self) -> dict:
+        st = self.state
+        rhs, lower, upper, coupling = st.lp_inputs()
+        kw = {}
+        if self.warm and self._prev is not None:
+            (x1, y1), (x3, y3), omega = self._prev
+            kw = dict(x0=x1, y0=y1, x3_0=x3, y3_0=y3, omega=omega)
+            # a warm start is checked (and possibly polished) at once
+            self.driver.options = dataclasses.replace(self.options, polish_kkt_start=self.options.polish_kkt,
+                                                      first_check_interval=max(1, self.options.check_interval // 2))
+        else:
+            self.driver.options = self.options
+        t0 = time.perf_counter()
+        out = self.driver.solve(rhs, lower, upper, coupling, st.biomass, **kw)
+        ok = out["certified"]
+        it = out["iterates"]
+        if "parsimonious_exchange" in it:
+            self._prev = (it["max_min"], it["parsimonious_exchange"], out["omega"])
+        st.euler(out["x"], ok)
+        torch.cuda.synchronize()
+        info = {"step": self.step_index, "certified": int(ok.sum()), "wall_seconds": time.perf_counter() - t0}
+        for name, si in out["stage_info"].items():
+            info[name] = {k: si.get(k) for k in ("certified", "iterations", "polish_attempts", "cg_steps")}
+        self.log.append(info)
+        self.step_index += 1
+        return dict(info, certified_mask=ok, x=out["x"], t_star=out["t_star"], transfer=out.get("transfer"))

@@ -56,6 +56,8 @@ class StageOptions:
     later_precision: str = "fp64"
     # iteration budget of stages 2 and 3 (None: the solve options' budget)
     later_max_iterations: Optional[int] = None
+    # active-set polish in stages 2 and 3 (None: as the solve options say)
+    later_polish: Optional[bool] = None
     # PDHG method of stage 1 and of the later (FP64) stages ("restarted" or
     # "halpern"; Halpern iterates need FP64 to reach the polish threshold)
     method: str = "restarted"
@@ -174,6 +176,8 @@ class CooperativeStages:
         opts = self.options
         if later and self.stages.later_max_iterations is not None:
             opts = dataclasses.replace(opts, max_iterations=int(self.stages.later_max_iterations))
+        if later and self.stages.later_polish is not None:
+            opts = dataclasses.replace(opts, polish=bool(self.stages.later_polish))
         if self.fallback and eng.precision == "fp32":
             info, x, y, status = eng.solve_with_fallback(opts)
         else:
@@ -181,24 +185,35 @@ class CooperativeStages:
             x, y, status = eng.state()
         return info, x, y, status
 
-    def solve(self, rhs, lower, upper, coupling, biomass, x0=None, y0=None) -> dict:
+    def solve(self, rhs, lower, upper, coupling, biomass, x0=None, y0=None, x3_0=None, y3_0=None,
+              omega=None) -> dict:
         """One cooperative step for every environment.
 
         rhs [B, m], lower/upper [B, n], coupling [B, T] (the present medium
         terms' coefficients -biomass_s), biomass [B, S] are device tensors (as
-        ``ConsortiumRollout.lp_inputs`` returns them).  Returns the chosen
-        fluxes x [B, n_flux], t*, the aggregate objective, per-stage status
-        and the mask of environments whose every stage certified."""
+        ``ConsortiumRollout.lp_inputs`` returns them).  (x0, y0) start stage 1;
+        (x3_0, y3_0) start stage 3 instead of stage 1's answer (a previous
+        dFBA step's stage-3 solution, projected onto the new bounds); omega
+        maps stage names to primal weights to start from.  Returns the chosen
+        fluxes x [B, n_flux], t*, the aggregate objective, per-stage status,
+        the final iterates and primal weights of each stage ("iterates",
+        "omega") and the mask of environments whose every stage certified."""
         t0 = time.perf_counter()
         md, st, dev = self.model, self.stages, self.device
         m, n = md.shape
         B = self.n_env
         out = {"stage_info": {}}
         # stage 1: max-min growth
+        omega = omega or {}
+        out["iterates"], out["omega"] = {}, {}
         eng = self.joint
         eng.update(md.c, rhs, lower, upper, var_values=coupling)
         eng.reset(x0, y0)
+        if x0 is not None and omega.get("max_min") is not None:
+            eng.set_primal_weight(omega["max_min"])
         info1, x1, y1, s1 = self._solve(eng)
+        out["iterates"]["max_min"] = (x1, y1)
+        out["omega"]["max_min"] = eng.primal_weight()
         ok = torch.as_tensor(s1 == _lib.STATUS_OPTIMAL, device=dev)
         t_star = torch.clamp(x1[:, md.t_col], min=0.0)
         floor = t_star * st.coexistence_fraction
@@ -236,14 +251,22 @@ class CooperativeStages:
                                          torch.full_like(aggregate, 1e6))
             lo3 = torch.cat([lo2, torch.zeros((B, self.n_aux), dtype=torch.float64, device=dev)], dim=1)
             up3 = torch.cat([up2, torch.full((B, self.n_aux), np.inf, dtype=torch.float64, device=dev)], dim=1)
-            xs = torch.minimum(torch.maximum(x, lo2), up2)
-            a0 = torch.abs(scale * xs[:, self.term_cols])
-            x3 = torch.cat([xs, a0], dim=1)
-            y3 = torch.cat([y, torch.zeros((B, extra), dtype=torch.float64, device=dev)], dim=1)
+            if x3_0 is not None:
+                x3 = torch.minimum(torch.maximum(x3_0, lo3), up3)
+                y3 = y3_0
+            else:
+                xs = torch.minimum(torch.maximum(x, lo2), up2)
+                a0 = torch.abs(scale * xs[:, self.term_cols])
+                x3 = torch.cat([xs, a0], dim=1)
+                y3 = torch.cat([y, torch.zeros((B, extra), dtype=torch.float64, device=dev)], dim=1)
             e3 = self.stage3
             e3.update(self.c3, rhs3, lo3.contiguous(), up3.contiguous(), var_values=var3)
             e3.reset(x3.contiguous(), y3.contiguous())
+            if x3_0 is not None and omega.get("parsimonious_exchange") is not None:
+                e3.set_primal_weight(omega["parsimonious_exchange"])
             info3, x3, y3, s3 = self._solve(e3, later=True)
+            out["iterates"]["parsimonious_exchange"] = (x3, y3)
+            out["omega"]["parsimonious_exchange"] = e3.primal_weight()
             ok &= torch.as_tensor(s3 == _lib.STATUS_OPTIMAL, device=dev)
             chosen = x3[:, :n]
             out["stage_info"]["parsimonious_exchange"] = info3
