@@ -143,6 +143,7 @@ struct lpe_handle {
     LpState* d_st = nullptr;
     int *d_flags = nullptr, *d_cnt = nullptr, *d_source = nullptr, *d_sel = nullptr, *d_passed = nullptr;
     double *d_om = nullptr, *d_iom = nullptr, *d_bnorm = nullptr, *d_cnorm = nullptr;
+    double* d_bnorm_pw = nullptr;   // rhs norm of the initial primal weight (unlimited rows left out)
     float *d_om_f = nullptr, *d_iom_f = nullptr;
 
     // reductions
@@ -468,6 +469,17 @@ constexpr int CHECK_RPW = 32;
 // per-block partial sums dominate.
 // LPE_POLISH_RPW / LPE_POLISH_LPL override both for launch-shape sweeps (read
 // when the handle sizes its partial buffers and at every launch).
+// Right-hand sides at least this large (original units) are unlimited
+// supplies for the initial primal weight.
+constexpr double LPE_RHS_UNLIMITED = 1e5;
+
+// A real-valued setting from the environment (sweeps), def when unset or outside [lo, hi].
+double env_double(const char* name, double def, double lo, double hi) {
+    const char* v = std::getenv(name);
+    if (!v) return def;
+    const double k = std::atof(v);
+    return (k >= lo && k <= hi) ? k : def;
+}
 int env_choice(const char* name, int lo, int hi) {
     const char* v = std::getenv(name);
     if (!v) return 0;
@@ -657,6 +669,8 @@ int launch_check_t(lpe_handle* h, int K, int certify, double ep, double ed, doub
     d.omega64 = h->d_om; d.omega32 = h->d_om_f; d.inv_omega64 = h->d_iom; d.inv_omega32 = h->d_iom_f;
     d.bnorm = h->d_bnorm; d.cnorm = h->d_cnorm; d.batch = h->B; d.ld = h->ld; d.K = K;
     d.restarted = restarted ? 1 : 0; d.hal = h->halpern; d.certify = certify; d.eps_p = ep; d.eps_d = ed; d.eps_g = eg;
+    d.pw_theta = env_double("LPE_PW_THETA", 0.5, 0.0, 1.0);
+    d.pw_step = env_double("LPE_PW_STEP", 0.0, 0.0, 50.0);
     hipLaunchKernelGGL(k_check_decide, dim3(ceil_div(h->B, 256)), dim3(256), 0, h->s, d);
     if (h->method == LPE_METHOD_RESTARTED) {
         ApplyArgs<T> ax;
@@ -738,11 +752,12 @@ int launch_infeasibility(lpe_handle* h, double eps) {
 
 // Per-LP norms of a batch-minor vector; the chunk partials use the block
 // partial buffer (sized for many more values per LP than rows / NORM_CHUNK).
-void launch_norms(lpe_handle* h, const double* v, int rows, int shared, double* out) {
+void launch_norms(lpe_handle* h, const double* v, int rows, int shared, double* out,
+                  const double* mask = nullptr, double cap = 0.0) {
     const int chunks = ceil_div(rows, NORM_CHUNK);
     if (chunks > 0)
         hipLaunchKernelGGL(k_norms_part, dim3(ceil_div(h->ld, 256), chunks), dim3(256), 0, h->s, v, rows, h->ld,
-                           shared, h->d_partial);
+                           shared, h->d_partial, mask, cap);
     hipLaunchKernelGGL(k_norms_final, dim3(ceil_div(h->ld, 256)), dim3(256), 0, h->s, (const double*)h->d_partial,
                        chunks, h->ld, out);
 }
@@ -1392,7 +1407,7 @@ int alloc_all(lpe_handle* h) {
     CHK(dalloc(h, &h->d_st, ld)); CHK(dalloc(h, &h->d_flags, ld)); CHK(dalloc(h, &h->d_cnt, ld));
     CHK(dalloc(h, &h->d_source, ld)); CHK(dalloc(h, &h->d_sel, ld)); CHK(dalloc(h, &h->d_passed, ld));
     CHK(dalloc(h, &h->d_om, ld)); CHK(dalloc(h, &h->d_iom, ld)); CHK(dalloc(h, &h->d_om_f, ld)); CHK(dalloc(h, &h->d_iom_f, ld));
-    CHK(dalloc(h, &h->d_bnorm, ld)); CHK(dalloc(h, &h->d_cnorm, ld));
+    CHK(dalloc(h, &h->d_bnorm, ld)); CHK(dalloc(h, &h->d_cnorm, ld)); CHK(dalloc(h, &h->d_bnorm_pw, ld));
     // partial buffers: max over kernels of n_rb * NV * ld
     int g;
     Shape sr = make_shape(std::max(h->m, 1), h->ld, h->B, CHECK_RPW, 1, &g);
@@ -1514,8 +1529,9 @@ int init_state(lpe_handle* h) {
     HIPCHK(hipMemcpyAsync(h->d_iom_f, iomf.data(), sizeof(float) * h->ld, hipMemcpyHostToDevice, h->s));
     if (h->method == LPE_METHOD_RESTARTED && h->have_vectors)
         hipLaunchKernelGGL(k_init_omega, dim3(ceil_div(h->B, 256)), dim3(256), 0, h->s, h->d_st, h->d_om, h->d_om_f,
-                           h->d_iom, h->d_iom_f, (const double*)h->d_bnorm, (const double*)h->d_cnorm,
-                           (const double*)nullptr, h->pw, h->B);
+                           h->d_iom, h->d_iom_f, (const double*)h->d_bnorm_pw, (const double*)h->d_cnorm,
+                           (const double*)nullptr, h->pw > 0.0 ? h->pw : env_double("LPE_PW_INIT", 0.0, 0.0, 1e8),
+                           h->B);
     // host vectors must outlive the async copies
     HIPCHK(hipStreamSynchronize(h->s));
     return LPE_OK;
@@ -1801,6 +1817,15 @@ int lpe_update(lpe_handle* h, const lpe_vectors* v, void* stream) {
     }
     launch_norms(h, (const double*)h->d_rhs_s, (int)m, 0, h->d_bnorm);
     launch_norms(h, (const double*)h->d_c_s, (int)n, h->shared_c, h->d_cnorm);
+    // The initial primal weight |c^| / |b^| leaves out rows whose right-hand
+    // side stands for an unlimited supply (the reference writes 1e6 for the
+    // water / proton / hydroxide medium rows, src/community_solver.py:1146-1148):
+    // counted, they set |b^| alone and start w orders of magnitude below the
+    // weight the restarts settle at (real-GEM reduced LPs: 7e-7 against 0.2;
+    // certified within 100,000 Halpern iterations 182 -> 214 of 256).  A
+    // model without such rows keeps the plain norm ratio.
+    launch_norms(h, (const double*)h->d_rhs_s, (int)m, 0, h->d_bnorm_pw, (const double*)h->d_rhs_o,
+                 env_double("LPE_PW_RHS_CAP", LPE_RHS_UNLIMITED, 0.0, INFINITY));
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->s));
     h->have_vectors = true;
@@ -2191,7 +2216,7 @@ int lpe_set_primal_weight(lpe_handle* h, const double* omega, void* stream) {
     double* tmp = h->d_metrics;   // scratch: [8][ld] doubles >= B
     HIPCHK(hipMemcpyAsync(tmp, omega, sizeof(double) * h->B, hipMemcpyHostToDevice, h->s));
     hipLaunchKernelGGL(k_init_omega, dim3(ceil_div(h->B, 256)), dim3(256), 0, h->s, h->d_st, h->d_om, h->d_om_f,
-                       h->d_iom, h->d_iom_f, (const double*)h->d_bnorm, (const double*)h->d_cnorm,
+                       h->d_iom, h->d_iom_f, (const double*)h->d_bnorm_pw, (const double*)h->d_cnorm,
                        (const double*)tmp, 1.0, h->B);
     HIPCHK(hipStreamSynchronize(h->s));
     return leave(h, stream);

@@ -901,7 +901,18 @@ struct DecideArgs {
     int hal;                // Halpern: cnt already counts this chunk, candidate 2 is T(z)
     int certify;
     double eps_p, eps_d, eps_g;
+    // primal-weight update at a restart: log w <- theta log(|dy|/|dx|) +
+    // (1 - theta) log w, the change limited to +-step when step > 0
+    double pw_theta, pw_step;
 };
+
+__device__ __forceinline__ double next_primal_weight(double om, double dx, double dy, double theta, double step) {
+    if (!(dx > 1e-10 && dy > 1e-10 && isfinite(dx) && isfinite(dy))) return om;
+    const double lo = log(om);
+    double nl = theta * log(dy / dx) + (1.0 - theta) * lo;
+    if (step > 0.0) nl = fmin(fmax(nl, lo - step), lo + step);
+    return fmin(fmax(exp(nl), 1e-8), 1e8);
+}
 
 __global__ void k_check_decide(DecideArgs a) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -966,10 +977,7 @@ __global__ void k_check_decide(DecideArgs a) {
             s.action = ACT_RESTART_AVG;
             const double dx = sqrt(a.cols[(long)(CC_N + CC_DX2) * ld + b]);
             const double dy = sqrt(a.rows[(long)(CR_N + CR_DY2) * ld + b]);
-            if (dx > 1e-10 && dy > 1e-10 && isfinite(dx) && isfinite(dy)) {
-                om = exp(0.5 * log(dy / dx) + 0.5 * log(om));
-                om = fmin(fmax(om, 1e-8), 1e8);
-            }
+            om = next_primal_weight(om, dx, dy, a.pw_theta, a.pw_step);
             s.omega = om;
             s.kkt_last = fp;
             s.kkt_prev = INFINITY;
@@ -994,10 +1002,7 @@ __global__ void k_check_decide(DecideArgs a) {
             s.action = best ? ACT_RESTART_AVG : ACT_RESTART_CUR;
             const double dx = sqrt(a.cols[(long)(best * CC_N + CC_DX2) * ld + b]);
             const double dy = sqrt(a.rows[(long)(best * CR_N + CR_DY2) * ld + b]);
-            if (dx > 1e-10 && dy > 1e-10 && isfinite(dx) && isfinite(dy)) {
-                om = exp(0.5 * log(dy / dx) + 0.5 * log(om));
-                om = fmin(fmax(om, 1e-8), 1e8);
-            }
+            om = next_primal_weight(om, dx, dy, a.pw_theta, a.pw_step);
             s.omega = om;
             s.kkt_last = kc;
             s.kkt_prev = INFINITY;
@@ -1939,13 +1944,18 @@ __global__ void __launch_bounds__(BLOCK) k_ref_metric(RefMetricArgs a) {
 // then the chunks summed in order per LP (deterministic).
 constexpr int NORM_CHUNK = 64;
 
-__global__ void k_norms_part(const double* __restrict__ v, int rows, int ld, int shared, double* __restrict__ part) {
+// mask/cap: entries whose |mask| is at least cap are left out (the primal
+// weight's right-hand-side norm leaves out rows with unlimited supply).
+__global__ void k_norms_part(const double* __restrict__ v, int rows, int ld, int shared, double* __restrict__ part,
+                             const double* __restrict__ mask = nullptr, double cap = 0.0) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= ld) return;
     const int r0 = blockIdx.y * NORM_CHUNK, r1 = min(rows, r0 + NORM_CHUNK);
     double s = 0.0;
     for (int r = r0; r < r1; ++r) {
-        const double x = shared ? v[r] : v[(long)r * ld + b];
+        const long i = shared ? r : (long)r * ld + b;
+        if (mask && !(fabs(mask[i]) < cap)) continue;
+        const double x = v[i];
         s += x * x;
     }
     part[(long)blockIdx.y * ld + b] = s;

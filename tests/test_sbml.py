@@ -121,8 +121,8 @@ def test_gem_consortium_dimensions_and_highs():
 GEM_EPS_PRIMAL = 1e-7
 
 GEM_PDHG = ("on the real-GEM consortium the parsimonious-exchange LP (stage 3) reaches no certificate: "
-            "0/8 after 300k FP64 iterations on the reduced LP with either method, with or without polish "
-            "(stage 1 certifies 7/8 through the reduction); covered up to stage 1 by "
+            "0/16 after 200k FP64 Halpern iterations on the reduced LP (0/8 after 300k with either method, "
+            "with or without polish; stage 1 certifies 16/16 through the reduction); covered up to stage 1 by "
             "test_gem_consortium_batch_against_highs and test_gem_rollout_steps_against_highs")
 
 
@@ -180,8 +180,9 @@ def test_gem_rollout_steps_against_highs(device):
             ref = highs.solve(a, rhs[b], lower[b], upper[b], md.c, md.neq)
             assert relative_objective_error(float(md.c @ x[b]), ref.fun) <= 1e-6, (step, b)
     print("certified per step", certified, flush=True)
-    assert certified[0] >= 4, certified
-    assert certified[1] >= 3, certified
+    # 5 and 6 of 8 with the primal weight started from the limited rows' norm
+    assert certified[0] >= 5, certified
+    assert certified[1] >= 5, certified
     ro.engine.close()
 
 
