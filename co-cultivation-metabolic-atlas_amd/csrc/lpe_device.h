@@ -56,6 +56,17 @@ __device__ __forceinline__ void store_vec(T* __restrict__ p, const T (&v)[N]) {
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Load of the shared sparse structure (row pointers, indices, slots, shared
+// values) through the constant address space.  With a wave-uniform index the
+// compiler emits scalar loads - batched per chunk and served by the scalar
+// cache - instead of a vector load, a wait and a readfirstlane per entry,
+// which serialised the structure reads ahead of every gather.  The structure
+// is never written while a kernel reads it.
+template <typename T>
+__device__ __forceinline__ T sload(const T* p, long i) {
+    return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+
 // XCD-aware block remap.  The grid is padded to a multiple of 8; block g is
 // dispatched to XCD g % 8, which receives the contiguous virtual range
 // [x * G/8, (x+1) * G/8).  Returns false for padding blocks.
@@ -88,7 +99,7 @@ __device__ __forceinline__ void row_dot(const int* __restrict__ ptr, const int* 
                                         long ld, int b0, int row, TA (&acc)[LPL]) {
 #pragma unroll
     for (int t = 0; t < LPL; ++t) acc[t] = TA(0);
-    const int s = ptr[row], e = ptr[row + 1];
+    const int s = sload(ptr, row), e = sload(ptr, row + 1);
     // Chunks of D entries: all (index, slot, value) scalars of the chunk are
     // loaded first, then all gathers are issued, then the products are
     // accumulated in storage order - the gathers of one chunk are in flight
@@ -101,7 +112,7 @@ __device__ __forceinline__ void row_dot(const int* __restrict__ ptr, const int* 
         TV a[D];
 #pragma unroll
         for (int u = 0; u < D; ++u) {
-            if (u < cnt) { col[u] = idx[k0 + u]; sl[u] = slot[k0 + u]; a[u] = val[k0 + u]; }
+            if (u < cnt) { col[u] = sload(idx, k0 + u); sl[u] = sload(slot, k0 + u); a[u] = sload(val, k0 + u); }
             else { col[u] = 0; sl[u] = -1; a[u] = TV(0); }
         }
         TX xv[D][LPL];
@@ -139,14 +150,14 @@ __device__ __forceinline__ void row_dot2(const int* __restrict__ ptr, const int*
                                          int b0, int row, double (&acc1)[LPL], double (&acc2)[LPL]) {
 #pragma unroll
     for (int t = 0; t < LPL; ++t) { acc1[t] = 0.0; acc2[t] = 0.0; }
-    const int s = ptr[row], e = ptr[row + 1];
+    const int s = sload(ptr, row), e = sload(ptr, row + 1);
     for (int k0 = s; k0 < e; k0 += D) {
         const int cnt = e - k0 < D ? e - k0 : D;
         int col[D], sl[D];
         double a[D];
 #pragma unroll
         for (int u = 0; u < D; ++u) {
-            if (u < cnt) { col[u] = idx[k0 + u]; sl[u] = slot[k0 + u]; a[u] = val[k0 + u]; }
+            if (u < cnt) { col[u] = sload(idx, k0 + u); sl[u] = sload(slot, k0 + u); a[u] = sload(val, k0 + u); }
             else { col[u] = 0; sl[u] = -1; a[u] = 0.0; }
         }
         double2 xv[D][LPL];
@@ -1321,10 +1332,11 @@ __global__ void __launch_bounds__(BLOCK) k_polish_rows_init(PolishRowsInitArgs<T
             }
             a.rmask[i] = on;
             double dg = 0.0;
-            for (int k = a.As.ptr[row]; k < a.As.ptr[row + 1]; ++k) {
-                const int sl = a.As.slot[k];
-                const double av = sl < 0 ? a.As.val[k] : a.As.var[(long)sl * ld + b0];
-                if (a.fmask[(long)a.As.idx[k] * ld + b0]) dg = dg + av * av;
+            const int k1 = sload(a.As.ptr, row + 1);
+            for (int k = sload(a.As.ptr, row); k < k1; ++k) {
+                const int sl = sload(a.As.slot, k);
+                const double av = sl < 0 ? sload(a.As.val, k) : a.As.var[(long)sl * ld + b0];
+                if (a.fmask[(long)sload(a.As.idx, k) * ld + b0]) dg = dg + av * av;
             }
             const double inv = (on && dg > 0.0) ? 1.0 / dg : 0.0;
             a.invdiag[i] = inv;
@@ -1926,10 +1938,10 @@ __global__ void __launch_bounds__(BLOCK) k_ref_metric(RefMetricArgs a) {
         const int row = uniform(r0 + r);
         if (row >= a.sh.rows) break;
         double s = 0.0;
-        const int k0 = a.A.ptr[row], k1 = a.A.ptr[row + 1];
+        const int k0 = sload(a.A.ptr, row), k1 = sload(a.A.ptr, row + 1);
         for (int k = k0; k < k1; ++k) {
-            const int sl = a.A.slot[k];
-            const double v = sl < 0 ? a.A.val[k] : a.A.var[(long)sl * ld + b0];
+            const int sl = sload(a.A.slot, k);
+            const double v = sl < 0 ? sload(a.A.val, k) : a.A.var[(long)sl * ld + b0];
             s = s + __builtin_fabs(v);
         }
         const double base = s > 0.0 ? a.eta / s : 1.0;
