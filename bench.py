@@ -605,6 +605,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--max-iterations", type=int, default=30000)
     ap.add_argument("--cg-iterations", type=int, default=200, help="CG steps per polish round")
+    ap.add_argument("--polish-kkt", type=float, default=1e-4,
+                    help="relative KKT error at which the target form tries its active-set polish")
     ap.add_argument("--fp32-iterations", type=int, default=6000,
                     help="FP32 budget per step; uncertified LPs continue on the FP64 fallback engine")
     ap.add_argument("--probe-kernels", action="store_true",
@@ -662,7 +664,7 @@ def main():
     K = args.steps
 
     target_options = lambda: SolveOptions(max_iterations=args.fp32_iterations, polish=True,  # noqa: E731
-                                          polish_cg_iterations=args.cg_iterations)
+                                          polish_cg_iterations=args.cg_iterations, polish_kkt=args.polish_kkt)
     warm, mlp_report = "previous", None
     if args.warm_start == "mlp":
         warm, mlp_report = train_warm_start(model, B, dev, args, target_options)
