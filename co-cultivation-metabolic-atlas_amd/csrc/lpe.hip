@@ -124,6 +124,7 @@ struct lpe_handle {
     double *d_val_o = nullptr, *d_tval_o = nullptr, *d_val_s = nullptr, *d_tval_s = nullptr;
     float *d_val_f = nullptr, *d_tval_f = nullptr;
     double *d_var_o = nullptr, *d_var_s = nullptr;
+    double* d_var_absmax = nullptr;   // [nvar] max |scaled value| over the batch (device side)
     float* d_var_f = nullptr;
     double *d_dr = nullptr, *d_dc = nullptr, *d_idr = nullptr, *d_idc = nullptr;
     double *d_sig = nullptr, *d_tau = nullptr;
@@ -1374,6 +1375,7 @@ int alloc_all(lpe_handle* h) {
     CHK(dalloc(h, &h->d_val_s, nnz)); CHK(dalloc(h, &h->d_tval_s, nnz));
     CHK(dalloc(h, &h->d_val_f, nnz)); CHK(dalloc(h, &h->d_tval_f, nnz));
     CHK(dalloc(h, &h->d_var_o, nv * ld)); CHK(dalloc(h, &h->d_var_s, nv * ld)); CHK(dalloc(h, &h->d_var_f, nv * ld));
+    CHK(dalloc(h, &h->d_var_absmax, nv));
     CHK(dalloc(h, &h->d_dr, m)); CHK(dalloc(h, &h->d_dc, n)); CHK(dalloc(h, &h->d_idr, m)); CHK(dalloc(h, &h->d_idc, n));
     CHK(dalloc(h, &h->d_sig, m)); CHK(dalloc(h, &h->d_tau, n)); CHK(dalloc(h, &h->d_sig_f, m)); CHK(dalloc(h, &h->d_tau_f, n));
     const bool ref = h->method == LPE_METHOD_FIXED;
@@ -1500,10 +1502,13 @@ void free_all(lpe_handle* h) {
 }
 
 int init_state(lpe_handle* h) {
+    // the host copy mirrors what k_init_state writes on the device (it is
+    // read back from the device after every check)
+    const double omega0 = (h->method == LPE_METHOD_RESTARTED && h->pw > 0.0) ? h->pw : 1.0;
     for (int b = 0; b < h->ld; ++b) {
         LpState& s = h->h_st[b];
         std::memset(&s, 0, sizeof(s));
-        s.omega = (h->method == LPE_METHOD_RESTARTED && h->pw > 0.0) ? h->pw : 1.0;
+        s.omega = omega0;
         s.kkt_last = INFINITY;
         s.kkt_prev = INFINITY;
         s.rel_kkt = INFINITY;
@@ -1512,28 +1517,14 @@ int init_state(lpe_handle* h) {
         s.status = b < h->B ? 0 : 1;
         s.asig = -1.0;
     }
-    HIPCHK(hipMemcpyAsync(h->d_st, h->h_st, sizeof(LpState) * h->ld, hipMemcpyHostToDevice, h->s));
-    std::vector<int> flags(h->ld), zeros(h->ld, 0);
-    std::vector<double> om(h->ld), iom(h->ld);
-    std::vector<float> omf(h->ld), iomf(h->ld);
-    for (int b = 0; b < h->ld; ++b) {
-        flags[b] = b < h->B ? 0 : 1;
-        om[b] = h->h_st[b].omega; iom[b] = 1.0 / om[b]; omf[b] = (float)om[b]; iomf[b] = (float)iom[b];
-    }
-    HIPCHK(hipMemcpyAsync(h->d_flags, flags.data(), sizeof(int) * h->ld, hipMemcpyHostToDevice, h->s));
-    HIPCHK(hipMemcpyAsync(h->d_cnt, zeros.data(), sizeof(int) * h->ld, hipMemcpyHostToDevice, h->s));
-    HIPCHK(hipMemcpyAsync(h->d_source, zeros.data(), sizeof(int) * h->ld, hipMemcpyHostToDevice, h->s));
-    HIPCHK(hipMemcpyAsync(h->d_om, om.data(), sizeof(double) * h->ld, hipMemcpyHostToDevice, h->s));
-    HIPCHK(hipMemcpyAsync(h->d_iom, iom.data(), sizeof(double) * h->ld, hipMemcpyHostToDevice, h->s));
-    HIPCHK(hipMemcpyAsync(h->d_om_f, omf.data(), sizeof(float) * h->ld, hipMemcpyHostToDevice, h->s));
-    HIPCHK(hipMemcpyAsync(h->d_iom_f, iomf.data(), sizeof(float) * h->ld, hipMemcpyHostToDevice, h->s));
+    hipLaunchKernelGGL(k_init_state, dim3(ceil_div(h->ld, 256)), dim3(256), 0, h->s, h->d_st, h->d_flags, h->d_cnt,
+                       h->d_source, h->d_om, h->d_om_f, h->d_iom, h->d_iom_f, omega0, h->B, h->ld);
     if (h->method == LPE_METHOD_RESTARTED && h->have_vectors)
         hipLaunchKernelGGL(k_init_omega, dim3(ceil_div(h->B, 256)), dim3(256), 0, h->s, h->d_st, h->d_om, h->d_om_f,
                            h->d_iom, h->d_iom_f, (const double*)h->d_bnorm_pw, (const double*)h->d_cnorm,
                            (const double*)nullptr, h->pw > 0.0 ? h->pw : env_double("LPE_PW_INIT", 0.0, 0.0, 1e8),
                            h->B);
-    // host vectors must outlive the async copies
-    HIPCHK(hipStreamSynchronize(h->s));
+    HIPCHK(hipGetLastError());
     return LPE_OK;
 }
 
@@ -1756,16 +1747,8 @@ int lpe_update(lpe_handle* h, const lpe_vectors* v, void* stream) {
     // scaled copies
     const bool ref = h->method == LPE_METHOD_FIXED;
     if (h->shared_c) {
-        std::vector<double> hc(n);
-        HIPCHK(hipMemcpyAsync(hc.data(), h->d_c_o, sizeof(double) * n, hipMemcpyDeviceToHost, h->s));
-        HIPCHK(hipStreamSynchronize(h->s));
-        for (long j = 0; j < n; ++j) hc[j] *= h->dc[j];
-        CHK(upload(h->d_c_s, hc.data(), n));
-        if (h->d_c_f) {
-            std::vector<float> f(n);
-            for (long j = 0; j < n; ++j) f[j] = (float)hc[j];
-            CHK(upload(h->d_c_f, f.data(), n));
-        }
+        hipLaunchKernelGGL(k_scale_shared, dim3(ceil_div(n, 256)), tb, 0, h->s, (const double*)h->d_c_o,
+                           (const double*)h->d_dc, h->d_c_s, h->d_c_f, (int)n);
     } else {
         hipLaunchKernelGGL((k_scale_copy<double>), ew_grid(n, ld), tb, 0, h->s, (const double*)h->d_c_o, h->d_c_s, (int)n,
                            (int)ld, ROWS_PER_BLOCK_EW, (const double*)h->d_dc, (const int*)nullptr, 0);
@@ -1803,15 +1786,13 @@ int lpe_update(lpe_handle* h, const lpe_vectors* v, void* stream) {
         r.sh = make_shape((int)n, (int)ld, h->B, 1, 1, &grid); r.A = op_orig(h, true); r.out = h->d_tauref; r.divide = 1;
         hipLaunchKernelGGL(k_ref_metric, dim3(grid), dim3(BLOCK), 0, h->s, r);
     } else if (h->nvar > 0) {
-        // shared metric from the largest per-LP magnitudes of the batch
-        std::vector<double> hv((size_t)h->nvar * ld);
-        HIPCHK(hipMemcpyAsync(hv.data(), h->d_var_s, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, h->s));
+        // shared metric from the largest per-LP magnitudes of the batch (reduced
+        // on the device: only nvar values cross to the host)
+        hipLaunchKernelGGL(k_var_absmax, dim3(h->nvar), tb, 0, h->s, (const double*)h->d_var_s, h->d_var_absmax,
+                           (int)B, (int)ld);
+        HIPCHK(hipMemcpyAsync(h->var_absmax.data(), h->d_var_absmax, sizeof(double) * h->nvar,
+                              hipMemcpyDeviceToHost, h->s));
         HIPCHK(hipStreamSynchronize(h->s));
-        for (int s = 0; s < h->nvar; ++s) {
-            double mx = 0.0;
-            for (long b = 0; b < B; ++b) mx = std::max(mx, std::fabs(hv[(size_t)s * ld + b]));
-            h->var_absmax[s] = mx;
-        }
         shared_metric(h);
         CHK(upload_metric(h));
     }

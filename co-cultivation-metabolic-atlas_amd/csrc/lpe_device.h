@@ -869,6 +869,34 @@ struct LpState {
 
 enum Action { ACT_NONE = 0, ACT_RESTART_CUR = 1, ACT_RESTART_AVG = 2, ACT_ACCEPT_CUR = 3, ACT_ACCEPT_AVG = 4 };
 
+// Per-LP solver state at a reset: LPs b < batch running with primal weight
+// omega0, padding LPs frozen; iteration counters and result sources zeroed.
+__global__ void k_init_state(LpState* st, int* flags, int* cnt, int* source, double* om, float* om_f, double* iom,
+                             float* iom_f, double omega0, int batch, int ld) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= ld) return;
+    LpState s;
+    s.omega = omega0;
+    s.kkt_last = INFINITY;
+    s.kkt_prev = INFINITY;
+    s.rel_kkt = INFINITY;
+    s.avg_div = 1.0;
+    s.iters = 0;
+    s.accepted_iter = -1;
+    s.status = b < batch ? 0 : 1;
+    s.action = 0;
+    s.restarts = 0;
+    s.source = 0;
+    s.asig = -1.0;
+    s.asig_same = 0;
+    s.pad_ = 0;
+    st[b] = s;
+    flags[b] = b < batch ? 0 : 1;
+    cnt[b] = 0;
+    source[b] = 0;
+    om[b] = omega0; iom[b] = 1.0 / omega0; om_f[b] = (float)omega0; iom_f[b] = (float)(1.0 / omega0);
+}
+
 // One thread per LP: a certificate marks a running LP primal (3) or dual (4)
 // infeasible and freezes it.
 __global__ void k_infeas_decide(LpState* st, int* flags, const double* rows, const double* cols, int batch, int ld,
@@ -1912,6 +1940,33 @@ __global__ void k_var_scale(const double* __restrict__ in, TO* __restrict__ out,
     if (b >= ld || s >= nvar) return;
     const double v = in[(long)s * ld + b];
     out[(long)s * ld + b] = (TO)(v * dr[vrow[s]] * dc[vcol[s]]);
+}
+
+// out[s] = max over the real LPs b < batch of |var[s][b]| (one block per slot;
+// the maximum does not depend on the order).
+__global__ void __launch_bounds__(256) k_var_absmax(const double* __restrict__ var, double* __restrict__ out,
+                                                    int batch, int ld) {
+    __shared__ double part[256];
+    const int s = blockIdx.x;
+    double mx = 0.0;
+    for (int b = threadIdx.x; b < batch; b += blockDim.x) mx = fmax(mx, fabs(var[(long)s * ld + b]));
+    part[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] = fmax(part[threadIdx.x], part[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[s] = part[0];
+}
+
+// Shared scaled objective: cs[j] = c[j] * dc[j] (and its FP32 copy when cf).
+__global__ void k_scale_shared(const double* __restrict__ c, const double* __restrict__ dc, double* __restrict__ cs,
+                               float* __restrict__ cf, int n) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const double v = c[j] * dc[j];
+    cs[j] = v;
+    if (cf) cf[j] = (float)v;
 }
 
 // Fixed-metric per-LP diagonal metric (src/gpu_pdhg_corrector.py:185-196):
