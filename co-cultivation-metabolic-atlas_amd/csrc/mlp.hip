@@ -347,6 +347,117 @@ __global__ void __launch_bounds__(256) k_gemm_kk(int M, int N, int K, const uint
     store_tile<EPI>(acc, m0 + wm, n0 + wn, lane, M, N, C, ldc, accumulate, ep);
 }
 
+// ---------------------------------------------------------------------------
+// k-contiguous GEMM with direct global-to-LDS loads (gfx950
+// global_load_lds_dwordx4: each lane's 16 bytes land at the wave's LDS base +
+// 16 x lane, no registers involved), four LDS stages, K in blocks of 32 (K a
+// multiple of 32).  A 128 x 32 operand tile is 8 KB = eight 1 KB wave loads;
+// row r of the tile holds its four 16-byte k parts at slots kp ^ ((r >> 1) & 3)
+// (the load picks each lane's k part accordingly), so the MFMA operand reads
+// of 32 consecutive rows hit distinct banks.  Each stage is its own LDS array,
+// so the compiler's wait before reading a stage covers only that stage's loads.
+// Rows past M (N) read row M - 1 (N - 1): their products reach only outputs
+// that are never stored.
+constexpr int DL_KB = 32, DL_NS = 4;
+
+struct DlStage {
+    uint16_t* a;    // [128][32] swizzled
+    uint16_t* b;
+};
+
+__device__ __forceinline__ void dl_issue(uint16_t* lds, const uint16_t* __restrict__ X, int ldx, int r0, int R,
+                                         int k0, int wave, int lane) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int c = wave * 2 + q;                     // 1 KB chunk of the tile: rows 16c .. 16c + 15
+        const int row = c * 16 + (lane >> 2);
+        const int kp = (lane & 3) ^ ((row >> 1) & 3);
+        const int gr = min(r0 + row, R - 1);
+        const uint16_t* src = X + (long)gr * ldx + k0 + kp * 8;
+        __builtin_amdgcn_global_load_lds((const void*)src,
+                                         (__attribute__((address_space(3))) void*)(lds + c * 512), 16, 0, 0);
+    }
+}
+
+__device__ __forceinline__ bf16x8_t dl_read(const uint16_t* lds, int row, int kp) {
+    return *reinterpret_cast<const bf16x8_t*>(lds + row * 32 + ((kp ^ ((row >> 1) & 3)) << 3));
+}
+
+// Split K as in k_gemm_kk (gridDim.z > 1, EPI 0): block z multiplies the k
+// range [z kchunk, (z + 1) kchunk) into C + z zstride (kchunk a multiple of 32).
+template <int EPI = 0>
+__global__ void __launch_bounds__(256) k_gemm_dl(int M, int N, int K, const uint16_t* __restrict__ A, int lda,
+                                                 const uint16_t* __restrict__ B, int ldb, float* __restrict__ C,
+                                                 int ldc, int accumulate, Epilogue ep, int kchunk = 0,
+                                                 long zstride = 0) {
+    __shared__ __align__(16) uint16_t A0[128 * 32], A1[128 * 32], A2[128 * 32], A3[128 * 32];
+    __shared__ __align__(16) uint16_t B0[128 * 32], B1[128 * 32], B2[128 * 32], B3[128 * 32];
+    if (gridDim.z > 1) {
+        const int kb = blockIdx.z * kchunk;
+        A += kb;
+        B += kb;
+        C += blockIdx.z * zstride;
+        K = min(K - kb, kchunk);
+    }
+    uint16_t* const As[DL_NS] = {A0, A1, A2, A3};
+    uint16_t* const Bs[DL_NS] = {B0, B1, B2, B3};
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int wm = (wave / 2) * 64, wn = (wave % 2) * 64;
+    float16_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+    const int nk = K / DL_KB;
+#pragma unroll
+    for (int s = 0; s < DL_NS - 1; ++s)
+        if (s < nk) {
+            dl_issue(As[s], A, lda, m0, M, s * DL_KB, wave, lane);
+            dl_issue(Bs[s], B, ldb, n0, N, s * DL_KB, wave, lane);
+        }
+    for (int kt0 = 0; kt0 < nk; kt0 += DL_NS) {
+#pragma unroll
+        for (int j = 0; j < DL_NS; ++j) {
+            const int kt = kt0 + j;
+            if (kt < nk) {
+                // stage kt's loads (this wave's) are complete; the barrier makes
+                // every wave's visible and ends the reads of stage kt - 1
+                const int later = min(DL_NS - 2, nk - 1 - kt);
+                if (later >= 2) __builtin_amdgcn_s_waitcnt(0xF78);        // vmcnt(8)
+                else if (later == 1) __builtin_amdgcn_s_waitcnt(0xF74);   // vmcnt(4)
+                else __builtin_amdgcn_s_waitcnt(0xF70);                   // vmcnt(0)
+                // a bare barrier: __syncthreads' fence would wait for every
+                // load in flight, the later stages' included
+                __builtin_amdgcn_s_barrier();
+                if (kt + DL_NS - 1 < nk) {
+                    const int ns = (j + DL_NS - 1) % DL_NS;
+                    dl_issue(As[ns], A, lda, m0, M, (kt + DL_NS - 1) * DL_KB, wave, lane);
+                    dl_issue(Bs[ns], B, ldb, n0, N, (kt + DL_NS - 1) * DL_KB, wave, lane);
+                }
+                const uint16_t* a_s = As[j];
+                const uint16_t* b_s = Bs[j];
+#pragma unroll
+                for (int ks = 0; ks < DL_KB / 16; ++ks) {
+                    bf16x8_t a[2], b[2];
+#pragma unroll
+                    for (int i = 0; i < 2; ++i) a[i] = dl_read(a_s, wm + i * 32 + lane % 32, ks * 2 + lane / 32);
+#pragma unroll
+                    for (int jj = 0; jj < 2; ++jj) b[jj] = dl_read(b_s, wn + jj * 32 + lane % 32, ks * 2 + lane / 32);
+#pragma unroll
+                    for (int i = 0; i < 2; ++i)
+#pragma unroll
+                        for (int jj = 0; jj < 2; ++jj)
+                            acc[i][jj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[jj], acc[i][jj], 0, 0, 0);
+                }
+            }
+        }
+    }
+    store_tile<EPI>(acc, m0 + wm, n0 + wn, lane, M, N, C, ldc, accumulate, ep);
+}
+
 // C[r][c] = (accumulate ? C[r][c] : 0) + sum over s = 0 .. S-1 of W[s][r][c], in that order.
 __global__ void k_splitk_reduce(const float* __restrict__ W, float* __restrict__ C, int M, int N, int ldc, int S,
                                 int accumulate) {
@@ -494,7 +605,9 @@ int ws_gemm_bf16(int M, int N, int K, const uint16_t* A, int lda, int trans_a, c
     if (!trans_a && !trans_b)
         hipLaunchKernelGGL((k_gemm<0, 0>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate, va, vb, Epilogue{});
     else if (!trans_a && trans_b)
-        if (va && vb)
+        if (va && vb && K % DL_KB == 0)
+            hipLaunchKernelGGL((k_gemm_dl<0>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate, Epilogue{}, 0, 0L);
+        else if (va && vb)
             hipLaunchKernelGGL((k_gemm_kk<0>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate, Epilogue{}, 0, 0L);
         else
             hipLaunchKernelGGL((k_gemm<0, 1>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate, va, vb, Epilogue{});
@@ -514,14 +627,23 @@ int ws_gemm_bf16_splitk(int M, int N, int K, const uint16_t* A, int lda, const u
     // k chunks are whole K blocks, so every slice's operand rows stay 16-byte aligned
     const int kchunk = ((K + splits - 1) / splits + KB - 1) / KB * KB;
     const int S = K == 0 ? 1 : (K + kchunk - 1) / kchunk;
+    const bool dl = K % DL_KB == 0;      // direct-to-LDS kernel (kchunk is a multiple of 32 too)
     if (S <= 1) {
-        hipLaunchKernelGGL((k_gemm_kk<0>), dim3(blocks(N, BN), blocks(M, BM)), dim3(256), 0, s, M, N, K, A, lda, B,
-                           ldb, C, ldc, accumulate, Epilogue{}, 0, 0L);
+        if (dl)
+            hipLaunchKernelGGL((k_gemm_dl<0>), dim3(blocks(N, BN), blocks(M, BM)), dim3(256), 0, s, M, N, K, A, lda,
+                               B, ldb, C, ldc, accumulate, Epilogue{}, 0, 0L);
+        else
+            hipLaunchKernelGGL((k_gemm_kk<0>), dim3(blocks(N, BN), blocks(M, BM)), dim3(256), 0, s, M, N, K, A, lda,
+                               B, ldb, C, ldc, accumulate, Epilogue{}, 0, 0L);
         return hipGetLastError() == hipSuccess ? 0 : -2;
     }
     const long zstride = (long)M * N;
-    hipLaunchKernelGGL((k_gemm_kk<0>), dim3(blocks(N, BN), blocks(M, BM), S), dim3(256), 0, s, M, N, K, A, lda, B,
-                       ldb, work, N, 0, Epilogue{}, kchunk, zstride);
+    if (dl)
+        hipLaunchKernelGGL((k_gemm_dl<0>), dim3(blocks(N, BN), blocks(M, BM), S), dim3(256), 0, s, M, N, K, A, lda,
+                           B, ldb, work, N, 0, Epilogue{}, kchunk, zstride);
+    else
+        hipLaunchKernelGGL((k_gemm_kk<0>), dim3(blocks(N, BN), blocks(M, BM), S), dim3(256), 0, s, M, N, K, A, lda,
+                           B, ldb, work, N, 0, Epilogue{}, kchunk, zstride);
     hipLaunchKernelGGL(k_splitk_reduce, dim3(blocks(zstride, 256)), dim3(256), 0, s, (const float*)work, C, M, N,
                        ldc, S, accumulate);
     return hipGetLastError() == hipSuccess ? 0 : -2;
@@ -536,7 +658,10 @@ int ws_gemm_bf16_act(int M, int N, int K, const uint16_t* A, int lda, const uint
     const int vb = (ldb % 8 == 0 && ((uintptr_t)B & 15) == 0) ? 1 : 0;
     Epilogue ep{};
     ep.bias = bias; ep.H = H; ep.ldh = ldh;
-    if (va && vb)
+    if (va && vb && K % DL_KB == 0)
+        hipLaunchKernelGGL((k_gemm_dl<1>), grid, dim3(256), 0, (hipStream_t)stream, M, N, K, A, lda, B, ldb,
+                           (float*)nullptr, 0, 0, ep, 0, 0L);
+    else if (va && vb)
         hipLaunchKernelGGL((k_gemm_kk<1>), grid, dim3(256), 0, (hipStream_t)stream, M, N, K, A, lda, B, ldb,
                            (float*)nullptr, 0, 0, ep, 0, 0L);
     else
@@ -559,7 +684,10 @@ int ws_gemm_bf16_start(int M, int nx, int ny, int K, const uint16_t* A, int lda,
     Epilogue ep{};
     ep.bias = bias; ep.scale = scale; ep.shift = shift; ep.lower = lower; ep.upper = upper;
     ep.x0 = x0; ep.y0 = y0; ep.nx = nx; ep.ny = ny; ep.neq = neq; ep.snap = snap;
-    if (va && vb)
+    if (va && vb && K % DL_KB == 0)
+        hipLaunchKernelGGL((k_gemm_dl<2>), grid, dim3(256), 0, (hipStream_t)stream, M, N, K, A, lda, B, ldb,
+                           (float*)nullptr, 0, 0, ep, 0, 0L);
+    else if (va && vb)
         hipLaunchKernelGGL((k_gemm_kk<2>), grid, dim3(256), 0, (hipStream_t)stream, M, N, K, A, lda, B, ldb,
                            (float*)nullptr, 0, 0, ep, 0, 0L);
     else
