@@ -586,7 +586,11 @@ __global__ void __launch_bounds__(BLOCK) k_check_rows(CheckRowsArgs<T> a) {
             const bool ineq = row >= a.neq;
             const double b = a.rhs[row * ld + b0];
             const double d = a.dr[row];
-            for (int which = 0; which < (a.with_avg ? 2 : 1); ++which) {
+            // unrolled so that the candidate index is a constant: the partials
+            // stay in registers (a dynamic index put them in scratch memory)
+#pragma unroll
+            for (int which = 0; which < 2; ++which) {
+                if (which == 1 && !a.with_avg) break;
                 double ax[1];
                 double yv;
                 if (which == 0) {
@@ -670,7 +674,10 @@ __global__ void __launch_bounds__(BLOCK) k_check_cols(CheckColsArgs<T> a) {
             const double lo = a.lower[col * ld + b0];
             const double hi = a.upper[col * ld + b0];
             const double d = a.dc[col];
-            for (int which = 0; which < (a.with_avg ? 2 : 1); ++which) {
+            // unrolled: the candidate index is a constant (partials in registers)
+#pragma unroll
+            for (int which = 0; which < 2; ++which) {
+                if (which == 1 && !a.with_avg) break;
                 double aty[1];
                 double xv;
                 if (which == 0) {

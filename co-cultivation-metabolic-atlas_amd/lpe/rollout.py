@@ -196,7 +196,9 @@ class ConsortiumRollout:
             info = eng.solve(opts)
             x, y, status = eng.state()
         certified = status == _lib.STATUS_OPTIMAL
-        torch.cuda.synchronize()
+        # the engine's work is joined to the caller's stream; waiting on that
+        # stream alone lets rollouts on other streams keep the device busy
+        torch.cuda.current_stream(self.device).synchronize()
         info["wall_seconds"] = time.perf_counter() - t0
         info["step"] = self.step_index
         info["start_mlp"] = start_mlp
