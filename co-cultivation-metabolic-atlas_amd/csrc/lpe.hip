@@ -1762,7 +1762,8 @@ int lpe_update(lpe_handle* h, const lpe_vectors* v, void* stream) {
     const long nc = h->shared_c ? n : B * n;
     const long nv = v->var_values ? B * h->nvar : 0;
     HIPCHK(hipMemsetAsync(h->d_err, 0, sizeof(int) * 8, h->s));
-    hipLaunchKernelGGL(k_validate, dim3(512), dim3(256), 0, h->s, v->c, nc, v->rhs, B * m, v->lower, v->upper, B * n,
+    // 2048 blocks (8 per CU) keep enough loads in flight to stream the inputs
+    hipLaunchKernelGGL(k_validate, dim3(2048), dim3(256), 0, h->s, v->c, nc, v->rhs, B * m, v->lower, v->upper, B * n,
                        v->var_values, nv, h->d_err);
     HIPCHK(hipMemcpyAsync(h->h_err, h->d_err, sizeof(int) * 8, hipMemcpyDeviceToHost, h->s));
     HIPCHK(hipStreamSynchronize(h->s));
@@ -1860,7 +1861,7 @@ int lpe_reset(lpe_handle* h, const double* x0, const double* y0, void* stream) {
     const long B = h->B, m = h->m, n = h->n, ld = h->ld;
     if (x0 || y0) {
         HIPCHK(hipMemsetAsync(h->d_err, 0, sizeof(int) * 8, h->s));
-        hipLaunchKernelGGL(k_validate, dim3(256), dim3(256), 0, h->s, x0, x0 ? B * n : 0, y0, y0 ? B * m : 0,
+        hipLaunchKernelGGL(k_validate, dim3(2048), dim3(256), 0, h->s, x0, x0 ? B * n : 0, y0, y0 ? B * m : 0,
                            (const double*)nullptr, (const double*)nullptr, 0L, (const double*)nullptr, 0L, h->d_err);
         HIPCHK(hipMemcpyAsync(h->h_err, h->d_err, sizeof(int) * 8, hipMemcpyDeviceToHost, h->s));
         HIPCHK(hipStreamSynchronize(h->s));

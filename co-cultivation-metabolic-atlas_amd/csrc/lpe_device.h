@@ -1295,15 +1295,17 @@ __global__ void __launch_bounds__(BLOCK) k_polish_cols_init(PolishColsInitArgs<T
         const int j = uniform(r0 + r);
         if (j >= a.sh.rows) break;
         const long i = j * ld + b0;
+        // the column's own operands are loaded before the gathers
+        const double cs = a.c_shared ? a.c[j] : a.c[i];
+        const double xraw = (double)a.x[i];
+        const double lo = a.lower[i], hi = a.upper[i];
         double aty[1];
         row_dot<double, T, double, 1>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.y, ld, b0, j, aty);
         const double d = a.dc[j];
-        const double cs = a.c_shared ? a.c[j] : a.c[i];
         const double rc = a.orig ? cs - aty[0] : (cs - aty[0]) / d;
         const double co = a.orig ? cs : cs / d;
         const double thr = a.tol_r * (1.0 + fabs(co));
-        double xv = a.orig ? (double)a.x[i] : (double)a.x[i] * d;
-        const double lo = a.lower[i], hi = a.upper[i];
+        double xv = a.orig ? xraw : xraw * d;
         double w = hi - lo;
         w = (isfinite(w) && w > 1.0) ? w : 1.0;
         uint8_t free_ = 1;
@@ -1354,24 +1356,44 @@ __global__ void __launch_bounds__(BLOCK) k_polish_rows_init(PolishRowsInitArgs<T
         for (int r = 0; r < a.sh.rpw; ++r) {
             const int row = uniform(r0 + r);
             if (row >= a.sh.rows) break;
+            const long i = row * ld + b0;
+            // the row's own operands are loaded before the gathers (their
+            // memory round trips overlap)
+            const double bb = a.rhs[i];
+            const double yraw = (double)a.y[i];
             double act[1];
             row_dot<double, double, double, 1>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.xs, ld, b0, row, act);
-            const long i = row * ld + b0;
-            const double bb = a.rhs[i];
             const double res = bb - act[0];
-            const double yv = a.orig ? (double)a.y[i] / a.dr[row] : (double)a.y[i];
+            const double yv = a.orig ? yraw / a.dr[row] : yraw;
             uint8_t on = 1;
             if (row >= a.neq) {
-                const double yo = a.orig ? (double)a.y[i] : yv * a.dr[row];
+                const double yo = a.orig ? yraw : yv * a.dr[row];
                 on = (yo < -a.tol_y || res <= a.tol_s * fmax(1.0, fabs(bb))) ? 1 : 0;
             }
             a.rmask[i] = on;
+            // Jacobi diagonal sum_k a^_ik^2 over the free columns, in storage
+            // order; the free-column masks (and per-LP values) of 8 entries
+            // are loaded together instead of one dependent load per entry
             double dg = 0.0;
             const int k1 = sload(a.As.ptr, row + 1);
-            for (int k = sload(a.As.ptr, row); k < k1; ++k) {
-                const int sl = sload(a.As.slot, k);
-                const double av = sl < 0 ? sload(a.As.val, k) : a.As.var[(long)sl * ld + b0];
-                if (a.fmask[(long)sload(a.As.idx, k) * ld + b0]) dg = dg + av * av;
+            for (int k0 = sload(a.As.ptr, row); k0 < k1; k0 += 8) {
+                const int cnt = k1 - k0 < 8 ? k1 - k0 : 8;
+                int cl[8], sl[8];
+                double av[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    if (u < cnt) { cl[u] = sload(a.As.idx, k0 + u); sl[u] = sload(a.As.slot, k0 + u); av[u] = sload(a.As.val, k0 + u); }
+                    else { cl[u] = 0; sl[u] = -1; av[u] = 0.0; }
+                }
+                uint8_t fm[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) fm[u] = u < cnt ? a.fmask[(long)cl[u] * ld + b0] : 0;
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (u < cnt && sl[u] >= 0) av[u] = a.As.var[(long)sl[u] * ld + b0];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (fm[u]) dg = dg + av[u] * av[u];
             }
             const double inv = (on && dg > 0.0) ? 1.0 / dg : 0.0;
             a.invdiag[i] = inv;
@@ -1767,10 +1789,12 @@ __global__ void __launch_bounds__(BLOCK) k_polish_final_cols(PolishFinalColsArgs
         const long i = col * ld + b0;
         double xv = a.xs[i];
         if (a.fmask[i]) {
+            // the bounds are loaded before the gathers (overlapping round trips)
+            const double lo = a.lower[i], hi = a.upper[i];
             double s[1];
             row_dot<double, double, double, 1>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.wp, ld, b0, col, s);
             xv = xv + a.dc[col] * s[0];
-            xv = fmin(fmax(xv, a.lower[i]), a.upper[i]);
+            xv = fmin(fmax(xv, lo), hi);
         }
         a.xout[i] = xv;
     }
