@@ -985,7 +985,7 @@ void cg_steps(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, in
     CgScalarArgs cs;
     cs.rho = h->d_rho; cs.rho0 = h->d_rho0; cs.alpha = h->d_alpha; cs.beta = h->d_beta;
     cs.act = h->d_cgact; cs.floor2 = h->d_cgfloor; cs.sel = v.sel; cs.rel = CG_REL; cs.batch = v.ld; cs.ld = v.ld;
-    cs.ptol2 = cg_ptol2_h(h);
+    cs.ptol2 = h->d_cgfloor + 2 * (long)v.ld;
     const bool pinf = cg_primal_inf();
     CgColsArgs c1;
     CgRowsArgs c2;
@@ -1080,10 +1080,16 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
         CgScalarArgs cs;
         cs.rho = h->d_rho; cs.rho0 = h->d_rho0; cs.alpha = h->d_alpha; cs.beta = h->d_beta;
         cs.act = h->d_cgact; cs.floor2 = h->d_cgfloor; cs.sel = v.sel; cs.rel = CG_REL; cs.batch = v.ld; cs.ld = v.ld;
-        cs.mode = 0; cs.d0 = h->d_red_b; cs.d1 = h->d_red_a; cs.plain = h->d_red_b + ld; cs.ptol2 = cg_ptol2_h(h);
+        cs.mode = 0; cs.d0 = h->d_red_b; cs.d1 = h->d_red_a; cs.plain = h->d_red_b + ld;
+        cs.ptol2 = h->d_cgfloor + 2 * ld;
+        // a second round (refinement from round 1's FP64 answer, orig) runs for
+        // LPs whose round-1 answer missed a gate by little: its dual floor is
+        // 100x lower and its primal stop 10x lower, since round 1's answer
+        // already meets round 1's stops and the CG would otherwise end at once
+        const double dscale = orig ? 1e-2 : 1.0, pscale2 = orig ? 1e-2 : 1.0;
         hipLaunchKernelGGL(k_cg_floor, dim3(ceil_div(v.ld, 256)), dim3(256), 0, h->s, (const double*)h->d_bnorm,
                            (const double*)h->d_cnorm, v.sel == h->d_sel ? (const int*)nullptr : (const int*)h->d_perm,
-                           h->d_cgfloor, v.ld);
+                           h->d_cgfloor, v.ld, dscale, cg_ptol2_h(h), pscale2);
         hipLaunchKernelGGL(k_cg_scalars, dim3(ceil_div(v.ld, 256)), dim3(256), 0, h->s, cs);
         std::vector<int> act(2 * ld);
         const int block_steps = cg_block_steps();
@@ -1505,7 +1511,7 @@ int alloc_all(lpe_handle* h) {
     CHK(dalloc(h, &h->d_q1, n * ld));
     CHK(dalloc(h, &h->d_invdiag, m * ld));
     CHK(dalloc(h, &h->d_rho, 2 * ld)); CHK(dalloc(h, &h->d_rho0, 2 * ld)); CHK(dalloc(h, &h->d_cgact, 2 * ld));
-    CHK(dalloc(h, &h->d_cgfloor, 2 * ld)); CHK(dalloc(h, &h->d_alpha, 2 * ld)); CHK(dalloc(h, &h->d_beta, 2 * ld));
+    CHK(dalloc(h, &h->d_cgfloor, 3 * ld)); CHK(dalloc(h, &h->d_alpha, 2 * ld)); CHK(dalloc(h, &h->d_beta, 2 * ld));
     CHK(dalloc(h, &h->d_xpol, n * ld)); CHK(dalloc(h, &h->d_ypol, m * ld));
     CHK(dalloc(h, &h->d_xres, n * ld)); CHK(dalloc(h, &h->d_yres, m * ld));
     CHK(dalloc(h, &h->d_xin, n * ld)); CHK(dalloc(h, &h->d_yin, m * ld));

@@ -1682,7 +1682,7 @@ struct CgScalarArgs {
     const double* d0;       // [ld] primal-system dot (rho or p.Mp)
     const double* d1;       // [ld] dual-system dot
     const double* plain;    // [ld] |r_p|^2 (modes 0 and 2)
-    double ptol2;           // primal stop: |r_p|^2 <= ptol2
+    const double* ptol2;    // [ld] primal stop: |r_p|^2 <= ptol2[b]
     double* rho;            // [2][ld]
     double* rho0;           // [2][ld]
     double* alpha;          // [2][ld]
@@ -1701,7 +1701,7 @@ __global__ void k_cg_scalars(CgScalarArgs a) {
     for (int s = 0; s < 2; ++s) {
         const long i = (long)s * a.ld + b;
         const double v = s == 0 ? a.d0[b] : a.d1[b];
-        const bool small = s == 0 && a.plain && a.plain[b] <= a.ptol2;
+        const bool small = s == 0 && a.plain && a.plain[b] <= a.ptol2[b];
         if (a.mode == 0) {
             a.rho[i] = v;
             a.rho0[i] = v;
@@ -2157,14 +2157,19 @@ __global__ void k_convert(const TI* __restrict__ in, TO* __restrict__ out, long 
 #ifndef LPE_CG_FLOOR_D
 #define LPE_CG_FLOOR_D 1e-11
 #endif
-__global__ void k_cg_floor(const double* bnorm, const double* cnorm, const int* perm, double* floor2, int ld) {
+// floor2[0, ld): the primal system's rho floor, [ld, 2 ld): the dual's,
+// [2 ld, 3 ld): the primal stop threshold ptol2 * pscale2.  dscale multiplies
+// the dual floor.
+__global__ void k_cg_floor(const double* bnorm, const double* cnorm, const int* perm, double* floor2, int ld,
+                           double dscale, double ptol2, double pscale2) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= ld) return;
     const int b = perm ? perm[j] : j;
     const double bn = b >= 0 ? bnorm[b] : 0.0, cn = b >= 0 ? cnorm[b] : 0.0;
-    const double fp = LPE_CG_FLOOR_P * (1.0 + bn), fd = LPE_CG_FLOOR_D * (1.0 + cn);
+    const double fp = LPE_CG_FLOOR_P * (1.0 + bn), fd = dscale * LPE_CG_FLOOR_D * (1.0 + cn);
     floor2[j] = fp * fp;
     floor2[ld + j] = fd * fd;
+    floor2[2 * ld + j] = ptol2 * pscale2;
 }
 
 // Per-LP primal weight: the given value, or ||c^|| / ||b^|| (the PDLP default
