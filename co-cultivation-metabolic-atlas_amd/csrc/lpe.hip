@@ -168,6 +168,7 @@ struct lpe_handle {
     long cg_steps_total = 0;   // CG steps of every polish round so far
     long cg_lp_steps_total = 0;  // sum over CG blocks of (LPs with an active system) x block steps
     double cg_ptol2 = 0.0;    // primal CG stop: |Dr (b - A x)|_2^2 <= (1e-8 min(dr))^2
+    double cg_gate = 1e-5;    // the certificate's primal gate of the polish in progress
     double *d_xpol = nullptr, *d_ypol = nullptr, *d_xres = nullptr, *d_yres = nullptr;
     double* d_metrics = nullptr;
     double *d_xin = nullptr, *d_yin = nullptr;    // user iterates for lpe_certify / lpe_spmv
@@ -954,9 +955,12 @@ double cg_primal_tol() {
     const double t = v ? std::atof(v) : 0.0;
     return (t > 0.0 && t < 1.0) ? t : LPE_CG_PRIMAL_TOL;
 }
+// The primal stop scales with the certificate's primal gate: 3e-6 for the
+// reference's 1e-5, 3e-8 for a caller's 1e-7.
 double cg_ptol2_h(const lpe_handle* h) {
-    if (!cg_primal_inf()) return h->cg_ptol2;
-    const double t = cg_primal_tol();
+    const double g = (h->cg_gate > 0.0 && h->cg_gate < 1e-5) ? h->cg_gate / 1e-5 : 1.0;
+    if (!cg_primal_inf()) return h->cg_ptol2 * g * g;
+    const double t = cg_primal_tol() * g;
     return t * t;
 }
 
@@ -1337,6 +1341,7 @@ void mark_full_certificates(lpe_handle* h, const std::vector<int>& passed) {
 
 int polish_selected(lpe_handle* h, int cg_iters, double tol, double ep, double ed, double eg) {
     const bool f32 = h->method != LPE_METHOD_FIXED && h->precision == LPE_PREC_FP32;
+    h->cg_gate = ep;
     std::vector<int> idx;
     for (int b = 0; b < h->B; ++b)
         if (h->h_sel[b]) idx.push_back(b);
