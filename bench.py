@@ -69,18 +69,20 @@ def _cg_cols_bytes(B, m, n, nnz, nvar, n_rb):
 def _cg_rows_bytes(B, m, n, nnz, nvar, n_rb):
     """One CG rows pass with the update, both systems, every LP active: q
     gathered once (2 n words), p read (2 m), w and r of both systems read and
-    written (8 m), the Jacobi diagonal (m) and row mask (m bytes), per-LP
-    coefficients, alpha / activity / selection, three partials per row block;
-    the shared CSR structure, values and row scales once."""
-    per_lp = (2 * n + 11 * m) * 8 + m + nvar * 8 + 16 + 12 + 3 * n_rb * 8
+    written (8 m), the FP32 Jacobi diagonal (4 m bytes; its support is the
+    row mask), per-LP coefficients, alpha / activity / selection, three
+    partials per row block; the shared CSR structure, values and row scales
+    once."""
+    per_lp = (2 * n + 10 * m) * 8 + 4 * m + nvar * 8 + 16 + 12 + 3 * n_rb * 8
     return B * per_lp + nnz * 16 + (m + 1) * 4 + m * 8
 
 
 def _cg_direction_bytes(B, m):
     """One CG direction update p = D^-1 r + beta p, both systems, every LP
-    active: r1, r2 and the Jacobi diagonal read, p (interleaved) read and
-    written (7 m FP64 words per LP), beta / activity / selection per LP."""
-    return B * (7 * m * 8 + 2 * 8 + 12)
+    active: r1, r2 read, p (interleaved) read and written (6 m FP64 words per
+    LP), the FP32 Jacobi diagonal (4 m bytes), beta / activity / selection
+    per LP."""
+    return B * (6 * m * 8 + 4 * m + 2 * 8 + 12)
 
 
 def _cg_step_bytes(m, n, nnz, nvar, ld):

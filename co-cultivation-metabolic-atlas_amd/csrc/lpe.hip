@@ -161,7 +161,8 @@ struct lpe_handle {
     double2* d_pcg = nullptr;                   // CG directions, interleaved (p1, p2) [m][ld]
     double2* d_qcg = nullptr;                   // CG products, interleaved (q1, q2) [n][ld]
     uint8_t *d_fmask = nullptr, *d_rmask = nullptr;
-    double *d_rho = nullptr, *d_alpha = nullptr, *d_beta = nullptr, *d_invdiag = nullptr, *d_rho0 = nullptr;
+    double *d_rho = nullptr, *d_alpha = nullptr, *d_beta = nullptr, *d_rho0 = nullptr;
+    float* d_invdiag = nullptr;
     int* d_cgact = nullptr;
     double* d_cgfloor = nullptr;
     int last_cg_steps = 0;
@@ -973,7 +974,7 @@ void cg_args(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, CgC
     c1.fmask = h->d_fmask; c1.sel = v.sel; c1.act = h->d_cgact; c1.partial = h->d_partial;
     c2.sh = sr; c2.A = op_scaled_v(h, false, v); c2.q = h->d_qcg; c2.p = h->d_pcg;
     c2.w1 = h->d_wp; c2.r1 = h->d_rp_; c2.w2 = h->d_wd; c2.r2 = h->d_rd;
-    c2.alpha = h->d_alpha; c2.invdiag = h->d_invdiag; c2.rmask = h->d_rmask;
+    c2.alpha = h->d_alpha; c2.invdiag = h->d_invdiag;
     c2.dr = (pinf || LPE_CG_PRIMAL_WEIGHTED) ? h->d_dr : nullptr; c2.sel = v.sel; c2.act = h->d_cgact;
     c2.partial = h->d_partial; c2.inf_norm = pinf ? 1 : 0; c2.reverse = rows_reverse();
     dd.sh = make_shape(std::max(h->m, 1), v.ld, v.ld, 1, 1, &grid_d);

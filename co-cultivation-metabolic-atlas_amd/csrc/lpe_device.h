@@ -19,6 +19,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <float.h>
 
 namespace lpe {
 
@@ -1333,7 +1334,8 @@ struct PolishRowsInitArgs {
     uint8_t* rmask;
     double* rp; double2* p; double* wp;   // p: the CG directions, interleaved (p1, p2); writes p1
     double* wd;
-    double* invdiag;         // [m][ld] 1 / (R A^ D A^T R)_ii, 0 on inactive rows
+    float* invdiag;          // [m][ld] 1 / (R A^ D A^T R)_ii in FP32, 0 on inactive rows and on
+                             // active rows without a free column
     int neq;
     double tol_y, tol_s;
     int plain_orig;          // 1: |r|^2 in original units (r / dr); 2: max_i r_i^2 in original
@@ -1395,8 +1397,14 @@ __global__ void __launch_bounds__(BLOCK) k_polish_rows_init(PolishRowsInitArgs<T
                 for (int u = 0; u < 8; ++u)
                     if (fm[u]) dg = dg + av[u] * av[u];
             }
-            const double inv = (on && dg > 0.0) ? 1.0 / dg : 0.0;
-            a.invdiag[i] = inv;
+            // The Jacobi preconditioner is stored in FP32 (any positive diagonal
+            // is a valid preconditioner; every use reads the same rounded value):
+            // half the preconditioner's bytes in the CG rows and direction passes.
+            // A positive entry stays positive and finite.
+            float ivf = (on && dg > 0.0) ? (float)(1.0 / dg) : 0.0f;
+            if (on && dg > 0.0) ivf = fminf(fmaxf(ivf, FLT_MIN), FLT_MAX);
+            a.invdiag[i] = ivf;
+            const double inv = (double)ivf;
             const double rv = on ? a.dr[row] * res : 0.0;
             a.rp[i] = rv; a.p[i].x = rv * inv; a.wp[i] = 0.0;
             a.wd[i] = on ? yv : 0.0;
@@ -1463,7 +1471,7 @@ struct PolishRowsArgs {
     const double* q1;
     double* setup_r;
     double2* setup_p;        // interleaved CG directions: writes p2
-    const double* invdiag;
+    const float* invdiag;
     const uint8_t* rmask;
     const int* sel;
     double* partial;         // [n_rb][2][ld]
@@ -1499,7 +1507,7 @@ __global__ void __launch_bounds__(BLOCK) k_polish_rows(PolishRowsArgs a) {
             for (int t = 0; t < LPL; ++t) {
                 if (sel[t]) {
                     const double sv1 = (a.rmask[i + t] ? 1.0 : 0.0) * t1[t];
-                    const double z = sv1 * a.invdiag[i + t];
+                    const double z = sv1 * (double)a.invdiag[i + t];
                     a.setup_r[i + t] = sv1; a.setup_p[i + t].y = z;
                     v[0][t] += sv1 * z;
                 }
@@ -1589,8 +1597,8 @@ struct CgRowsArgs {
     double* w1; double* r1;  // [m][ld], primal system
     double* w2; double* r2;  // dual system
     const double* alpha;     // [2][ld]
-    const double* invdiag;   // Jacobi preconditioner, 0 off the active rows
-    const uint8_t* rmask;
+    const float* invdiag;    // Jacobi preconditioner, 0 off the active rows (and on
+                             // active rows without a free column, whose A^ q is 0)
     const double* dr;        // row scale: the primal stop measure in original units (r / dr), or nullptr (scaled)
     const int* sel;
     const int* act;          // [2][ld]
@@ -1639,8 +1647,10 @@ __global__ void __launch_bounds__(WPB * WAVE) k_cg_rows(CgRowsArgs a) {
                 on[t] = 0.0; iv[t] = 0.0; w1v[t] = 0.0; r1v[t] = 0.0; w2v[t] = 0.0; r2v[t] = 0.0;
                 pv[t] = make_double2(0.0, 0.0);
                 if (!(on1[t] || on2[t])) continue;
-                on[t] = a.rmask[i + t] ? 1.0 : 0.0;
-                iv[t] = a.invdiag[i + t];
+                // the row mask is the preconditioner's support: a row off it is
+                // inactive or has no free column (its R A^ q is 0 either way)
+                iv[t] = (double)a.invdiag[i + t];
+                on[t] = iv[t] != 0.0 ? 1.0 : 0.0;
                 pv[t] = a.p[i + t];
                 if (on1[t]) { w1v[t] = a.w1[i + t]; r1v[t] = a.r1[i + t]; }
                 if (on2[t]) { w2v[t] = a.w2[i + t]; r2v[t] = a.r2[i + t]; }
@@ -1730,7 +1740,7 @@ struct CgDirArgs {
     double2* p;             // interleaved (p1, p2)
     const double* r1; const double* r2;
     const double* beta;
-    const double* invdiag;
+    const float* invdiag;
     const int* sel;
     const int* act;
     int reverse;            // walk the LP tiles last to first
@@ -1754,7 +1764,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_direction(CgDirArgs a) {
         const int row = r0 + r;
         if (row >= a.sh.rows) break;
         const long i = row * ld + b;
-        const double iv = a.invdiag[i];
+        const double iv = (double)a.invdiag[i];
         double2 pv = a.p[i];
         if (on1) pv.x = a.r1[i] * iv + be1 * pv.x;
         if (on2) pv.y = a.r2[i] * iv + be2 * pv.y;
