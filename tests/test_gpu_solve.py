@@ -270,3 +270,39 @@ def test_polish_matches_oracle_active_set_least_squares(device):
         assert status[b] == _lib.STATUS_OPTIMAL, (b, status)
         got = float(md.c @ xg[b])
         assert abs(got - obj) <= 1e-9 * max(1.0, abs(obj)), (b, got, obj)
+
+
+def test_status_equals_a_fresh_certificate_after_whole_batch_polish(device):
+    """A solve whose LPs are all accepted by one polish of the whole batch
+    does not certify its answers a second time (that polish certificate is
+    the same computation over the same data).  The statuses it returns must
+    equal a fresh device certificate of the returned (x, y) and the host
+    certificate - also when some LPs are not started near their optimum and
+    the final certificate does run."""
+    from lpe.engine import SolveOptions
+    # 48 LPs in a 64-wide layout: polishing more than half of it runs on the
+    # whole batch (fewer LPs run on a compacted view and are certified again)
+    ub = _uptake(batch=48)
+    refs = [highs.solve(*ub.problem(b)) for b in range(ub.batch)]
+    x0 = np.array([r.x for r in refs])
+    y0 = np.array([r.y for r in refs])
+    for far in (False, True):
+        eng = _engine(ub.S, ub.neq, ub.batch, precision="fp32", shared_c=True)
+        _load_uptake(eng, ub)
+        xs, ys = x0.copy(), y0.copy()
+        if far:
+            xs[1::3] = 0.0          # a third of the batch starts at the origin
+            ys[1::3] = 0.0
+        eng.reset(xs, ys)
+        opts = SolveOptions(max_iterations=20000, check_interval=64, polish=True, polish_kkt=1e-4,
+                            polish_kkt_start=1e-4)
+        info = eng.solve(opts)
+        x, y, status = eng.state()
+        assert info["certified"] == ub.batch, (far, info)
+        fresh = eng.certify(x, y).cpu().numpy()
+        xn, yn = x.cpu().numpy(), y.cpu().numpy()
+        for b in range(ub.batch):
+            host = cert.certificate(*ub.problem(b), xn[b], yn[b])
+            assert (status[b] == 1) == bool(fresh[b, 4]) == host["certificate_passed"], (far, b)
+            assert cert.relative_objective_error(host["objective"], refs[b].fun) <= REL_OBJ_TOL
+        eng.close()
