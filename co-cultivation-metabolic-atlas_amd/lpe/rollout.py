@@ -119,6 +119,7 @@ class ConsortiumRollout:
         self.log: list[dict] = []
         self.track_active = False   # record per-step active-set changes in the log
         self._active = None
+        self._inputs = None         # (rhs, lower, upper) reused by step()
 
     def lp_inputs(self, conc=None, biomass=None):
         """(rhs [B,m], lower [B,n], upper [B,n], coupling values [B,3*K]) on the device."""
@@ -139,6 +140,28 @@ class ConsortiumRollout:
         coupling = -torch.clamp(biomass, min=1e-12)[:, self._term_species]
         return rhs, lower, upper, coupling.contiguous()
 
+    def _step_inputs(self):
+        """``lp_inputs()`` for the step, into buffers kept across steps: only
+        the Michaelis-Menten uptake bounds and the medium rows change, so they
+        are rewritten in place instead of rebuilding the [B, n] and [B, m]
+        arrays (the engine copies its inputs at ``update``; ``lp_inputs``
+        still returns fresh arrays to other callers)."""
+        if self._inputs is None:
+            rhs, lower, upper, coupling = self.lp_inputs()
+            self._inputs = (rhs, lower, upper)
+            return rhs, lower, upper, coupling
+        rhs, lower, upper = self._inputs
+        md = self.model
+        t = self._mm_terms
+        s, i = self._term_species[t], self._term_medium[t]
+        ci = self.conc[:, i]
+        mm = -self._vmax[s, i] * ci / (self._km[s, i] + ci)
+        lower[:, self._term_cols[t]] = torch.clamp(mm, min=-BOUND)
+        rhs[:, md.neq:md.neq + self.conc.shape[1]] = torch.where(self._unlimited, UNLIMITED_SUPPLY,
+                                                                  self.conc / self.dt)
+        coupling = -torch.clamp(self.biomass, min=1e-12)[:, self._term_species]
+        return rhs, lower, upper, coupling.contiguous()
+
     def euler(self, x: torch.Tensor, mask: Optional[torch.Tensor] = None):
         """Medium and biomass update from the solved fluxes (skipped for LPs
         that did not certify, whose environment keeps its state)."""
@@ -157,7 +180,7 @@ class ConsortiumRollout:
         self.conc, self.biomass = conc, biomass
 
     def step(self, *, advance: bool = True) -> dict:
-        rhs, lower, upper, coupling = self.lp_inputs()
+        rhs, lower, upper, coupling = self._step_inputs()
         eng = self.engine
         t0 = time.perf_counter()
         eng.update(self.c, rhs, lower, upper, var_values=coupling)
@@ -207,8 +230,11 @@ class ConsortiumRollout:
         self.log.append(info)
         if advance:
             ok = torch.as_tensor(certified, device=self.device)
-            self.x = torch.where(ok[:, None], x, self.x) if self.x is not None else x
-            self.y = torch.where(ok[:, None], y, self.y) if self.y is not None else y
+            if self.x is None or bool(np.all(certified)):
+                self.x, self.y = x, y
+            else:
+                self.x = torch.where(ok[:, None], x, self.x)
+                self.y = torch.where(ok[:, None], y, self.y)
             self.euler(x, ok)
             self.omega = eng.primal_weight()
             self.step_index += 1
