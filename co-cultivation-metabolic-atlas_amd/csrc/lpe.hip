@@ -160,6 +160,7 @@ struct lpe_handle {
     double* d_q1 = nullptr;                     // dual set-up product (and lpe_spmv scratch)
     double2* d_pcg = nullptr;                   // CG directions, interleaved (p1, p2) [m][ld]
     double2* d_qcg = nullptr;                   // CG products, interleaved (q1, q2) [n][ld]
+    double* d_p1 = nullptr;                     // primal-only CG directions [m][ld] (after the dual stops)
     uint8_t *d_fmask = nullptr, *d_rmask = nullptr;
     double *d_rho = nullptr, *d_alpha = nullptr, *d_beta = nullptr, *d_rho0 = nullptr;
     float* d_invdiag = nullptr;
@@ -527,6 +528,13 @@ int cg_lpl(int ld) {
     return (k == 2 && ld >= 128) ? 2 : 1;
 }
 
+// Primal-only CG passes once every dual system stopped (LPE_CG_SPLIT=0 turns
+// them off, for comparison; the answers are the same).
+bool cg_split_enabled() {
+    const char* v = std::getenv("LPE_CG_SPLIT");
+    return !(v && v[0] == '0');
+}
+
 // Gather depth of the CG passes (sparse entries whose gathers are in flight
 // together per wave): 4 (54-63 VGPRs at one LP per lane, 8 waves per SIMD;
 // measured at 4096 LPs: columns pass 0.288 against 0.362 ms at depth 8, rows
@@ -608,6 +616,37 @@ int launch_cg_rows(lpe_handle* h, const CgRowsArgs& a, int ldv) {
     } else {
         if (d == 8) launch_cg_rows_w<1, 8>(h, b, grid, waves); else launch_cg_rows_w<1, 4>(h, b, grid, waves);
     }
+    return b.sh.n_rb;
+}
+
+// Primal-only CG passes (one LP per lane, after every dual system stopped).
+template <int D>
+void launch_cg_cols_p_w(lpe_handle* h, const CgColsPArgs& b, int grid, int waves) {
+    if (waves == 16) hipLaunchKernelGGL((k_cg_cols_p<D, 16>), dim3(grid), dim3(16 * WAVE), 0, h->s, b);
+    else hipLaunchKernelGGL((k_cg_cols_p<D, 4>), dim3(grid), dim3(4 * WAVE), 0, h->s, b);
+}
+
+template <int D>
+void launch_cg_rows_p_w(lpe_handle* h, const CgRowsPArgs& b, int grid, int waves) {
+    if (waves == 16) hipLaunchKernelGGL((k_cg_rows_p<D, 16>), dim3(grid), dim3(16 * WAVE), 0, h->s, b);
+    else hipLaunchKernelGGL((k_cg_rows_p<D, 4>), dim3(grid), dim3(4 * WAVE), 0, h->s, b);
+}
+
+int launch_cg_cols_p(lpe_handle* h, const CgColsPArgs& a, int ldv) {
+    int grid;
+    CgColsPArgs b = a;
+    const int waves = cg_waves();
+    b.sh = make_shape_waves(a.sh.rows, ldv, ldv, cg_rpw(ldv), 1, waves, &grid);
+    if (cg_gather_depth() == 8) launch_cg_cols_p_w<8>(h, b, grid, waves); else launch_cg_cols_p_w<4>(h, b, grid, waves);
+    return b.sh.n_rb;
+}
+
+int launch_cg_rows_p(lpe_handle* h, const CgRowsPArgs& a, int ldv) {
+    int grid;
+    CgRowsPArgs b = a;
+    const int waves = cg_waves();
+    b.sh = make_shape_waves(a.sh.rows, ldv, ldv, cg_rpw(ldv), 1, waves, &grid);
+    if (cg_gather_depth() == 8) launch_cg_rows_p_w<8>(h, b, grid, waves); else launch_cg_rows_p_w<4>(h, b, grid, waves);
     return b.sh.n_rb;
 }
 
@@ -1010,22 +1049,58 @@ void cg_steps(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, in
     }
 }
 
+// `steps` primal-only CG steps (every dual system has stopped): the same
+// sequence as cg_steps on the contiguous p1 (d_p1) and q1 (d_q1, free after
+// the dual set-up).
+void cg_steps_p(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, int steps) {
+    CgScalarArgs cs;
+    cs.rho = h->d_rho; cs.rho0 = h->d_rho0; cs.alpha = h->d_alpha; cs.beta = h->d_beta;
+    cs.act = h->d_cgact; cs.floor2 = h->d_cgfloor; cs.sel = v.sel; cs.rel = CG_REL; cs.batch = v.ld; cs.ld = v.ld;
+    cs.ptol2 = h->d_cgfloor + 2 * (long)v.ld;
+    const bool pinf = cg_primal_inf();
+    CgColsPArgs c1;
+    c1.sh = sc; c1.AT = op_scaled_v(h, true, v); c1.p1 = h->d_p1; c1.q1 = h->d_q1; c1.fmask = h->d_fmask;
+    c1.sel = v.sel; c1.act = h->d_cgact; c1.partial = h->d_partial;
+    CgRowsPArgs c2;
+    c2.sh = sr; c2.A = op_scaled_v(h, false, v); c2.q1 = h->d_q1; c2.p1 = h->d_p1; c2.w1 = h->d_wp; c2.r1 = h->d_rp_;
+    c2.alpha = h->d_alpha; c2.invdiag = h->d_invdiag; c2.dr = (pinf || LPE_CG_PRIMAL_WEIGHTED) ? h->d_dr : nullptr;
+    c2.sel = v.sel; c2.act = h->d_cgact; c2.partial = h->d_partial; c2.inf_norm = pinf ? 1 : 0;
+    c2.reverse = rows_reverse();
+    int grid_d = 0;
+    CgDirPArgs dd;
+    dd.sh = make_shape(std::max(h->m, 1), v.ld, v.ld, 1, 1, &grid_d);
+    dd.p1 = h->d_p1; dd.r1 = h->d_rp_; dd.beta = h->d_beta; dd.invdiag = h->d_invdiag; dd.sel = v.sel;
+    dd.act = h->d_cgact; dd.reverse = dir_reverse();
+    for (int k = 0; k < steps; ++k) {
+        const int nrb1 = launch_cg_cols_p(h, c1, v.ld);
+        reduce_ld(h, h->d_partial, h->d_red_a, h->d_ops_sum, 2, nrb1, v.ld);
+        cs.mode = 1; cs.d0 = h->d_red_a; cs.d1 = h->d_red_a + v.ld; cs.plain = nullptr;
+        hipLaunchKernelGGL(k_cg_scalars, dim3(ceil_div(v.ld, 256)), dim3(256), 0, h->s, cs);
+        const int nrb2 = launch_cg_rows_p(h, c2, v.ld);
+        reduce_ld(h, h->d_partial, h->d_red_a, pinf ? h->d_ops_pinf : h->d_ops_sum, 3, nrb2, v.ld);
+        cs.mode = 2; cs.d0 = h->d_red_a; cs.d1 = h->d_red_a + v.ld; cs.plain = h->d_red_a + 2 * (long)v.ld;
+        hipLaunchKernelGGL(k_cg_scalars, dim3(ceil_div(v.ld, 256)), dim3(256), 0, h->s, cs);
+        hipLaunchKernelGGL(k_cg_direction_p, dim3(grid_d), dim3(BLOCK), 0, h->s, dd);
+    }
+}
+
 // cg_block_steps() preconditioned-CG steps replayed from a hipGraph captured once per
 // view width and launch shape (the kernels read the selection and all vectors
 // from fixed buffers, so the graph is reused across attempts).
-int cg_block(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, int grid_r, int grid_c) {
-    const int steps = cg_block_steps();
+int cg_block(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, int grid_r, int grid_c, int steps,
+             bool primal_only) {
     const long long key = (long long)v.ld | ((long long)sr.rpw << 32) | ((long long)cg_lpl(v.ld) << 40) |
                           ((long long)steps << 44) | (v.sel == h->d_sel ? 0LL : 1LL << 52) |
                           ((long long)(cg_gather_depth() == 8 ? 1 : 0) << 53) |
                           ((long long)(cg_waves() == 16 ? 1 : 0) << 54) | ((long long)(cg_rpw(v.ld) - 1) << 55) |
                           ((long long)rows_reverse() << 60) | ((long long)dir_reverse() << 61) |
-                          ((long long)(cg_primal_inf() ? 1 : 0) << 51);
+                          ((long long)(cg_primal_inf() ? 1 : 0) << 51) | ((long long)(primal_only ? 1 : 0) << 62);
     hipGraphExec_t& g_exec = h->cg_graphs[key];
     if (!g_exec) {
         hipGraph_t g;
         HIPCHK(hipStreamBeginCapture(h->s, hipStreamCaptureModeThreadLocal));
-        cg_steps(h, v, sr, sc, steps);
+        if (primal_only) cg_steps_p(h, v, sr, sc, steps);
+        else cg_steps(h, v, sr, sc, steps);
         HIPCHK(hipStreamEndCapture(h->s, &g));
         hipError_t e = hipGraphInstantiate(&g_exec, g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
@@ -1098,13 +1173,22 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
         hipLaunchKernelGGL(k_cg_scalars, dim3(ceil_div(v.ld, 256)), dim3(256), 0, h->s, cs);
         std::vector<int> act(2 * ld);
         const int block_steps = cg_block_steps();
-        const int blocks = (cg_iters + block_steps - 1) / block_steps;
+        // The dual system stops before the primal one; once no LP has an
+        // active dual system the remaining steps run the primal-only passes on
+        // a contiguous p1 (half the gathered and streamed vector bytes).  After
+        // the first two blocks the blocks are half as long until that switch,
+        // so that it comes sooner.
+        const bool can_split = cg_lpl(v.ld) == 1 && h->m > 0 && cg_split_enabled();
+        bool split = false;
         int used = 0;
-        for (int k = 0; k < blocks; ++k) {
+        for (int k = 0; used < cg_iters; ++k) {
             HIPCHK(hipMemcpyAsync(act.data(), h->d_cgact, sizeof(int) * 2 * ld, hipMemcpyDeviceToHost, h->s));
             HIPCHK(hipStreamSynchronize(h->s));
-            long n_active = 0;
-            for (long q = 0; q < ld; ++q) n_active += (act[q] != 0 || act[ld + q] != 0) ? 1 : 0;
+            long n_active = 0, n_dual = 0;
+            for (long q = 0; q < ld; ++q) {
+                n_active += (act[q] != 0 || act[ld + q] != 0) ? 1 : 0;
+                n_dual += act[ld + q] != 0 ? 1 : 0;
+            }
             if (k > 0) {
                 const bool any = n_active > 0;
                 if (std::getenv("LPE_TRACE_CG")) {
@@ -1114,10 +1198,19 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
                                  na[0], na[1], ld);
                 }
                 if (!any) break;
+                if (can_split && !split && n_dual == 0) {
+                    const long count = (long)h->m * ld;
+                    hipLaunchKernelGGL(k_cg_split, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, h->s,
+                                       (const double2*)h->d_pcg, h->d_p1, count);
+                    HIPCHK(hipGetLastError());
+                    split = true;
+                }
             }
-            CHK(cg_block(h, v, sr, sc, grid_r, grid_c));
-            used += block_steps;
-            h->cg_lp_steps_total += n_active * block_steps;
+            const int steps = (can_split && !split && used >= 2 * block_steps && block_steps >= 2) ? block_steps / 2
+                                                                                                  : block_steps;
+            CHK(cg_block(h, v, sr, sc, grid_r, grid_c, steps, split));
+            used += steps;
+            h->cg_lp_steps_total += n_active * steps;
             if (std::getenv("LPE_TRACE_CG")) {
                 // residual history of the first selected LP (both systems)
                 HIPCHK(hipStreamSynchronize(h->s));
@@ -1513,7 +1606,7 @@ int alloc_all(lpe_handle* h) {
     CHK(dalloc(h, &h->d_xs, n * ld)); CHK(dalloc(h, &h->d_fmask, n * ld)); CHK(dalloc(h, &h->d_rmask, m * ld));
     CHK(dalloc(h, &h->d_rp_, m * ld)); CHK(dalloc(h, &h->d_wp, m * ld));
     CHK(dalloc(h, &h->d_rd, m * ld)); CHK(dalloc(h, &h->d_wd, m * ld));
-    CHK(dalloc(h, &h->d_pcg, m * ld)); CHK(dalloc(h, &h->d_qcg, n * ld));
+    CHK(dalloc(h, &h->d_pcg, m * ld)); CHK(dalloc(h, &h->d_qcg, n * ld)); CHK(dalloc(h, &h->d_p1, m * ld));
     CHK(dalloc(h, &h->d_q1, n * ld));
     CHK(dalloc(h, &h->d_invdiag, m * ld));
     CHK(dalloc(h, &h->d_rho, 2 * ld)); CHK(dalloc(h, &h->d_rho0, 2 * ld)); CHK(dalloc(h, &h->d_cgact, 2 * ld));

@@ -1772,6 +1772,148 @@ __global__ void __launch_bounds__(BLOCK) k_cg_direction(CgDirArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Primal-only CG passes.  The dual least-squares system stops several steps
+// before the primal correction (in steady rollout steps: after about 25 of 40
+// steps, for every LP); from then on the interleaved (p1, p2) / (q1, q2)
+// vectors would move twice the bytes the primal system needs.  Once no LP
+// has an active dual system, p1 is copied out of the interleaved p
+// (k_cg_split) and the remaining steps run these kernels on the contiguous
+// p1, q1.  Each computes the primal half of its two-system counterpart with
+// the same operations in the same order (row_dot accumulates exactly as
+// row_dot2 does per system), so the primal answer does not change; the dual
+// partials are written as 0.
+// ---------------------------------------------------------------------------
+__global__ void k_cg_split(const double2* __restrict__ p, double* __restrict__ p1, long count) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p1[i] = p[i].x;
+}
+
+struct CgColsPArgs {
+    Shape sh;
+    Op<double> AT;           // scaled FP64
+    const double* p1;        // [m][ld]
+    double* q1;              // [n][ld]
+    const uint8_t* fmask;
+    const int* sel;
+    const int* act;          // [2][ld] (system 0 read)
+    double* partial;         // [n_rb][2][ld]: |q1|^2, 0
+};
+
+template <int D, int WPB>
+__global__ void __launch_bounds__(WPB * WAVE) k_cg_cols_p(CgColsPArgs a) {
+    __shared__ double lds[WPB * 2 * WAVE];
+    int tile, rb;
+    if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int b0 = tile * WAVE + lane;
+    const bool valid = b0 < a.sh.ld;
+    const long ld = a.sh.ld;
+    double v[2][1] = {{0.0}, {0.0}};
+    const bool on = valid && a.sel[b0] != 0 && a.act[b0];
+    if (on) {
+        const int r0 = (rb * WPB + wave) * a.sh.rpw;
+        for (int r = 0; r < a.sh.rpw; ++r) {
+            const int col = uniform(r0 + r);
+            if (col >= a.sh.rows) break;
+            const long i = col * ld + b0;
+            const double f = a.fmask[i] ? 1.0 : 0.0;
+            double s1[1];
+            row_dot<double, double, double, 1, D>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.p1, ld, b0,
+                                                  col, s1);
+            const double q = f * s1[0];
+            v[0][0] += q * q;
+            a.q1[i] = q;
+        }
+    }
+    const int ops[2] = {RSUM, RSUM};
+    block_partials<2, 1, WPB>(v, ops, a.partial, rb, ld, b0, lds, valid);
+}
+
+struct CgRowsPArgs {
+    Shape sh;
+    Op<double> A;            // scaled FP64
+    const double* q1;        // [n][ld]
+    const double* p1;        // [m][ld]
+    double* w1; double* r1;  // [m][ld]
+    const double* alpha;     // [2][ld] (system 0 read)
+    const float* invdiag;
+    const double* dr;        // row scale for the primal stop measure, or nullptr
+    const int* sel;
+    const int* act;
+    double* partial;         // [n_rb][3][ld]: r_p.D^-1 r_p, 0, primal stop measure
+    int inf_norm;
+    int reverse;
+};
+
+template <int D, int WPB>
+__global__ void __launch_bounds__(WPB * WAVE) k_cg_rows_p(CgRowsPArgs a) {
+    __shared__ double lds[WPB * 3 * WAVE];
+    int tile, rb;
+    if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
+    if (a.reverse) tile = a.sh.n_tiles - 1 - tile;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int b0 = tile * WAVE + lane;
+    const bool valid = b0 < a.sh.ld;
+    const long ld = a.sh.ld;
+    double v[3][1] = {{0.0}, {0.0}, {0.0}};
+    const bool on1 = valid && a.sel[b0] != 0 && a.act[b0];
+    const double al = on1 ? a.alpha[b0] : 0.0;
+    if (on1) {
+        const int r0 = (rb * WPB + wave) * a.sh.rpw;
+        for (int r = 0; r < a.sh.rpw; ++r) {
+            const int row = uniform(r0 + r);
+            if (row >= a.sh.rows) break;
+            const long i = row * ld + b0;
+            const double drow = a.dr ? a.dr[row] : 1.0;
+            const double iv = (double)a.invdiag[i];
+            const double on = iv != 0.0 ? 1.0 : 0.0;
+            const double pv = a.p1[i], wv = a.w1[i], rv = a.r1[i];
+            double t1[1];
+            row_dot<double, double, double, 1, D>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.q1, ld, b0, row,
+                                                  t1);
+            a.w1[i] = wv + al * pv;
+            const double n1 = rv - al * (on * t1[0]);
+            a.r1[i] = n1;
+            v[0][0] += n1 * n1 * iv;
+            const double u = a.dr ? n1 / drow : n1;
+            if (a.inf_norm) v[2][0] = fmax(v[2][0], u * u);
+            else v[2][0] += u * u;
+        }
+    }
+    const int ops[3] = {RSUM, RSUM, a.inf_norm ? RMAX : RSUM};
+    block_partials<3, 1, WPB>(v, ops, a.partial, rb, ld, b0, lds, valid);
+}
+
+struct CgDirPArgs {
+    Shape sh;               // rows = m; one row per wave, 64 LPs per wave
+    double* p1;
+    const double* r1;
+    const double* beta;
+    const float* invdiag;
+    const int* sel;
+    const int* act;
+    int reverse;
+};
+
+__global__ void __launch_bounds__(BLOCK) k_cg_direction_p(CgDirPArgs a) {
+    int tile, rb;
+    if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
+    if (a.reverse) tile = a.sh.n_tiles - 1 - tile;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int b = tile * WAVE + lane;
+    if (b >= a.sh.ld || !a.sel[b] || !a.act[b]) return;
+    const long ld = a.sh.ld;
+    const double be = a.beta[b];
+    const int r0 = (rb * WAVES_PER_BLOCK + wave) * a.sh.rpw;
+    for (int r = 0; r < a.sh.rpw; ++r) {
+        const int row = r0 + r;
+        if (row >= a.sh.rows) break;
+        const long i = row * ld + b;
+        a.p1[i] = a.r1[i] * (double)a.invdiag[i] + be * a.p1[i];
+    }
+}
+
 // x = clip(xs + Dc * fmask * A^T w_p, l, u) (original units).
 struct PolishFinalColsArgs {
     Shape sh;
