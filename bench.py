@@ -801,7 +801,10 @@ def main():
                   "frac": round((pdhg_step_bytes + cg_step_bytes) / step_s / 1e9 / HBM_PEAK_GBS, 4),
                   "pdhg_only_frac": round(pdhg_step_bytes / step_s / 1e9 / HBM_PEAK_GBS, 4),
                   "note": "PDHG iterations and polish CG steps over the timed wall time per step; checks, "
-                          "polish set-up, certificates and start scoring are not counted"}
+                          "polish set-up, certificates and start scoring are not counted; every CG LP-step is "
+                          "counted at the two-system passes' bytes, although the steps after every dual system "
+                          "stopped run the primal-only passes (about half the bytes), so cg_bytes and frac are "
+                          "upper bounds"}
 
     # plain form on the same LPs: FP64 restarted PDHG, cold, no polish
     plain_idx = []
