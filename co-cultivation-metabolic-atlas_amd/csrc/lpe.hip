@@ -619,34 +619,46 @@ int launch_cg_rows(lpe_handle* h, const CgRowsArgs& a, int ldv) {
     return b.sh.n_rb;
 }
 
-// Primal-only CG passes (one LP per lane, after every dual system stopped).
-template <int D>
-void launch_cg_cols_p_w(lpe_handle* h, const CgColsPArgs& b, int grid, int waves) {
-    if (waves == 16) hipLaunchKernelGGL((k_cg_cols_p<D, 16>), dim3(grid), dim3(16 * WAVE), 0, h->s, b);
-    else hipLaunchKernelGGL((k_cg_cols_p<D, 4>), dim3(grid), dim3(4 * WAVE), 0, h->s, b);
+// Primal-only CG passes (after every dual system stopped), one LP per lane.
+// LPE_CG_PLPL=2 runs two LPs per lane (16-byte gathers and streams, as the
+// interleaved two-system vectors have), for sweeps: measured at 4096 LPs it
+// was not faster (117.4k against 121.3k certified LPs/s, mean of three).
+int cg_plpl(int ld) {
+    const int k = env_choice("LPE_CG_PLPL", 1, 2);
+    return (k == 2 && ld >= 128) ? 2 : 1;
 }
 
-template <int D>
+template <int LPL, int D>
+void launch_cg_cols_p_w(lpe_handle* h, const CgColsPArgs& b, int grid, int waves) {
+    if (waves == 16) hipLaunchKernelGGL((k_cg_cols_p<LPL, D, 16>), dim3(grid), dim3(16 * WAVE), 0, h->s, b);
+    else hipLaunchKernelGGL((k_cg_cols_p<LPL, D, 4>), dim3(grid), dim3(4 * WAVE), 0, h->s, b);
+}
+
+template <int LPL, int D>
 void launch_cg_rows_p_w(lpe_handle* h, const CgRowsPArgs& b, int grid, int waves) {
-    if (waves == 16) hipLaunchKernelGGL((k_cg_rows_p<D, 16>), dim3(grid), dim3(16 * WAVE), 0, h->s, b);
-    else hipLaunchKernelGGL((k_cg_rows_p<D, 4>), dim3(grid), dim3(4 * WAVE), 0, h->s, b);
+    if (waves == 16) hipLaunchKernelGGL((k_cg_rows_p<LPL, D, 16>), dim3(grid), dim3(16 * WAVE), 0, h->s, b);
+    else hipLaunchKernelGGL((k_cg_rows_p<LPL, D, 4>), dim3(grid), dim3(4 * WAVE), 0, h->s, b);
 }
 
 int launch_cg_cols_p(lpe_handle* h, const CgColsPArgs& a, int ldv) {
     int grid;
     CgColsPArgs b = a;
-    const int waves = cg_waves();
-    b.sh = make_shape_waves(a.sh.rows, ldv, ldv, cg_rpw(ldv), 1, waves, &grid);
-    if (cg_gather_depth() == 8) launch_cg_cols_p_w<8>(h, b, grid, waves); else launch_cg_cols_p_w<4>(h, b, grid, waves);
+    const int waves = cg_waves(), lpl = cg_plpl(ldv);
+    b.sh = make_shape_waves(a.sh.rows, ldv, ldv, cg_rpw(ldv), lpl, waves, &grid);
+    const bool d8 = cg_gather_depth() == 8;
+    if (lpl == 2) { if (d8) launch_cg_cols_p_w<2, 8>(h, b, grid, waves); else launch_cg_cols_p_w<2, 4>(h, b, grid, waves); }
+    else { if (d8) launch_cg_cols_p_w<1, 8>(h, b, grid, waves); else launch_cg_cols_p_w<1, 4>(h, b, grid, waves); }
     return b.sh.n_rb;
 }
 
 int launch_cg_rows_p(lpe_handle* h, const CgRowsPArgs& a, int ldv) {
     int grid;
     CgRowsPArgs b = a;
-    const int waves = cg_waves();
-    b.sh = make_shape_waves(a.sh.rows, ldv, ldv, cg_rpw(ldv), 1, waves, &grid);
-    if (cg_gather_depth() == 8) launch_cg_rows_p_w<8>(h, b, grid, waves); else launch_cg_rows_p_w<4>(h, b, grid, waves);
+    const int waves = cg_waves(), lpl = cg_plpl(ldv);
+    b.sh = make_shape_waves(a.sh.rows, ldv, ldv, cg_rpw(ldv), lpl, waves, &grid);
+    const bool d8 = cg_gather_depth() == 8;
+    if (lpl == 2) { if (d8) launch_cg_rows_p_w<2, 8>(h, b, grid, waves); else launch_cg_rows_p_w<2, 4>(h, b, grid, waves); }
+    else { if (d8) launch_cg_rows_p_w<1, 8>(h, b, grid, waves); else launch_cg_rows_p_w<1, 4>(h, b, grid, waves); }
     return b.sh.n_rb;
 }
 
@@ -1094,7 +1106,8 @@ int cg_block(lpe_handle* h, const View& v, const Shape& sr, const Shape& sc, int
                           ((long long)(cg_gather_depth() == 8 ? 1 : 0) << 53) |
                           ((long long)(cg_waves() == 16 ? 1 : 0) << 54) | ((long long)(cg_rpw(v.ld) - 1) << 55) |
                           ((long long)rows_reverse() << 60) | ((long long)dir_reverse() << 61) |
-                          ((long long)(cg_primal_inf() ? 1 : 0) << 51) | ((long long)(primal_only ? 1 : 0) << 62);
+                          ((long long)(cg_primal_inf() ? 1 : 0) << 51) | ((long long)(primal_only ? 1 : 0) << 62) |
+                          ((long long)(primal_only && cg_plpl(v.ld) == 2 ? 1 : 0) << 59);
     hipGraphExec_t& g_exec = h->cg_graphs[key];
     if (!g_exec) {
         hipGraph_t g;

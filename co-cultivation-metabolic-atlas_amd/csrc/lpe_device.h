@@ -1800,34 +1800,47 @@ struct CgColsPArgs {
     double* partial;         // [n_rb][2][ld]: |q1|^2, 0
 };
 
-template <int D, int WPB>
+template <int LPL, int D, int WPB>
 __global__ void __launch_bounds__(WPB * WAVE) k_cg_cols_p(CgColsPArgs a) {
-    __shared__ double lds[WPB * 2 * WAVE];
+    __shared__ double lds[WPB * 2 * WAVE * LPL];
     int tile, rb;
     if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    const int b0 = tile * WAVE + lane;
+    const int b0 = tile * WAVE * LPL + lane * LPL;
     const bool valid = b0 < a.sh.ld;
     const long ld = a.sh.ld;
-    double v[2][1] = {{0.0}, {0.0}};
-    const bool on = valid && a.sel[b0] != 0 && a.act[b0];
-    if (on) {
+    double v[2][LPL];
+    bool on[LPL];
+    bool any = false;
+#pragma unroll
+    for (int t = 0; t < LPL; ++t) {
+        v[0][t] = 0.0; v[1][t] = 0.0;
+        on[t] = valid && a.sel[b0 + t] != 0 && a.act[b0 + t];
+        any = any || on[t];
+    }
+    if (any) {
         const int r0 = (rb * WPB + wave) * a.sh.rpw;
         for (int r = 0; r < a.sh.rpw; ++r) {
             const int col = uniform(r0 + r);
             if (col >= a.sh.rows) break;
             const long i = col * ld + b0;
-            const double f = a.fmask[i] ? 1.0 : 0.0;
-            double s1[1];
-            row_dot<double, double, double, 1, D>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.p1, ld, b0,
-                                                  col, s1);
-            const double q = f * s1[0];
-            v[0][0] += q * q;
-            a.q1[i] = q;
+            double f[LPL];
+#pragma unroll
+            for (int t = 0; t < LPL; ++t) f[t] = on[t] && a.fmask[i + t] ? 1.0 : 0.0;
+            double s1[LPL];
+            row_dot<double, double, double, LPL, D>(a.AT.ptr, a.AT.idx, a.AT.slot, a.AT.val, a.AT.var, a.p1, ld, b0,
+                                                    col, s1);
+#pragma unroll
+            for (int t = 0; t < LPL; ++t) {
+                if (!on[t]) continue;
+                const double q = f[t] * s1[t];
+                v[0][t] += q * q;
+                a.q1[i + t] = q;
+            }
         }
     }
     const int ops[2] = {RSUM, RSUM};
-    block_partials<2, 1, WPB>(v, ops, a.partial, rb, ld, b0, lds, valid);
+    block_partials<2, LPL, WPB>(v, ops, a.partial, rb, ld, b0, lds, valid);
 }
 
 struct CgRowsPArgs {
@@ -1846,43 +1859,62 @@ struct CgRowsPArgs {
     int reverse;
 };
 
-template <int D, int WPB>
+template <int LPL, int D, int WPB>
 __global__ void __launch_bounds__(WPB * WAVE) k_cg_rows_p(CgRowsPArgs a) {
-    __shared__ double lds[WPB * 3 * WAVE];
+    __shared__ double lds[WPB * 3 * WAVE * LPL];
     int tile, rb;
     if (!map_block(a.sh.n_real, a.sh.n_rb, tile, rb)) return;
     if (a.reverse) tile = a.sh.n_tiles - 1 - tile;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    const int b0 = tile * WAVE + lane;
+    const int b0 = tile * WAVE * LPL + lane * LPL;
     const bool valid = b0 < a.sh.ld;
     const long ld = a.sh.ld;
-    double v[3][1] = {{0.0}, {0.0}, {0.0}};
-    const bool on1 = valid && a.sel[b0] != 0 && a.act[b0];
-    const double al = on1 ? a.alpha[b0] : 0.0;
-    if (on1) {
+    double v[3][LPL];
+    double al[LPL];
+    bool on1[LPL];
+    bool any = false;
+#pragma unroll
+    for (int t = 0; t < LPL; ++t) {
+        v[0][t] = 0.0; v[1][t] = 0.0; v[2][t] = 0.0;
+        on1[t] = valid && a.sel[b0 + t] != 0 && a.act[b0 + t];
+        al[t] = on1[t] ? a.alpha[b0 + t] : 0.0;
+        any = any || on1[t];
+    }
+    if (any) {
         const int r0 = (rb * WPB + wave) * a.sh.rpw;
         for (int r = 0; r < a.sh.rpw; ++r) {
             const int row = uniform(r0 + r);
             if (row >= a.sh.rows) break;
             const long i = row * ld + b0;
             const double drow = a.dr ? a.dr[row] : 1.0;
-            const double iv = (double)a.invdiag[i];
-            const double on = iv != 0.0 ? 1.0 : 0.0;
-            const double pv = a.p1[i], wv = a.w1[i], rv = a.r1[i];
-            double t1[1];
-            row_dot<double, double, double, 1, D>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.q1, ld, b0, row,
-                                                  t1);
-            a.w1[i] = wv + al * pv;
-            const double n1 = rv - al * (on * t1[0]);
-            a.r1[i] = n1;
-            v[0][0] += n1 * n1 * iv;
-            const double u = a.dr ? n1 / drow : n1;
-            if (a.inf_norm) v[2][0] = fmax(v[2][0], u * u);
-            else v[2][0] += u * u;
+            // the row's streamed operands are loaded before the gathers
+            double iv[LPL], pv[LPL], wv[LPL], rv[LPL];
+#pragma unroll
+            for (int t = 0; t < LPL; ++t) {
+                iv[t] = 0.0; pv[t] = 0.0; wv[t] = 0.0; rv[t] = 0.0;
+                if (!on1[t]) continue;
+                iv[t] = (double)a.invdiag[i + t];
+                pv[t] = a.p1[i + t]; wv[t] = a.w1[i + t]; rv[t] = a.r1[i + t];
+            }
+            double t1[LPL];
+            row_dot<double, double, double, LPL, D>(a.A.ptr, a.A.idx, a.A.slot, a.A.val, a.A.var, a.q1, ld, b0, row,
+                                                    t1);
+#pragma unroll
+            for (int t = 0; t < LPL; ++t) {
+                if (!on1[t]) continue;
+                const double on = iv[t] != 0.0 ? 1.0 : 0.0;
+                a.w1[i + t] = wv[t] + al[t] * pv[t];
+                const double n1 = rv[t] - al[t] * (on * t1[t]);
+                a.r1[i + t] = n1;
+                v[0][t] += n1 * n1 * iv[t];
+                const double u = a.dr ? n1 / drow : n1;
+                if (a.inf_norm) v[2][t] = fmax(v[2][t], u * u);
+                else v[2][t] += u * u;
+            }
         }
     }
     const int ops[3] = {RSUM, RSUM, a.inf_norm ? RMAX : RSUM};
-    block_partials<3, 1, WPB>(v, ops, a.partial, rb, ld, b0, lds, valid);
+    block_partials<3, LPL, WPB>(v, ops, a.partial, rb, ld, b0, lds, valid);
 }
 
 struct CgDirPArgs {
