@@ -1179,7 +1179,15 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
         // LPs whose round-1 answer missed a gate by little: its dual floor is
         // 100x lower and its primal stop 10x lower, since round 1's answer
         // already meets round 1's stops and the CG would otherwise end at once
-        const double dscale = orig ? 1e-2 : 1.0, pscale2 = orig ? 1e-2 : 1.0;
+        // The first round's dual floor is 5x looser on FP32 engines (polished
+        // from FP32 iterates; measured on the 4096-LP consortium rollout with
+        // factors 1 / 2 / 3 / 5 / 10: 123k / 126k / 127k / 133k / 131k certified
+        // LPs/s, 10 sending many LPs of the first steps to the second round)
+        // than on FP64 engines (the cooperative stages' FP64 parsimonious-
+        // exchange polishes fail more often with the looser floor: 45 against
+        // 24 attempts in the cold step).
+        const bool f32_engine = h->method != LPE_METHOD_FIXED && h->precision == LPE_PREC_FP32;
+        const double dscale = orig ? 1e-2 : (f32_engine ? 5.0 : 1.0), pscale2 = orig ? 1e-2 : 1.0;
         hipLaunchKernelGGL(k_cg_floor, dim3(ceil_div(v.ld, 256)), dim3(256), 0, h->s, (const double*)h->d_bnorm,
                            (const double*)h->d_cnorm, v.sel == h->d_sel ? (const int*)nullptr : (const int*)h->d_perm,
                            h->d_cgfloor, v.ld, dscale, cg_ptol2_h(h), pscale2);

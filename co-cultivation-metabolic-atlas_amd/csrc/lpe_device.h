@@ -2329,12 +2329,11 @@ __global__ void k_convert(const TI* __restrict__ in, TO* __restrict__ out, long 
 }
 
 // CG stopping floors: rho = r.D^-1 r below (1e-13 (1 + |b^|))^2 (primal,
-// rounding level) or (1e-12 (1 + |c^|))^2 (dual; reduced costs far below
-// what the 1e-7 relative gap gate resolves).
+// rounding level) or (f (1 + |c^|))^2 (dual least squares: reduced costs
+// small enough for the 1e-7 relative gap gate on nearly every LP; the few
+// that miss it by little are refined by the polish's second round, whose
+// dual floor is 100x lower).  f = LPE_CG_FLOOR_D times the caller's dscale.
 // perm maps the view's columns to LPs (nullptr: identity).
-// Absolute stop of the two CG systems: rho <= (f (1 + |b^|))^2 (primal) and
-// (f (1 + |c^|))^2 (dual least squares), below which a system has reached the
-// rounding level of its right-hand side.
 #ifndef LPE_CG_FLOOR_P
 #define LPE_CG_FLOOR_P 1e-13
 #endif
