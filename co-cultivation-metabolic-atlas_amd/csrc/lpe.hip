@@ -1203,8 +1203,10 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
         // The dual system stops before the primal one; once no LP has an
         // active dual system the remaining steps run the primal-only passes on
         // a contiguous p1 (half the gathered and streamed vector bytes).  After
-        // the first two blocks the blocks are half as long until that switch,
-        // so that it comes sooner.
+        // the first block the blocks are half as long until that switch, so
+        // that it comes sooner (4096 consortium LPs: the dual stops after about
+        // 13 steps; half blocks from step 10 on: 143.9k against 138.5k
+        // certified LPs/s, from step 20 on as before).
         const bool can_split = cg_lpl(v.ld) == 1 && h->m > 0 && cg_split_enabled();
         bool split = false;
         int used = 0;
@@ -1233,7 +1235,7 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
                     split = true;
                 }
             }
-            const int steps = (can_split && !split && used >= 2 * block_steps && block_steps >= 2) ? block_steps / 2
+            const int steps = (can_split && !split && used >= block_steps && block_steps >= 2) ? block_steps / 2
                                                                                                   : block_steps;
             CHK(cg_block(h, v, sr, sc, grid_r, grid_c, steps, split));
             used += steps;
