@@ -1961,15 +1961,18 @@ int lpe_update(lpe_handle* h, const lpe_vectors* v, void* stream) {
         r.sh = make_shape((int)n, (int)ld, h->B, 1, 1, &grid); r.A = op_orig(h, true); r.out = h->d_tauref; r.divide = 1;
         hipLaunchKernelGGL(k_ref_metric, dim3(grid), dim3(BLOCK), 0, h->s, r);
     } else if (h->nvar > 0) {
-        // shared metric from the largest per-LP magnitudes of the batch (reduced
-        // on the device: only nvar values cross to the host)
+        // shared metric from the largest per-LP magnitudes of the batch, all on
+        // the device (no host round trip; the host's sig / tau keep the values
+        // of the create call and lpe_get_scaling reads the device's)
         hipLaunchKernelGGL(k_var_absmax, dim3(h->nvar), tb, 0, h->s, (const double*)h->d_var_s, h->d_var_absmax,
                            (int)B, (int)ld);
-        HIPCHK(hipMemcpyAsync(h->var_absmax.data(), h->d_var_absmax, sizeof(double) * h->nvar,
-                              hipMemcpyDeviceToHost, h->s));
-        HIPCHK(hipStreamSynchronize(h->s));
-        shared_metric(h);
-        CHK(upload_metric(h));
+        hipLaunchKernelGGL(k_shared_metric, dim3(ceil_div(m + n, 256)), dim3(256), 0, h->s, (const int*)h->d_rp,
+                           (const int*)h->d_ci, (const int*)h->d_rslot, (const double*)h->d_val_o,
+                           (const int*)h->d_cp, (const int*)h->d_ri, (const int*)h->d_cslot,
+                           (const double*)h->d_tval_o, (const double*)h->d_dr, (const double*)h->d_dc,
+                           (const double*)h->d_var_absmax, h->eta, (int)m, (int)n, h->d_sig, h->d_tau, h->d_sig_f,
+                           h->d_tau_f);
+        HIPCHK(hipGetLastError());
     }
     launch_norms(h, (const double*)h->d_rhs_s, (int)m, 0, h->d_bnorm);
     launch_norms(h, (const double*)h->d_c_s, (int)n, h->shared_c, h->d_cnorm);
@@ -2367,8 +2370,11 @@ int lpe_get_scaling(lpe_handle* h, double* row_scale, double* col_scale, double*
         }
         return LPE_OK;
     }
-    if (sigma) std::copy(h->sig.begin(), h->sig.end(), sigma);
-    if (tau) std::copy(h->tau.begin(), h->tau.end(), tau);
+    // the shared metric lives on the device (an update with per-LP
+    // coefficients recomputes it there)
+    HIPCHK(hipStreamSynchronize(h->s));
+    if (sigma && h->m > 0) HIPCHK(hipMemcpy(sigma, h->d_sig, sizeof(double) * h->m, hipMemcpyDeviceToHost));
+    if (tau) HIPCHK(hipMemcpy(tau, h->d_tau, sizeof(double) * h->n, hipMemcpyDeviceToHost));
     return LPE_OK;
 }
 

@@ -2157,6 +2157,43 @@ __global__ void k_var_scale(const double* __restrict__ in, TO* __restrict__ out,
     out[(long)s * ld + b] = (TO)(v * dr[vrow[s]] * dc[vcol[s]]);
 }
 
+// Shared PDHG metric from the batch's largest per-LP coefficient magnitudes
+// (after k_var_absmax): sigma_i = eta / sum_k |a^_ik| over row i, tau_j =
+// eta / sum_k |a^_kj| over column j, an entry with a per-LP value counting
+// var_absmax[slot]; 1 where the sum is 0.  Rows sum in CSR order, columns in
+// CSC order (ascending row), each term |a_k dr_i dc_j| formed as on the host,
+// so the values equal the host restatement's bit for bit.  Thread t < m: row
+// t; m <= t < m + n: column t - m.
+__global__ void k_shared_metric(const int* __restrict__ rp, const int* __restrict__ ci, const int* __restrict__ rslot,
+                                const double* __restrict__ val, const int* __restrict__ cp,
+                                const int* __restrict__ ri, const int* __restrict__ cslot,
+                                const double* __restrict__ tval, const double* __restrict__ dr,
+                                const double* __restrict__ dc, const double* __restrict__ var_absmax, double eta,
+                                int m, int n, double* __restrict__ sig, double* __restrict__ tau,
+                                float* __restrict__ sig_f, float* __restrict__ tau_f) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < m) {
+        double rs = 0.0;
+        for (int k = rp[t]; k < rp[t + 1]; ++k) {
+            const int sl = rslot[k];
+            rs += sl < 0 ? fabs(val[k] * dr[t] * dc[ci[k]]) : var_absmax[sl];
+        }
+        const double v = rs > 0.0 ? eta / rs : 1.0;
+        sig[t] = v;
+        sig_f[t] = (float)v;
+    } else if (t < m + n) {
+        const int j = t - m;
+        double cs = 0.0;
+        for (int k = cp[j]; k < cp[j + 1]; ++k) {
+            const int sl = cslot[k];
+            cs += sl < 0 ? fabs(tval[k] * dr[ri[k]] * dc[j]) : var_absmax[sl];
+        }
+        const double v = cs > 0.0 ? eta / cs : 1.0;
+        tau[j] = v;
+        tau_f[j] = (float)v;
+    }
+}
+
 // out[s] = max over the real LPs b < batch of |var[s][b]| (one block per slot;
 // the maximum does not depend on the order).
 __global__ void __launch_bounds__(256) k_var_absmax(const double* __restrict__ var, double* __restrict__ out,
