@@ -607,6 +607,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--max-iterations", type=int, default=30000)
     ap.add_argument("--cg-iterations", type=int, default=200, help="CG steps per polish round")
+    ap.add_argument("--check-interval", type=int, default=0,
+                    help="PDHG iterations between device checks of the timed rollout (0: the autotuned chunk)")
     ap.add_argument("--polish-kkt", type=float, default=1e-4,
                     help="relative KKT error at which the target form tries its active-set polish")
     ap.add_argument("--fp32-iterations", type=int, default=6000,
@@ -688,6 +690,8 @@ def main():
         eng.update(model.c, rhs, lower, upper, var_values=coupling)
         tuning = autotune(eng)
     del warm_ro
+    if args.check_interval > 0:
+        tuning = dict(tuning, chunk=args.check_interval)
 
     # the timed rollout: steps 0..K-1 from the initial conditions
     n_check = max(1, int(round(args.cpu_sample * B)))
