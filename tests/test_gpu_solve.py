@@ -306,3 +306,31 @@ def test_status_equals_a_fresh_certificate_after_whole_batch_polish(device):
             assert (status[b] == 1) == bool(fresh[b, 4]) == host["certificate_passed"], (far, b)
             assert cert.relative_objective_error(host["objective"], refs[b].fun) <= REL_OBJ_TOL
         eng.close()
+
+
+def test_primal_only_cg_passes_give_the_same_answers(device, monkeypatch):
+    """Once every dual system has stopped, the polish's remaining CG steps run
+    the primal-only passes on a contiguous copy of p1.  They carry the same
+    arithmetic as the primal half of the two-system passes, and each LP's CG
+    stops at the same step whatever the block lengths, so the answers are
+    bit-identical to the two-system passes (LPE_CG_SPLIT=0)."""
+    from lpe.engine import SolveOptions
+    model = workloads.consortium(seed=4, shapes=((120, 180), (150, 220), (90, 130)), n_medium=20)
+    cb = workloads.consortium_batch(batch=96, seed=4, model=model)
+    md = cb.model
+    answers = []
+    for split in ("0", "1"):
+        monkeypatch.setenv("LPE_CG_SPLIT", split)
+        eng = _engine(md.A, md.neq, cb.batch, precision="fp32", shared_c=True,
+                      var_positions=md.coupling_nnz.ravel())
+        eng.update(md.c, cb.rhs(), cb.lower(), cb.upper(), var_values=cb.coupling_values().reshape(cb.batch, -1))
+        eng.reset()
+        info = eng.solve(SolveOptions(max_iterations=4000, check_interval=64, polish=True))
+        x, y, status = eng.state()
+        answers.append((x.cpu().numpy(), y.cpu().numpy(), status.copy(), info["cg_steps"]))
+        eng.close()
+    (x0, y0, s0, c0), (x1, y1, s1, c1) = answers
+    assert (s0 == s1).all()
+    assert int((s1 == 1).sum()) >= cb.batch // 2, s1
+    np.testing.assert_array_equal(x0, x1)
+    np.testing.assert_array_equal(y0, y1)
