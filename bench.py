@@ -874,6 +874,12 @@ def main():
                      "algorithmic_bytes_per_launch": kernels[dominant]["algorithmic_bytes_per_launch"],
                      "avg_launch_ms": kernels[dominant]["avg_launch_ms"],
                      "step_kernels": kernels,
+                     "step_kernels_note": "all-LPs-active launch times (hipEvents on the engine stream); "
+                                          "full_launches_per_step counts every CG LP-step as a two-system "
+                                          "launch, while the steps after every dual system stopped run the "
+                                          "primal-only passes (k_cg_*_p, about half the bytes), so the CG "
+                                          "ms_per_step figures are upper bounds; the kernel trace of the "
+                                          "timed steps (profiles/) attributes the time launch by launch",
                      "primal_fp64_avg_launch_ms": None if ms_primal_fp64 is None else round(ms_primal_fp64, 5),
                      "whole_step": whole_step},
     }
