@@ -171,6 +171,7 @@ struct lpe_handle {
     long cg_lp_steps_total = 0;  // sum over CG blocks of (LPs with an active system) x block steps
     double cg_ptol2 = 0.0;    // primal CG stop: |Dr (b - A x)|_2^2 <= (1e-8 min(dr))^2
     double cg_gate = 1e-5;    // the certificate's primal gate of the polish in progress
+    bool cg_at_start = false; // the polish in progress is of the starting point (no PDHG iteration yet)
     double *d_xpol = nullptr, *d_ypol = nullptr, *d_xres = nullptr, *d_yres = nullptr;
     double* d_metrics = nullptr;
     double *d_xin = nullptr, *d_yin = nullptr;    // user iterates for lpe_certify / lpe_spmv
@@ -1188,13 +1189,16 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
         // the first steps need the second round) than on FP64 engines (the
         // cooperative stages' FP64 parsimonious-exchange polishes fail more
         // often with a looser floor: 45 against 24 attempts in the cold step).
+        // A polish after PDHG iterations (the transient first rollout steps,
+        // whose iterates are further from the optimum) keeps 5x: at 8x many of
+        // its LPs needed the second round (156k against 150k certified LPs/s).
         // The same holds for the first round's primal stop: 8e-6 instead of
         // 3e-6 on FP32 engines (at the reference's 1e-5 primal gate; an answer
         // whose residual ends above the gate after the clip to the bounds is
         // refined by the second round at a 10x lower stop): 131k -> 140k
         // certified LPs/s (5e-6: 137k) on the same box.
         const bool f32_engine = h->method != LPE_METHOD_FIXED && h->precision == LPE_PREC_FP32;
-        const double dscale = orig ? 1e-2 : (f32_engine ? 8.0 : 1.0);
+        const double dscale = orig ? 1e-2 : (f32_engine ? (h->cg_at_start ? 8.0 : 5.0) : 1.0);
         const double pscale2 = orig ? 1e-2 : (f32_engine ? (8.0 / 3.0) * (8.0 / 3.0) : 1.0);
         hipLaunchKernelGGL(k_cg_floor, dim3(ceil_div(v.ld, 256)), dim3(256), 0, h->s, (const double*)h->d_bnorm,
                            (const double*)h->d_cnorm, v.sel == h->d_sel ? (const int*)nullptr : (const int*)h->d_perm,
@@ -2163,7 +2167,9 @@ int lpe_solve(lpe_handle* h, const lpe_solve_opts* o, lpe_solve_info* info, void
         last_polish = iters;
         HIPCHK(hipEventRecord(e0, h->s));
         std::vector<int> selected(h->h_sel, h->h_sel + h->ld);
+        h->cg_at_start = k == 0;
         CHK(polish_selected(h, cg, 1e-6, ep, ed, eg));
+        h->cg_at_start = false;
         HIPCHK(hipEventRecord(e1, h->s));
         HIPCHK(hipEventSynchronize(e1));
         HIPCHK(hipEventElapsedTime(&ms, e0, e1));
