@@ -1179,8 +1179,8 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
         // a second round (refinement from round 1's FP64 answer, orig) runs for
         // LPs whose round-1 answer missed a gate by little: its dual floor is
         // 100x lower and its primal stop 10x lower, since round 1's answer
-        // already meets round 1's stops and the CG would otherwise end at once
-        // The first round's dual floor is 8x looser on FP32 engines (polished
+        // already meets round 1's stops and the CG would otherwise end at once.
+        // The first round's dual floor is 12x looser on FP32 engines (polished
         // from FP32 iterates; measured on the 4096-LP consortium rollout with
         // factors 1 / 2 / 3 / 5 / 10: 123k / 126k / 127k / 133k / 131k certified
         // LPs/s, and once the primal-only passes ran after the dual stopped,
@@ -1189,16 +1189,20 @@ int polish_t(lpe_handle* h, const View& v, const int* hsel, const T* xin, const 
         // the first steps need the second round) than on FP64 engines (the
         // cooperative stages' FP64 parsimonious-exchange polishes fail more
         // often with a looser floor: 45 against 24 attempts in the cold step).
-        // A polish after PDHG iterations (the transient first rollout steps,
-        // whose iterates are further from the optimum) keeps 5x: at 8x many of
-        // its LPs needed the second round (156k against 150k certified LPs/s).
+        // That holds for a polish of the starting point (the steady rollout
+        // steps; 8 / 10 / 12 / 16 / 24: 151k / 154k / 157k / 152k / 153k
+        // certified LPs/s, means of two runs, each against 12x on its box).
+        // A polish after PDHG iterations
+        // (the transient first rollout steps, whose iterates are further from
+        // the optimum) keeps 5x: at 8x many of its LPs needed the second round
+        // (156k against 150k certified LPs/s).
         // The same holds for the first round's primal stop: 8e-6 instead of
         // 3e-6 on FP32 engines (at the reference's 1e-5 primal gate; an answer
         // whose residual ends above the gate after the clip to the bounds is
         // refined by the second round at a 10x lower stop): 131k -> 140k
         // certified LPs/s (5e-6: 137k) on the same box.
         const bool f32_engine = h->method != LPE_METHOD_FIXED && h->precision == LPE_PREC_FP32;
-        const double dscale = orig ? 1e-2 : (f32_engine ? (h->cg_at_start ? 8.0 : 5.0) : 1.0);
+        const double dscale = orig ? 1e-2 : (f32_engine ? (h->cg_at_start ? 12.0 : 5.0) : 1.0);
         const double pscale2 = orig ? 1e-2 : (f32_engine ? (8.0 / 3.0) * (8.0 / 3.0) : 1.0);
         hipLaunchKernelGGL(k_cg_floor, dim3(ceil_div(v.ld, 256)), dim3(256), 0, h->s, (const double*)h->d_bnorm,
                            (const double*)h->d_cnorm, v.sel == h->d_sel ? (const int*)nullptr : (const int*)h->d_perm,
